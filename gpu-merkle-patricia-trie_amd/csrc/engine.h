@@ -1,0 +1,114 @@
+// engine.h -- host runtime: a forest of tries resident in HBM.
+//
+// One Forest owns the node pools, the key arena, value chunks and the tries'
+// headers on one device, and runs every operation on its own HIP stream.  A
+// single trie (the reference's GpuMPT::Compress::MPT, gpu_mpt.cuh:18-192) is a
+// forest of one; batched and served workloads use one forest for many tries.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace gmpt {
+
+struct BatchDesc {
+  const uint8_t *keys_hexs = nullptr;
+  const int32_t *keys_indexs = nullptr;    // inclusive [start, end] pairs
+  const uint8_t *values_bytes = nullptr;
+  const int64_t *values_indexs = nullptr;  // inclusive [start, end] pairs
+  const uint64_t *values_hps = nullptr;    // caller's host pointers (nullable)
+  const int32_t *trie_ids = nullptr;       // nullable: all keys go to trie 0
+  uint32_t n = 0;
+  uint64_t keys_bytes = 0;                 // totals (computed from the indexs for host batches)
+  uint64_t values_total = 0;
+  bool on_device = false;                  // all pointers above are device pointers
+  bool adopt_values = false;               // device batch: nodes point into values_bytes (caller keeps it alive)
+};
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+struct OpStats {
+  double last_hash_ms = 0;    // device time of the hash kernels of the last hash()
+  uint32_t hash_launches = 0;
+  uint32_t hashed_nodes = 0;
+  uint32_t restarts = 0;      // LockNU optimistic restarts (last op)
+  uint32_t lock_fails = 0;    // LockNU failed latch upgrades (last op)
+  uint32_t fallback_sorts = 0;
+  uint32_t regroups = 0;       // PhaseNU group regroupings (split groups around a subtree)
+  uint64_t perms = 0;          // Keccak-f permutations of the last hash
+  uint64_t enc_bytes = 0;      // node encoding bytes of the last hash
+};
+
+class Forest {
+ public:
+  Forest(int device, uint32_t ntries);
+  ~Forest();
+  bool ok() const { return init_ok_; }
+
+  // PhaseNU (sort-based) and LockNU (latch-coupled) batch inserts.  On success
+  // *targets / *n_targets name the dirty list to pass to hash().
+  int puts_2phase(const BatchDesc &b, void **targets, int *n_targets);
+  int puts_latching(const BatchDesc &b, void **targets, int *n_targets);
+  // PhaseHC over the dirty list, then every trie's root hash is refreshed.
+  int hash(void *targets, int n_targets);
+  int root_hash(uint32_t trie, uint8_t *out, int *size);
+  // get: value size (-1 absent) and the caller's host pointer per key.
+  int gets(const BatchDesc &keys, uint64_t *hps, int32_t *sizes);
+  // get with value bytes copied out (offsets per key into out).
+  int get_values(const BatchDesc &keys, uint8_t *out, int64_t out_cap, int64_t *offsets, int32_t *sizes);
+  // Every node on the paths of the given keys: 32-byte hash slots and RLP
+  // encodings with inclusive [start, end] index pairs (reference nodeset).
+  int export_nodes(const BatchDesc &keys, uint8_t **hashes, uint8_t **encs, uint64_t **enc_indexs,
+                   uint64_t *num);
+  int reset_trie(uint32_t trie);
+  // Empties every trie and recycles all pool memory (no reallocation).
+  int clear();
+  int synchronize();
+  uint32_t ntries() const { return ntries_; }
+  uint64_t pool_bytes_used() const;
+  uint64_t pool_bytes_reserved() const;
+  const OpStats &stats() const { return stats_; }
+  hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
+  Pools pools() const;
+
+ private:
+  int set_device();
+  int ensure_pools(uint64_t nb, uint64_t ns, uint64_t nv, uint64_t key_bytes);
+  int grow(DevBuf &b, size_t need, size_t keep);
+  int ensure_ws(DevBuf &b, size_t bytes);
+  int sync_counters();
+  int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
+  uint8_t *value_space(uint64_t bytes);
+
+  int device_ = 0;
+  uint32_t ntries_ = 0;
+  bool init_ok_ = false;
+  hipStream_t stream_ = nullptr;
+  DevBuf branch_, shrt_, value_, keys_, tries_, dirty_;
+  Counters *d_cnt_ = nullptr;
+  Counters *h_cnt_ = nullptr;  // pinned
+  uint32_t *h_scal_ = nullptr; // pinned
+  Counters cnt_{};             // host mirror after the last sync
+  uint64_t key_used_ = 0;
+  uint64_t lock_seq_ = 1;    // LockNU order stamps (PhaseNU values carry 0)
+  std::vector<DevBuf> vchunks_;
+  size_t vchunk_used_ = 0;
+  DevBuf st_keys_, st_kidx_, st_vidx_, st_hps_, st_tid_;
+  DevBuf ws_, ws2_;
+  DevBuf hc_sorted_, hc_hist_, hc_cursor_;
+  std::vector<uint32_t> h_hist_;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  OpStats stats_;
+};
+
+void set_error(const std::string &e);
+const char *last_error();
+
+}  // namespace gmpt

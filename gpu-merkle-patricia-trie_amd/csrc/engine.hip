@@ -1,0 +1,658 @@
+// engine.hip -- host runtime of the forest: pools, staging, PhaseNU / PhaseHC.
+#include "engine.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "get.h"
+#include "locknu.h"
+#include "phasehc.h"
+#include "phasenu.h"
+#include "prims.h"
+
+namespace gmpt {
+
+namespace {
+thread_local std::string g_err;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Carves aligned sub-buffers out of one workspace allocation.
+struct Carver {
+  uint8_t *base;
+  size_t off = 0;
+  explicit Carver(void *b) : base((uint8_t *)b) {}
+  template <typename T>
+  T *take(size_t count) {
+    off = align_up(off, 256);
+    T *p = (T *)(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+int st_levels_for(size_t ne) {
+  int k = 1;
+  while (((size_t)1 << k) < ne) k++;
+  return k + 1;
+}
+
+size_t nu_ws_bytes(size_t n, bool tries) {
+  size_t ne = 2 * n + 2;
+  size_t k = (size_t)st_levels_for(ne);
+  size_t b = 0;
+  b += n * (4 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
+  b += n * (4 * 7 + 1 + sizeof(Stop));
+  b += ne * (4 + 4 + 4 + 4) + k * ne;
+  b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
+  b += 64 * 256;  // alignment slack for ~40 carves
+  return b;
+}
+
+}  // namespace
+
+void set_error(const std::string &e) { g_err = e; }
+const char *last_error() { return g_err.c_str(); }
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e__ = (expr);                                                       \
+    if (e__ != hipSuccess) {                                                       \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e__));              \
+      return -1;                                                                   \
+    }                                                                              \
+  } while (0)
+
+Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
+  if (hipSetDevice(device) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    return;
+  }
+  if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return;
+  if (hipMalloc(&d_cnt_, sizeof(Counters)) != hipSuccess) return;
+  if (hipHostMalloc(&h_cnt_, sizeof(Counters)) != hipSuccess) return;
+  if (hipHostMalloc(&h_scal_, 64) != hipSuccess) return;
+  if (hipMemset(d_cnt_, 0, sizeof(Counters)) != hipSuccess) return;
+  if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return;
+  if (grow(tries_, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1), 0) != 0) return;
+  if (hipMemset(tries_.p, 0, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1)) != hipSuccess) return;
+  h_hist_.resize(kHcBins);
+  init_ok_ = true;
+}
+
+Forest::~Forest() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
+                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_hist_, &hc_cursor_};
+  for (DevBuf *b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (DevBuf &b : vchunks_)
+    if (b.p) (void)hipFree(b.p);
+  if (d_cnt_) (void)hipFree(d_cnt_);
+  if (h_cnt_) (void)hipHostFree(h_cnt_);
+  if (h_scal_) (void)hipHostFree(h_scal_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Forest::set_device() {
+  HIP_TRY(hipSetDevice(device_));
+  return 0;
+}
+
+Pools Forest::pools() const {
+  Pools P;
+  P.branch = (Branch *)branch_.p;
+  P.shrt = (Short *)shrt_.p;
+  P.value = (Value *)value_.p;
+  P.keys = (uint8_t *)keys_.p;
+  P.tries = (TrieHdr *)tries_.p;
+  P.dirty = (ref_t *)dirty_.p;
+  P.cnt = d_cnt_;
+  P.cap_branch = (uint32_t)std::min<size_t>(branch_.cap / sizeof(Branch), 0x3fffffffu);
+  P.cap_short = (uint32_t)std::min<size_t>(shrt_.cap / sizeof(Short), 0x3fffffffu);
+  P.cap_value = (uint32_t)std::min<size_t>(value_.cap / sizeof(Value), 0x3fffffffu);
+  P.cap_dirty = (uint32_t)std::min<size_t>(dirty_.cap / sizeof(ref_t), 0xffffffffu);
+  return P;
+}
+
+uint64_t Forest::pool_bytes_used() const {
+  uint64_t v = 0;
+  for (size_t i = 0; i < vchunks_.size(); i++) v += (i + 1 == vchunks_.size()) ? vchunk_used_ : vchunks_[i].cap;
+  return (uint64_t)cnt_.n_branch * sizeof(Branch) + (uint64_t)cnt_.n_short * sizeof(Short) +
+         (uint64_t)cnt_.n_value * sizeof(Value) + key_used_ + v;
+}
+
+uint64_t Forest::pool_bytes_reserved() const {
+  uint64_t v = 0;
+  for (const DevBuf &b : vchunks_) v += b.cap;
+  return branch_.cap + shrt_.cap + value_.cap + keys_.cap + dirty_.cap + v;
+}
+
+// Regrow a device buffer to at least `need` bytes, keeping the first `keep`.
+int Forest::grow(DevBuf &b, size_t need, size_t keep) {
+  if (b.cap >= need) return 0;
+  size_t cap = std::max(need, b.cap + b.cap / 2);
+  cap = align_up(cap, 4096);
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, cap);
+  if (e != hipSuccess) {
+    set_error(std::string("device pool allocation failed: ") + hipGetErrorString(e));
+    return -2;
+  }
+  if (b.p && keep) HIP_TRY(hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, stream_));
+  // fresh slots read as zero (empty children, unlocked, clean) even before a
+  // writer publishes them, which keeps optimistic LockNU readers in bounds
+  HIP_TRY(hipMemsetAsync((uint8_t *)p + keep, 0, cap - keep, stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  if (b.p) HIP_TRY(hipFree(b.p));
+  b.p = p;
+  b.cap = cap;
+  return 0;
+}
+
+int Forest::ensure_ws(DevBuf &b, size_t bytes) {
+  if (b.cap >= bytes) return 0;
+  if (b.p) {
+    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t cap = align_up(bytes + bytes / 4, 1 << 20);
+  hipError_t e = hipMalloc(&b.p, cap);
+  if (e != hipSuccess) {
+    set_error(std::string("workspace allocation failed: ") + hipGetErrorString(e));
+    return -2;
+  }
+  b.cap = cap;
+  return 0;
+}
+
+int Forest::ensure_pools(uint64_t nb, uint64_t ns, uint64_t nv, uint64_t key_bytes) {
+  int r;
+  if ((r = grow(branch_, (cnt_.n_branch + nb) * sizeof(Branch), cnt_.n_branch * sizeof(Branch)))) return r;
+  if ((r = grow(shrt_, (cnt_.n_short + ns) * sizeof(Short), cnt_.n_short * sizeof(Short)))) return r;
+  if ((r = grow(value_, (cnt_.n_value + nv) * sizeof(Value), cnt_.n_value * sizeof(Value)))) return r;
+  if ((r = grow(keys_, key_used_ + key_bytes + 64, key_used_))) return r;
+  // every branch / short node is listed at most once
+  size_t dcap = (branch_.cap / sizeof(Branch) + shrt_.cap / sizeof(Short)) * sizeof(ref_t);
+  if ((r = grow(dirty_, dcap, cnt_.n_dirty * sizeof(ref_t)))) return r;
+  if ((uint64_t)(cnt_.n_branch + nb) > 0x3fffffffull || (uint64_t)(cnt_.n_short + ns) > 0x3fffffffull ||
+      (uint64_t)(cnt_.n_value + nv) > 0x3fffffffull) {
+    set_error("node pool exceeds 2^30 nodes of one kind");
+    return -2;
+  }
+  return 0;
+}
+
+int Forest::sync_counters() {
+  HIP_TRY(hipMemcpyAsync(h_cnt_, d_cnt_, sizeof(Counters), hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  cnt_ = *h_cnt_;
+  if (cnt_.err != ERR_NONE) {
+    uint32_t code = cnt_.err, arg = cnt_.err_arg;
+    HIP_TRY(hipMemsetAsync(&d_cnt_->err, 0, 2 * sizeof(uint32_t), stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    const char *what = code == ERR_BAD_KEY ? "malformed key (needs hex nibbles < 16 and one terminator 16 at the end, "
+                                             "at most 254 nibbles) or trie id out of range"
+                       : code == ERR_POOL  ? "node pool exhausted"
+                                           : "internal error";
+    set_error(std::string(what) + " (code " + std::to_string(code) + ", at " + std::to_string(arg) + ")");
+    return code == ERR_BAD_KEY ? -3 : -4;
+  }
+  return 0;
+}
+
+uint8_t *Forest::value_space(uint64_t bytes) {
+  bytes = align_up(bytes + 8, 256);
+  if (vchunks_.empty() || vchunk_used_ + bytes > vchunks_.back().cap) {
+    DevBuf c;
+    c.cap = align_up(std::max<uint64_t>(bytes, 64ull << 20), 1 << 20);
+    if (hipMalloc(&c.p, c.cap) != hipSuccess) {
+      set_error("value arena allocation failed");
+      return nullptr;
+    }
+    vchunks_.push_back(c);
+    vchunk_used_ = 0;
+  }
+  uint8_t *p = (uint8_t *)vchunks_.back().p + vchunk_used_;
+  vchunk_used_ += bytes;
+  return p;
+}
+
+// Host batches are copied to device staging; values go into the value arena
+// (device batches too, unless the caller lets the trie adopt its buffer).
+int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool copy_values) {
+  *d = b;
+  uint32_t n = b.n;
+  if (!b.on_device) {
+    // extents of the host buffers: the largest end index of any pair (the
+    // kernels reject pairs outside them)
+    uint64_t kb = 0, vb = 0;
+    for (uint32_t i = 0; i < n; i++) {
+      int64_t e = b.keys_indexs[2 * (size_t)i + 1];
+      if (e >= 0 && (uint64_t)e + 1 > kb) kb = (uint64_t)e + 1;
+      if (b.values_indexs) {
+        int64_t ve = b.values_indexs[2 * (size_t)i + 1];
+        if (ve >= 0 && (uint64_t)ve + 1 > vb) vb = (uint64_t)ve + 1;
+      }
+    }
+    d->keys_bytes = kb;
+    d->values_total = vb;
+    int r;
+    if ((r = ensure_ws(st_keys_, kb + 64))) return r;
+    if ((r = ensure_ws(st_kidx_, 8ull * n + 64))) return r;
+    if ((r = ensure_ws(st_vidx_, 16ull * n + 64))) return r;
+    HIP_TRY(hipMemcpyAsync(st_keys_.p, b.keys_hexs, kb, hipMemcpyHostToDevice, stream_));
+    HIP_TRY(hipMemcpyAsync(st_kidx_.p, b.keys_indexs, 8ull * n, hipMemcpyHostToDevice, stream_));
+    d->keys_hexs = (const uint8_t *)st_keys_.p;
+    d->keys_indexs = (const int32_t *)st_kidx_.p;
+    if (b.values_indexs) {
+      HIP_TRY(hipMemcpyAsync(st_vidx_.p, b.values_indexs, 16ull * n, hipMemcpyHostToDevice, stream_));
+      d->values_indexs = (const int64_t *)st_vidx_.p;
+    }
+    if (b.values_hps) {
+      if ((r = ensure_ws(st_hps_, 8ull * n + 64))) return r;
+      HIP_TRY(hipMemcpyAsync(st_hps_.p, b.values_hps, 8ull * n, hipMemcpyHostToDevice, stream_));
+      d->values_hps = (const uint64_t *)st_hps_.p;
+    }
+    if (b.trie_ids) {
+      if ((r = ensure_ws(st_tid_, 4ull * n + 64))) return r;
+      HIP_TRY(hipMemcpyAsync(st_tid_.p, b.trie_ids, 4ull * n, hipMemcpyHostToDevice, stream_));
+      d->trie_ids = (const int32_t *)st_tid_.p;
+    }
+    d->on_device = true;
+    if (copy_values) {
+      uint8_t *vs = value_space(vb);
+      if (!vs) return -2;
+      if (vb) HIP_TRY(hipMemcpyAsync(vs, b.values_bytes, vb, hipMemcpyHostToDevice, stream_));
+      d->values_bytes = vs;
+    }
+  } else if (copy_values && !b.adopt_values) {
+    uint8_t *vs = value_space(b.values_total);
+    if (!vs) return -2;
+    if (b.values_total)
+      HIP_TRY(hipMemcpyAsync(vs, b.values_bytes, b.values_total, hipMemcpyDeviceToDevice, stream_));
+    d->values_bytes = vs;
+  }
+  *vbase = (uint64_t)(uintptr_t)d->values_bytes;
+  return 0;
+}
+
+int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
+  int r;
+  if ((r = set_device())) return r;
+  if (b.n > 0x3fffffffu) {
+    set_error("batch too large");
+    return -3;
+  }
+  if (b.on_device && b.n && (b.keys_bytes == 0)) {
+    set_error("device batch needs keys_bytes and values_total");
+    return -3;
+  }
+  stats_.fallback_sorts = 0;
+  stats_.regroups = 0;
+  if (b.n > 0) {
+    BatchDesc d;
+    uint64_t vbase = 0;
+    if ((r = stage_batch(b, &d, &vbase, true))) return r;
+    const uint32_t n = b.n;
+    if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, n, d.keys_bytes / 2 + 8ull * n))) return r;
+    const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
+    if ((r = ensure_ws(ws_, nu_ws_bytes(n, has_tid)))) return r;
+    Carver c(ws_.p);
+    NuArgs a;
+    memset(&a, 0, sizeof(a));
+    a.P = pools();
+    a.kh = d.keys_hexs;
+    a.kidx = d.keys_indexs;
+    a.vidx = d.values_indexs;
+    a.hps = d.values_hps;
+    a.tid = has_tid ? d.trie_ids : nullptr;
+    a.n = n;
+    a.ntries = ntries_;
+    a.kh_bytes = d.keys_bytes;
+    a.v_bytes = d.values_total;
+    a.kbase = key_used_;
+    a.vbase = vbase;
+    size_t ne = 2 * (size_t)n + 2;
+    a.klen = c.take<uint32_t>(n);
+    a.kpoff = c.take<uint64_t>(n);
+    a.sk0 = c.take<uint64_t>(n);
+    a.sk1 = c.take<uint64_t>(n);
+    a.sv0 = c.take<uint32_t>(n);
+    a.sv1 = c.take<uint32_t>(n);
+    if (has_tid) {
+      a.tk0 = c.take<uint32_t>(n);
+      a.tk1 = c.take<uint32_t>(n);
+    }
+    a.scal = c.take<uint32_t>(16);
+    a.lcp = c.take<uint8_t>(n);
+    a.flags = c.take<uint8_t>(n);
+    a.run = c.take<uint32_t>(n);
+    a.u_idx = c.take<uint32_t>(n);
+    a.u_trie = c.take<uint32_t>(n);
+    a.u_lcp = c.take<uint8_t>(n);
+    a.u_first = c.take<uint8_t>(n);
+    a.stop = c.take<Stop>(n);
+    a.grp = c.take<uint32_t>(n);
+    a.ginfo = c.take<GroupInfo>(n);
+    a.ecount = c.take<uint32_t>(n);
+    a.merge = c.take<uint32_t>(n);
+    a.u_idx2 = c.take<uint32_t>(n);
+    a.u_trie2 = c.take<uint32_t>(n);
+    a.perm0 = c.take<uint32_t>(n);
+    a.perm1 = c.take<uint32_t>(n);
+    a.gkey0 = c.take<uint32_t>(n);
+    a.gkey1 = c.take<uint32_t>(n);
+    a.u_lcp2 = c.take<uint8_t>(n);
+    a.stop2 = c.take<Stop>(n);
+    a.e_src = c.take<uint32_t>(ne);
+    a.e_grp = c.take<uint32_t>(ne);
+    a.st_levels = st_levels_for(ne);
+    a.st_stride = align_up(ne, 256);
+    a.st = c.take<uint8_t>(a.st_stride * (size_t)a.st_levels);
+    a.repof = c.take<uint32_t>(ne);
+    a.bid = c.take<uint32_t>(ne);
+    void *temp = c.take<uint8_t>(std::max(radix_temp_bytes(n), scan_temp_bytes(ne)));
+    if (c.off > ws_.cap) {
+      set_error("workspace carve overflow");
+      return -4;
+    }
+    HIP_TRY(hipMemsetAsync(a.scal, 0, 64, stream_));
+    HIP_TRY(hipMemsetAsync(a.scal + SC_W, 0xff, 4, stream_));
+    HIP_TRY(nu_prepare_keys(a, temp, stream_));
+    uint32_t trie_bits = 0;
+    if (has_tid) {
+      while ((1ull << trie_bits) < ntries_) trie_bits++;
+    }
+    uint64_t *sk = nullptr;
+    uint32_t *sv = nullptr;
+    HIP_TRY(nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv));
+    HIP_TRY(nu_adjacent(a, sk, sv, 0, stream_));
+    if ((r = sync_counters())) return r;
+    if (cnt_.flag_fallback) {
+      // Two distinct keys share the 16-nibble window: sort by the full keys.
+      stats_.fallback_sorts++;
+      HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+      HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      uint32_t W = h_scal_[SC_W], maxlen = h_scal_[SC_MAXLEN];
+      int nwords = (int)((maxlen > W ? maxlen - W : 1) + 15) / 16;
+      HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv));
+      HIP_TRY(nu_adjacent(a, sk, sv, 1, stream_));
+    }
+    HIP_TRY(nu_unique(a, sv, temp, stream_));
+    HIP_TRY(nu_descend_and_group(a, temp, stream_));
+    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
+    if ((r = sync_counters())) return r;
+    if (h_scal_[SC_MERGE]) {
+      // a split group straddles a subtree: make every group contiguous
+      uint32_t m = h_scal_[SC_M_UNIQ];
+      HIP_TRY(nu_regroup(a, m, m, temp, stream_));
+      stats_.regroups++;
+    }
+    HIP_TRY(nu_emit(a, temp, stream_));
+    HIP_TRY(nu_build(a, stream_));
+    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
+    if ((r = sync_counters())) return r;
+    key_used_ += *(uint64_t *)(h_scal_ + SC_KTOT);
+    key_used_ = align_up(key_used_, 8);
+  }
+  if (targets) *targets = dirty_.p;
+  if (n_targets) *n_targets = (int)cnt_.n_dirty;
+  return 0;
+}
+
+int Forest::hash(void *targets, int n_targets) {
+  int r;
+  if ((r = set_device())) return r;
+  if (targets && targets != dirty_.p) {
+    set_error("hash targets do not belong to this trie");
+    return -3;
+  }
+  (void)n_targets;  // the whole dirty list is hashed: a subset would leave stale ancestors
+  uint32_t n = cnt_.n_dirty;
+  stats_.hash_launches = 0;
+  stats_.hashed_nodes = n;
+  stats_.last_hash_ms = 0;
+  if (n > 0) {
+    Pools P = pools();
+    if ((r = ensure_ws(hc_sorted_, sizeof(ref_t) * (size_t)n))) return r;
+    if ((r = ensure_ws(hc_hist_, sizeof(uint32_t) * kHcBins))) return r;
+    if ((r = ensure_ws(hc_cursor_, sizeof(uint32_t) * kHcBins))) return r;
+    if ((r = ensure_ws(ws2_, scan_temp_bytes(kHcBins)))) return r;
+    ref_t *sorted = (ref_t *)hc_sorted_.p;
+    HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (uint32_t *)hc_hist_.p, (uint32_t *)hc_cursor_.p, sorted,
+                        ws2_.p, stream_));
+    HIP_TRY(hipMemcpyAsync(h_hist_.data(), hc_hist_.p, sizeof(uint32_t) * kHcBins, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipEventRecord(ev0_, stream_));
+    uint32_t start = 0;
+    for (uint32_t lv = 0; lv < kHcLevels; lv++) {
+      uint32_t cnt = 0;
+      for (uint32_t c = 0; c < kHcClasses; c++) cnt += h_hist_[lv * kHcClasses + c];
+      if (cnt) {
+        HIP_TRY(hc_hash_range(P, sorted + start, cnt, stream_));
+        stats_.hash_launches++;
+      }
+      start += cnt;
+    }
+    HIP_TRY(hipEventRecord(ev1_, stream_));
+  }
+  HIP_TRY(hc_roots(pools(), ntries_, stream_));
+  HIP_TRY(hipMemsetAsync(&d_cnt_->n_dirty, 0, sizeof(uint32_t), stream_));
+  if ((r = sync_counters())) return r;
+  stats_.perms = cnt_.n_perms;
+  stats_.enc_bytes = cnt_.enc_bytes;
+  HIP_TRY(hipMemsetAsync(&d_cnt_->n_perms, 0, 16, stream_));
+  if (n > 0) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+    stats_.last_hash_ms = ms;
+  }
+  return 0;
+}
+
+int Forest::root_hash(uint32_t trie, uint8_t *out, int *size) {
+  int r;
+  if ((r = set_device())) return r;
+  if (trie >= ntries_) {
+    set_error("trie id out of range");
+    return -3;
+  }
+  TrieHdr h;
+  HIP_TRY(hipMemcpyAsync(&h, (TrieHdr *)tries_.p + trie, sizeof(TrieHdr), hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  *size = (int)h.root_hash_size;
+  if (out) memcpy(out, h.root_hash, h.root_hash_size);
+  return 0;
+}
+
+int Forest::reset_trie(uint32_t trie) {
+  int r;
+  if ((r = set_device())) return r;
+  if (trie >= ntries_) {
+    set_error("trie id out of range");
+    return -3;
+  }
+  HIP_TRY(hipMemsetAsync((TrieHdr *)tries_.p + trie, 0, sizeof(TrieHdr), stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  return 0;
+}
+
+int Forest::clear() {
+  int r;
+  if ((r = set_device())) return r;
+  HIP_TRY(hipMemsetAsync(d_cnt_, 0, sizeof(Counters), stream_));
+  HIP_TRY(hipMemsetAsync(tries_.p, 0, sizeof(TrieHdr) * ntries_, stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  memset(&cnt_, 0, sizeof(cnt_));
+  key_used_ = 0;
+  // keep the first value chunk, release the others
+  for (size_t i = 1; i < vchunks_.size(); i++) HIP_TRY(hipFree(vchunks_[i].p));
+  if (vchunks_.size() > 1) vchunks_.resize(1);
+  vchunk_used_ = 0;
+  return 0;
+}
+
+int Forest::synchronize() {
+  int r;
+  if ((r = set_device())) return r;
+  HIP_TRY(hipStreamSynchronize(stream_));
+  return 0;
+}
+
+int Forest::gets(const BatchDesc &keys, uint64_t *hps, int32_t *sizes) {
+  int r;
+  if ((r = set_device())) return r;
+  uint32_t n = keys.n;
+  if (n == 0) return 0;
+  BatchDesc d;
+  uint64_t vb;
+  if ((r = stage_batch(keys, &d, &vb, false))) return r;
+  if ((r = ensure_ws(ws2_, 16ull * n + 1024))) return r;
+  Carver c(ws2_.p);
+  uint64_t *d_hp = c.take<uint64_t>(n);
+  uint64_t *d_addr = c.take<uint64_t>(n);
+  int32_t *d_size = c.take<int32_t>(n);
+  if (c.off > ws2_.cap && (r = ensure_ws(ws2_, c.off))) return r;
+  GetArgs g{pools(), d.keys_hexs, d.keys_indexs, ntries_ > 1 ? d.trie_ids : nullptr, n, d.keys_bytes, ntries_, d_hp, d_addr, d_size};
+  HIP_TRY(launch_gets(g, stream_));
+  if (hps) HIP_TRY(hipMemcpyAsync(hps, d_hp, 8ull * n, hipMemcpyDeviceToHost, stream_));
+  if (sizes) HIP_TRY(hipMemcpyAsync(sizes, d_size, 4ull * n, hipMemcpyDeviceToHost, stream_));
+  return sync_counters();
+}
+
+int Forest::get_values(const BatchDesc &keys, uint8_t *out, int64_t out_cap, int64_t *offsets, int32_t *sizes) {
+  int r;
+  if ((r = set_device())) return r;
+  uint32_t n = keys.n;
+  if (n == 0) return 0;
+  BatchDesc d;
+  uint64_t vb;
+  if ((r = stage_batch(keys, &d, &vb, false))) return r;
+  size_t need = 32ull * n + 4096;
+  if ((r = ensure_ws(ws2_, need))) return r;
+  Carver c(ws2_.p);
+  uint64_t *d_hp = c.take<uint64_t>(n);
+  uint64_t *d_addr = c.take<uint64_t>(n);
+  int32_t *d_size = c.take<int32_t>(n);
+  uint64_t *d_off = c.take<uint64_t>(n);
+  GetArgs g{pools(), d.keys_hexs, d.keys_indexs, ntries_ > 1 ? d.trie_ids : nullptr, n, d.keys_bytes, ntries_, d_hp, d_addr, d_size};
+  HIP_TRY(launch_gets(g, stream_));
+  std::vector<int32_t> hs(n);
+  HIP_TRY(hipMemcpyAsync(hs.data(), d_size, 4ull * n, hipMemcpyDeviceToHost, stream_));
+  if ((r = sync_counters())) return r;
+  std::vector<uint64_t> ho(n);
+  uint64_t tot = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    ho[i] = tot;
+    if (hs[i] > 0) tot += (uint64_t)hs[i];
+  }
+  if ((int64_t)tot > out_cap) {
+    set_error("get_values: output buffer too small (" + std::to_string(tot) + " bytes needed)");
+    return -3;
+  }
+  DevBuf obuf;
+  if (tot) {
+    HIP_TRY(hipMalloc(&obuf.p, tot));
+    HIP_TRY(hipMemcpyAsync(d_off, ho.data(), 8ull * n, hipMemcpyHostToDevice, stream_));
+    HIP_TRY(launch_gather_values(d_addr, d_size, d_off, n, (uint8_t *)obuf.p, stream_));
+    HIP_TRY(hipMemcpyAsync(out, obuf.p, tot, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipFree(obuf.p));
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    if (offsets) offsets[i] = (int64_t)ho[i];
+    if (sizes) sizes[i] = hs[i];
+  }
+  return 0;
+}
+
+int Forest::export_nodes(const BatchDesc &keys, uint8_t **hashes, uint8_t **encs, uint64_t **enc_indexs,
+                         uint64_t *num) {
+  int r;
+  if ((r = set_device())) return r;
+  *hashes = nullptr;
+  *encs = nullptr;
+  *enc_indexs = nullptr;
+  *num = 0;
+  uint32_t n = keys.n;
+  if (n == 0) return 0;
+  BatchDesc d;
+  uint64_t vb;
+  if ((r = stage_batch(keys, &d, &vb, false))) return r;
+  return export_paths(this, d, hashes, encs, enc_indexs, num);
+}
+
+int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
+  int r;
+  if ((r = set_device())) return r;
+  if (b.n > 0x3fffffffu) {
+    set_error("batch too large");
+    return -3;
+  }
+  if (b.on_device && b.n && (b.keys_bytes == 0)) {
+    set_error("device batch needs keys_bytes and values_total");
+    return -3;
+  }
+  if (b.n > 0) {
+    BatchDesc d;
+    uint64_t vbase = 0;
+    if ((r = stage_batch(b, &d, &vbase, true))) return r;
+    const uint32_t n = b.n;
+    if ((r = ensure_pools(n, 2ull * n, n, d.keys_bytes / 2 + 8ull * n))) return r;
+    if ((r = ensure_ws(ws_, 12ull * n + 4096 + scan_temp_bytes(n)))) return r;
+    HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
+    Carver c(ws_.p);
+    NuArgs a;
+    memset(&a, 0, sizeof(a));
+    a.P = pools();
+    a.kh = d.keys_hexs;
+    a.kidx = d.keys_indexs;
+    a.vidx = d.values_indexs;
+    a.hps = d.values_hps;
+    a.tid = ntries_ > 1 ? d.trie_ids : nullptr;
+    a.n = n;
+    a.ntries = ntries_;
+    a.kh_bytes = d.keys_bytes;
+    a.v_bytes = d.values_total;
+    a.kbase = key_used_;
+    a.vbase = vbase;
+    a.klen = c.take<uint32_t>(n);
+    a.kpoff = c.take<uint64_t>(n);
+    a.scal = c.take<uint32_t>(16);
+    a.seq_base = lock_seq_;
+    a.base_branch = cnt_.n_branch;
+    a.base_short = cnt_.n_short;
+    a.base_value = cnt_.n_value;
+    void *temp = c.take<uint8_t>(scan_temp_bytes(n));
+    if (c.off > ws_.cap) {
+      set_error("workspace carve overflow");
+      return -4;
+    }
+    HIP_TRY(hipMemsetAsync(a.scal, 0, 64, stream_));
+    HIP_TRY(hipMemsetAsync(a.scal + SC_W, 0xff, 4, stream_));
+    HIP_TRY(nu_prepare_keys(a, temp, stream_));
+    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
+    if ((r = sync_counters())) return r;
+    uint64_t ktot = *(uint64_t *)(h_scal_ + SC_KTOT);
+    HIP_TRY(lock_insert(a, key_used_ + ktot, stream_));
+    if ((r = sync_counters())) return r;
+    key_used_ = align_up(key_used_ + ktot, 8);
+    lock_seq_ += n;
+    stats_.restarts = cnt_.n_restarts;
+    stats_.lock_fails = cnt_.n_lock_fail;
+  }
+  if (targets) *targets = dirty_.p;
+  if (n_targets) *n_targets = (int)cnt_.n_dirty;
+  return 0;
+}
+
+}  // namespace gmpt

@@ -1,0 +1,107 @@
+// phasenu.h -- PhaseNU: sort-based batch insert (host launch interface).
+#pragma once
+#include "common.h"
+
+namespace gmpt {
+
+// Where a key's descent into the existing trie stopped.
+enum : uint8_t {
+  STOP_ROOT_EMPTY = 0,   // the trie is empty: the key starts a new root
+  STOP_BRANCH_SLOT = 1,  // an empty slot of an existing branch
+  STOP_SHORT_SPLIT = 2,  // the key diverges inside an existing short node
+  STOP_LEAF_UPDATE = 3,  // the key exists (leaf short node): replace its value
+  STOP_VALUE_UPDATE = 4, // the key exists (value in a branch's slot 16)
+};
+
+struct Stop {
+  ref_t stop;        // short node (split / leaf update), or branch (value update)
+  ref_t attach;      // branch whose slot gets the new subtree, REF_NULL for the trie root
+  uint16_t base;     // first nibble the new subtree covers
+  uint16_t div;      // split: absolute nibble where the key leaves the short node
+  uint8_t kind;
+  uint8_t slot;
+  uint8_t less;      // split: key orders before the short node's subtree
+  uint8_t pad;
+};
+
+struct GroupInfo {
+  ref_t attach;
+  ref_t split;       // short node being split (REF_NULL if none)
+  uint32_t trie;
+  uint16_t base;
+  uint8_t slot;
+  uint8_t has_split;
+};
+
+// Device pointers of one PhaseNU call.  All arrays are sized by the host from
+// n (the batch size); kernels read the actual counts (m, M) from device words.
+struct NuArgs {
+  Pools P;
+  // batch (device)
+  const uint8_t *kh;
+  const int32_t *kidx;
+  const int64_t *vidx;
+  const uint64_t *hps;   // nullable
+  const int32_t *tid;    // nullable
+  uint32_t n;
+  uint32_t ntries;
+  uint64_t kh_bytes;     // bytes in kh (index pairs are checked against it)
+  uint64_t v_bytes;      // bytes of the batch's values
+  uint64_t kbase;        // this batch's offset in the key arena
+  uint64_t vbase;        // device address of this batch's value bytes
+  // per input key
+  uint32_t *klen;
+  uint64_t *kpoff;       // relative to kbase after the scan
+  // sort
+  uint64_t *sk0, *sk1;
+  uint32_t *sv0, *sv1;
+  uint32_t *tk0, *tk1;
+  uint32_t *scal;        // device scalars, see SC_*
+  // sorted adjacent info
+  uint8_t *lcp;          // lcp with previous sorted key (absolute nibbles), valid if !newtrie
+  uint8_t *flags;        // bit0 dup-of-previous, bit1 new trie
+  uint32_t *run;         // run id (inclusive scan result)
+  // unique keys (m)
+  uint32_t *u_idx, *u_trie;
+  uint8_t *u_lcp, *u_first;  // lcp with previous unique key; first key of its trie
+  Stop *stop;
+  uint32_t *grp;         // group id of each unique key
+  GroupInfo *ginfo;
+  uint32_t *ecount;      // E entries emitted per unique key, scanned to positions
+  uint32_t *merge;       // per run: run whose group it joins (split groups around a subtree)
+  uint32_t *u_idx2, *u_trie2, *perm0, *perm1, *gkey0, *gkey1;
+  uint8_t *u_lcp2;
+  Stop *stop2;
+  // E elements (M <= 2m)
+  uint32_t *e_src;       // bit31: virtual (short node idx) else unique index
+  uint32_t *e_grp;
+  uint8_t *st;           // sparse table, levels x stride bytes; level 0 = split-point lcp+1
+  size_t st_stride;
+  int st_levels;
+  uint32_t *repof;       // per split point: representative split point (or ~0u)
+  uint32_t *bid;         // per split point: branch slot (reps only)
+  // LockNU
+  uint64_t seq_base;     // batch order stamp of key 0
+  uint32_t base_branch, base_short, base_value;  // pre-assigned node slots
+};
+
+// device scalar slots in NuArgs::scal
+enum { SC_W = 0, SC_MAXLEN = 1, SC_M_UNIQ = 2, SC_NGROUP = 3, SC_NE = 4, SC_KTOT = 6 /* u64 */, SC_MERGE = 8,
+       SC_COUNT = 16 };
+
+hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s);
+hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
+                   uint32_t **sv);
+hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s);
+hipError_t nu_unique(const NuArgs &a, const uint32_t *sv, void *temp, hipStream_t s);
+// Descends every unique key and groups them by stop point; sets SC_MERGE when
+// a split group is interrupted by keys below the split node (nu_regroup).
+hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s);
+// Makes every group contiguous (stable sort of the unique keys by group),
+// swapping the unique-key arrays of `a`, and regroups.
+hipError_t nu_regroup(NuArgs &a, uint32_t m, uint32_t nruns, void *temp, hipStream_t s);
+// Emits the element sequence E of every group.
+hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s);
+hipError_t nu_build(const NuArgs &a, hipStream_t s);
+
+}  // namespace gmpt

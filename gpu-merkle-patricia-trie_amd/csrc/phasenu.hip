@@ -1,0 +1,1036 @@
+// phasenu.hip -- PhaseNU: sort-based batch insert into empty or existing tries.
+//
+// Reference: puts_2phase_with_valuehp (mpt-with-compress/include/mpt/gpu_mpt.cuh:
+// 1262-1399) runs split (gpu_mpt_kernels.cuh:3592-3673), a per-nibble full
+// expansion insert (:3823-4056) and compress (:4420-4666).  This version sorts
+// the batch's nibble paths instead, so each trie node is built once:
+//   1. pack hex keys into the key arena; sort (trie, 16-nibble window) pairs
+//   2. adjacent LCPs, dedup keeping the last occurrence (last writer wins,
+//      matching the CPU MPT's sequential puts_baseline, cpu_mpt.cuh:359-378)
+//   3. descend each unique key into the existing trie to its stop point
+//   4. per stop point ("group") build the new subtree from the LCP array: each
+//      distinct LCP interval is a branch, gaps between depths are extension
+//      short nodes, each key a leaf short node; a short node that new keys
+//      split joins its group as one more element ("virtual" element)
+// The result is the same trie the sequential insert produces.
+#include <limits.h>
+#include "phasenu.h"
+#include "prims.h"
+
+namespace gmpt {
+
+namespace {
+
+constexpr int kB = 256;
+
+__device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+inline unsigned nblk(size_t n) { return (unsigned)((n + kB - 1) / kB); }
+
+// A key stored in the key arena: nibble j at absolute path position j, the
+// terminator at position len-1.
+struct KeyRef {
+  const uint8_t *p;
+  uint32_t len;
+  __device__ __forceinline__ uint32_t nib(uint32_t j) const { return j + 1 == len ? kNibbleT : packed_nibble(p, j); }
+};
+
+__device__ __forceinline__ KeyRef key_of(const NuArgs &a, uint32_t idx) {
+  KeyRef k;
+  k.p = a.P.keys + a.kbase + a.kpoff[idx];
+  k.len = a.klen[idx];
+  return k;
+}
+
+// Nibble at absolute position j (inside the node's key range) of short node S.
+__device__ __forceinline__ uint32_t short_nib(const Pools &P, const Short &S, uint32_t j) {
+  uint32_t rel = j - S.depth;
+  if (S.term && rel + 1 == S.key_len) return kNibbleT;
+  return packed_nibble(P.keys + S.key_off, S.key_start + rel);
+}
+
+// ------------------------------------------------------------- 1. keys ----
+
+__device__ __forceinline__ bool key_range_ok(const NuArgs &a, uint32_t i) {
+  int32_t s = a.kidx[2 * i], e = a.kidx[2 * i + 1];
+  return s >= 0 && e >= s && (uint64_t)e < a.kh_bytes && e - s + 1 <= kMaxKeyNibbles;
+}
+
+__global__ __launch_bounds__(256) void k_keys_scan(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t lcp0 = UINT_MAX, len = 0;
+  if (i < a.n) {
+    bool ok = key_range_ok(a, i);
+    int32_t L = 1;
+    if (ok) {
+      int32_t s = a.kidx[2 * i], e = a.kidx[2 * i + 1];
+      L = e - s + 1;
+      const uint8_t *k = a.kh + s;
+      ok = k[L - 1] == kNibbleT;
+      for (int j = 0; ok && j < L - 1; j++) ok = k[j] < 16;
+      // common prefix with key 0 (terminator counts as a distinct symbol)
+      if (ok && key_range_ok(a, 0)) {
+        int32_t s0 = a.kidx[0], e0 = a.kidx[1];
+        const uint8_t *k0 = a.kh + s0;
+        int32_t L0 = e0 - s0 + 1;
+        uint32_t p = 0;
+        while ((int32_t)p < L && (int32_t)p < L0 && k[p] == k0[p]) p++;
+        lcp0 = p;
+      } else {
+        lcp0 = 0;
+      }
+    }
+    if (ok && a.tid && (uint32_t)a.tid[i] >= a.ntries) ok = false;
+    if (ok) {
+      int64_t vs = a.vidx[2 * i], ve = a.vidx[2 * i + 1];
+      ok = vs >= 0 && ve >= vs - 1 && (uint64_t)(ve + 1) <= a.v_bytes && ve - vs + 1 < (1ll << 31);
+    }
+    if (!ok) {
+      raise_error(a.P.cnt, ERR_BAD_KEY, i);
+      L = 1;
+      lcp0 = 0;
+    }
+    len = (uint32_t)L;
+    a.klen[i] = len;
+    a.kpoff[i] = (uint64_t)((len / 2 + 8) & ~7u);  // packed slot bytes (>= 8)
+  }
+  // wave-reduce then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    lcp0 = min(lcp0, (uint32_t)__shfl_xor(lcp0, off));
+    len = max(len, (uint32_t)__shfl_xor(len, off));
+  }
+  if (__lane_id() == 0) {
+    atomicMin(&a.scal[SC_W], lcp0);
+    atomicMax(&a.scal[SC_MAXLEN], len);
+  }
+}
+
+// Packs nibbles two per byte into the key arena and builds the sort window:
+// 16 nibbles from the batch's common prefix W, terminator and padding as 0.
+__global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
+  uint32_t i = grid_tid();
+  if (i >= a.n) return;
+  uint32_t L = a.klen[i];
+  const uint8_t *k = a.kh + (key_range_ok(a, i) ? a.kidx[2 * i] : 0);
+  uint8_t *dst = a.P.keys + a.kbase + a.kpoff[i];
+  uint32_t nb = (L / 2 + 8) & ~7u;
+  uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
+  for (uint32_t b = 0; b < nb; b += 8) {
+    uint64_t w = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      uint32_t j = 2 * (b + q);
+      uint32_t hi = j < nn ? k[j] : 0u;
+      uint32_t lo = j + 1 < nn ? k[j + 1] : 0u;
+      w |= (uint64_t)((hi << 4) | lo) << (8 * q);
+    }
+    *(uint64_t *)(dst + b) = w;
+  }
+  if (!a.sk0) return;  // LockNU packs keys without sorting them
+  uint32_t W = a.scal[SC_W];
+  uint64_t win = 0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    uint32_t j = W + q;
+    uint64_t v = j < nn ? (uint64_t)k[j] : 0ull;
+    win |= v << (60 - 4 * q);
+  }
+  a.sk0[i] = win;
+  a.sv0[i] = i;
+}
+
+// Full-key sort support: word `w` (16 nibbles from W + 16w) of key sv[j], or
+// the key length for w < 0.
+__global__ __launch_bounds__(256) void k_gather_word(NuArgs a, const uint32_t *sv, uint64_t *sk, int w) {
+  uint32_t j = grid_tid();
+  if (j >= a.n) return;
+  uint32_t i = sv[j];
+  uint32_t L = a.klen[i];
+  if (w < 0) {
+    sk[j] = L;
+    return;
+  }
+  const uint8_t *k = a.kh + (key_range_ok(a, i) ? a.kidx[2 * i] : 0);
+  uint32_t nn = L - 1;
+  uint32_t W = a.scal[SC_W] + 16u * (uint32_t)w;
+  uint64_t win = 0;
+  for (int q = 0; q < 16; q++) {
+    uint32_t jj = W + q;
+    uint64_t v = jj < nn ? (uint64_t)k[jj] : 0ull;
+    win |= v << (60 - 4 * q);
+  }
+  sk[j] = win;
+}
+
+__global__ __launch_bounds__(256) void k_gather_trie(NuArgs a, const uint32_t *sv, uint32_t *tk) {
+  uint32_t j = grid_tid();
+  if (j >= a.n) return;
+  tk[j] = a.tid ? (uint32_t)a.tid[sv[j]] : 0u;
+}
+
+// ----------------------------------------------------- 2. adjacent/dedup --
+
+__device__ __forceinline__ uint32_t trie_of(const NuArgs &a, uint32_t idx) {
+  return a.tid ? (uint32_t)a.tid[idx] : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, const uint32_t *sv, int full) {
+  uint32_t j = grid_tid();
+  if (j >= a.n) return;
+  uint32_t ib = sv[j];
+  uint8_t fl = 0, l = 0;
+  if (j == 0) {
+    fl = 2;
+  } else {
+    uint32_t ia = sv[j - 1];
+    if (trie_of(a, ia) != trie_of(a, ib)) {
+      fl = 2;
+    } else {
+      KeyRef ka = key_of(a, ia), kb = key_of(a, ib);
+      uint32_t W = a.scal[SC_W];
+      uint32_t mn = min(ka.len, kb.len);
+      uint32_t p;
+      bool ident;
+      uint64_t x = full ? 0ull : (sk[j - 1] ^ sk[j]);
+      if (x != 0) {
+        p = W + (uint32_t)(__clzll((long long)x) >> 2);
+        if (ka.len != kb.len) p = min(p, mn - 1);
+        ident = false;
+      } else {
+        // windows agree (or full mode): compare nibble by nibble; with
+        // different lengths the shorter key's terminator (0 in the window)
+        // differs from the longer key's nibble at mn - 1
+        p = full ? 0u : min(W + 16u, mn);
+        if (ka.len != kb.len) p = min(p, mn - 1);
+        while (p < mn && ka.nib(p) == kb.nib(p)) p++;
+        ident = (p == mn) && ka.len == kb.len;
+        if (!ident && !full) atomicOr(&a.P.cnt->flag_fallback, 1u);
+      }
+      fl = ident ? 1 : 0;
+      l = (uint8_t)min(p, 255u);
+    }
+  }
+  a.lcp[j] = l;
+  a.flags[j] = fl;
+  a.run[j] = (fl & 1) ? 0u : 1u;
+}
+
+// After the inclusive scan of run starts: run r spans sorted positions with
+// run[j] == r+1.  The first position carries the LCP to the previous distinct
+// key, the last position the surviving (last-written) input index.
+__global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint32_t *sv, const uint32_t *run_excl) {
+  uint32_t j = grid_tid();
+  if (j >= a.n) return;
+  bool first = (a.flags[j] & 1) == 0;
+  bool last = (j + 1 == a.n) || ((a.flags[j + 1] & 1) == 0);
+  // run_excl is the exclusive scan of run-start flags: a run start's value is
+  // its run index; later positions of the run count that start already.
+  uint32_t r = first ? run_excl[j] : run_excl[j] - 1u;
+  if (first) {
+    a.u_lcp[r] = a.lcp[j];
+    a.u_first[r] = (a.flags[j] >> 1) & 1;
+    a.u_trie[r] = trie_of(a, sv[j]);
+  }
+  if (last) a.u_idx[r] = sv[j];
+  if (j + 1 == a.n) a.scal[SC_M_UNIQ] = r + 1;
+}
+
+// --------------------------------------------------- 3. descend + groups --
+
+__global__ __launch_bounds__(256) void k_descend(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  bool active = i < m;
+  const Pools &P = a.P;
+  Stop st;
+  st.stop = REF_NULL;
+  st.attach = REF_NULL;
+  st.base = 0;
+  st.div = 0;
+  st.kind = STOP_ROOT_EMPTY;
+  st.slot = 0;
+  st.less = 0;
+  st.pad = 0;
+  KeyRef k;
+  ref_t node = REF_NULL, parent = REF_NULL;
+  uint32_t pos = 0, pslot = 0;
+  bool done = !active;
+  if (active) {
+    uint32_t idx = a.u_idx[i];
+    k = key_of(a, idx);
+    uint32_t t = a.u_trie[i];
+    if (t < a.ntries) {
+      node = P.tries[t].root;
+    } else {
+      done = true;
+      raise_error(P.cnt, ERR_BAD_KEY, idx);
+    }
+  }
+  // the loop is written so every lane reaches the wave-wide dirty marks
+  for (int guard = 0; guard < 2 * kMaxKeyNibbles + 4; guard++) {
+    bool want_mark = false;
+    ref_t mark_ref = REF_NULL;
+    if (!done) {
+      uint32_t ty = ref_type(node);
+      if (node == REF_NULL) {
+        if (parent == REF_NULL) {
+          st.kind = STOP_ROOT_EMPTY;
+          st.base = 0;
+        } else {
+          st.kind = STOP_BRANCH_SLOT;
+          st.attach = parent;
+          st.slot = (uint8_t)pslot;
+          st.base = (uint16_t)pos;
+        }
+        done = true;
+      } else if (ty == T_BRANCH) {
+        const Branch &B = P.branch[ref_idx(node)];
+        want_mark = true;
+        mark_ref = node;
+        if (pos >= k.len || B.depth != pos) {
+          raise_error(P.cnt, ERR_INTERNAL, 12);
+          done = true;
+          break;
+        }
+        uint32_t nib = k.nib(pos);
+        parent = node;
+        pslot = nib;
+        node = B.child[nib];
+        pos += 1;
+      } else if (ty == T_SHORT) {
+        const ref_t cur = node;
+        const Short &S = P.shrt[ref_idx(cur)];
+        uint32_t ml = S.key_len;
+        uint32_t p = 0;
+        while (p < ml && pos + p < k.len && short_nib(P, S, pos + p) == k.nib(pos + p)) p++;
+        if (p == ml) {
+          want_mark = true;
+          mark_ref = cur;
+          if (ref_type(S.child) == T_VALUE) {
+            st.kind = STOP_LEAF_UPDATE;
+            st.stop = cur;
+            st.attach = parent;
+            st.slot = (uint8_t)pslot;
+            st.base = (uint16_t)pos;
+            done = true;
+          } else {
+            parent = cur;
+            pslot = 0;
+            node = S.child;
+            pos += ml;
+          }
+        } else if (pos + p >= k.len) {
+          raise_error(P.cnt, ERR_INTERNAL, 11);
+          done = true;
+        } else {
+          uint32_t d = pos + p;
+          uint32_t kn = k.nib(d), sn = short_nib(P, S, d);
+          st.kind = STOP_SHORT_SPLIT;
+          st.stop = cur;
+          st.attach = parent;
+          st.slot = (uint8_t)pslot;
+          st.base = (uint16_t)pos;
+          st.div = (uint16_t)d;
+          st.less = (kn == kNibbleT) || (sn != kNibbleT && kn < sn);
+          done = true;
+        }
+      } else {  // value in a branch slot: the key ends here
+        st.kind = STOP_VALUE_UPDATE;
+        st.stop = parent;
+        st.attach = parent;
+        st.slot = (uint8_t)pslot;
+        st.base = (uint16_t)pos;
+        done = true;
+      }
+    }
+    mark_dirty(P, mark_ref, want_mark);
+    if (__ballot(!done) == 0) break;
+  }
+  if (active) {
+    if (!done) raise_error(P.cnt, ERR_INTERNAL, 10);
+    a.stop[i] = st;
+  }
+}
+
+__device__ __forceinline__ bool same_group(const Stop &x, uint32_t tx, const Stop &y, uint32_t ty) {
+  if (tx != ty) return false;
+  if (x.kind == STOP_VALUE_UPDATE || y.kind == STOP_VALUE_UPDATE) return false;
+  bool xs = x.kind == STOP_SHORT_SPLIT || x.kind == STOP_LEAF_UPDATE;
+  bool ys = y.kind == STOP_SHORT_SPLIT || y.kind == STOP_LEAF_UPDATE;
+  if (xs || ys) return xs && ys && x.stop == y.stop;
+  if (x.kind != y.kind) return false;
+  if (x.kind == STOP_ROOT_EMPTY) return true;
+  return x.attach == y.attach && x.slot == y.slot;  // branch slot
+}
+
+__global__ __launch_bounds__(256) void k_group_flags(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= a.n) return;
+  if (i >= m) {
+    a.ecount[i] = 0;
+    return;
+  }
+  bool start = i == 0 || !same_group(a.stop[i - 1], a.u_trie[i - 1], a.stop[i], a.u_trie[i]);
+  a.ecount[i] = start ? 1u : 0u;
+}
+
+// grp[] = exclusive scan of the start flags in ecount[]; make it inclusive,
+// i.e. 1-based group ids.
+__global__ __launch_bounds__(256) void k_group_fix(NuArgs a) {
+  uint32_t i = grid_tid();
+  if (i >= a.n) return;
+  a.grp[i] += a.ecount[i];
+}
+
+// grp[] now holds the inclusive scan; group id = grp - 1.
+__global__ __launch_bounds__(256) void k_group_info(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= m) return;
+  uint32_t g = a.grp[i] - 1;
+  const Stop st = a.stop[i];
+  bool start = i == 0 || a.grp[i - 1] != a.grp[i];
+  if (start) {
+    GroupInfo gi;
+    gi.attach = st.attach;
+    gi.split = (st.kind == STOP_SHORT_SPLIT || st.kind == STOP_LEAF_UPDATE) ? st.stop : REF_NULL;
+    gi.trie = a.u_trie[i];
+    gi.base = st.base;
+    gi.slot = st.slot;
+    gi.has_split = 0;
+    a.ginfo[g] = gi;
+  }
+  if (i + 1 == m) a.scal[SC_NGROUP] = g + 1;
+}
+
+__global__ __launch_bounds__(256) void k_group_split(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= m) return;
+  if (a.stop[i].kind == STOP_SHORT_SPLIT) a.ginfo[a.grp[i] - 1].has_split = 1;
+}
+
+
+// A split group whose short node S is an extension can be interrupted in the
+// sorted key order by keys that pass through S (they sort between the keys
+// ordered before S's subtree and those after it).  The run after the
+// subtree looks up the run before it (binary search for the start of the
+// subtree's key block) and records that it belongs to that run's group.
+__device__ bool key_before_block(const NuArgs &a, uint32_t j, uint32_t trie, const KeyRef &kr, const Short &S,
+                                 uint32_t blen) {
+  uint32_t tj = a.u_trie[j];
+  if (tj != trie) return tj < trie;
+  KeyRef kj = key_of(a, a.u_idx[j]);
+  for (uint32_t q = 0; q < blen; q++) {
+    uint32_t p = q < S.depth ? kr.nib(q) : short_nib(a.P, S, q);
+    uint32_t x = q < kj.len ? kj.nib(q) : kNibbleT;
+    if (x != p) return x == kNibbleT || x < p;  // terminator orders first
+  }
+  return false;  // kj has the whole prefix: inside the block
+}
+
+__global__ __launch_bounds__(256) void k_group_merge(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= m) return;
+  uint32_t run = a.grp[i] - 1;
+  bool start = i == 0 || a.grp[i - 1] != a.grp[i];
+  if (!start) return;
+  a.merge[run] = run;
+  const Stop st = a.stop[i];
+  if (st.kind != STOP_SHORT_SPLIT || st.less) return;
+  const Short &S = a.P.shrt[ref_idx(st.stop)];
+  if (S.term) return;  // a leaf's subtree is one key, already grouped with it
+  uint32_t trie = a.u_trie[i];
+  KeyRef kr = key_of(a, a.u_idx[i]);
+  uint32_t blen = S.depth + S.key_len;
+  uint32_t lo = 0, hi = i;  // first j in [0, i) not before the block
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) / 2;
+    if (key_before_block(a, mid, trie, kr, S, blen))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo == 0) return;
+  uint32_t j = lo - 1;
+  const Stop sj = a.stop[j];
+  if (a.u_trie[j] == trie && sj.kind == STOP_SHORT_SPLIT && sj.stop == st.stop && sj.less) {
+    a.merge[run] = a.grp[j] - 1;
+    a.scal[SC_MERGE] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gkey(NuArgs a, uint32_t m) {
+  uint32_t i = grid_tid();
+  if (i >= m) return;
+  uint32_t run = a.grp[i] - 1;
+  uint32_t tgt = a.merge[run];
+  // the target of a target is itself (a run joins at most one earlier run)
+  a.gkey0[i] = tgt;
+  a.perm0[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_permute(NuArgs a, const uint32_t *perm, uint32_t m) {
+  uint32_t i = grid_tid();
+  if (i >= m) return;
+  uint32_t p = perm[i];
+  a.u_idx2[i] = a.u_idx[p];
+  a.u_trie2[i] = a.u_trie[p];
+  a.u_lcp2[i] = a.u_lcp[p];
+  a.stop2[i] = a.stop[p];
+}
+
+__device__ __forceinline__ bool is_update(uint8_t kind) {
+  return kind == STOP_LEAF_UPDATE || kind == STOP_VALUE_UPDATE;
+}
+
+// E layout per group: [keys ordered before the split node's subtree]
+// [the split node as one virtual element] [keys after it]; updates emit nothing.
+struct Emit {
+  bool self, vbefore, vafter;
+};
+
+__device__ __forceinline__ Emit emit_of(const NuArgs &a, uint32_t i, uint32_t m) {
+  Emit e{false, false, false};
+  const Stop st = a.stop[i];
+  if (is_update(st.kind)) return e;
+  e.self = true;
+  uint32_t g = a.grp[i];
+  const GroupInfo &gi = a.ginfo[g - 1];
+  if (!gi.has_split) return e;
+  if (!st.less) {
+    // first key ordered after the split subtree?
+    bool first = true;
+    if (i > 0 && a.grp[i - 1] == g) {
+      const Stop p = a.stop[i - 1];
+      first = is_update(p.kind) || p.less;
+    }
+    e.vbefore = first;
+  } else {
+    // last key ordered before the subtree, with no key after it in the group?
+    bool last = true;
+    uint32_t j = i + 1;
+    if (j < m && a.grp[j] == g && is_update(a.stop[j].kind)) j++;
+    if (j < m && a.grp[j] == g) last = false;
+    e.vafter = last;
+  }
+  return e;
+}
+
+__global__ __launch_bounds__(256) void k_emit_count(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= a.n) return;
+  uint32_t c = 0;
+  if (i < m) {
+    Emit e = emit_of(a, i, m);
+    c = (uint32_t)e.self + (uint32_t)e.vbefore + (uint32_t)e.vafter;
+  }
+  a.ecount[i] = c;
+}
+
+// ecount[] holds the exclusive scan.  Level 0 of the sparse table gets, at
+// split point q (between E[q] and E[q+1]), lcp(E[q], E[q+1]) + 1, or 0 across
+// a group boundary.
+__global__ __launch_bounds__(256) void k_emit_write(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= m) return;
+  Emit e = emit_of(a, i, m);
+  if (!e.self) {
+    if (i + 1 == m) a.scal[SC_NE] = a.ecount[i];
+    return;
+  }
+  const Stop st = a.stop[i];
+  uint32_t g = a.grp[i];
+  uint32_t pos = a.ecount[i];
+  uint8_t *sp = a.st;  // level 0
+  const GroupInfo &gi = a.ginfo[g - 1];
+  // does an earlier E element of this group exist, and what is its lcp to us?
+  auto prev_in_group = [&](uint32_t ii) -> int {
+    // index of the previous non-update unique key in the group, or -1
+    if (ii == 0) return -1;
+    uint32_t j = ii - 1;
+    if (a.grp[j] != g) return -1;
+    if (is_update(a.stop[j].kind)) {
+      if (j == 0) return -1;
+      j--;
+      if (a.grp[j] != g) return -1;
+    }
+    return (int)j;
+  };
+  if (e.vbefore) {
+    int pj = prev_in_group(i);
+    a.e_src[pos] = 0x80000000u | ref_idx(gi.split);
+    a.e_grp[pos] = g - 1;
+    if (pos > 0) sp[pos - 1] = pj >= 0 ? (uint8_t)(a.stop[pj].div + 1) : 0;
+    pos++;
+    a.e_src[pos] = i;
+    a.e_grp[pos] = g - 1;
+    sp[pos - 1] = (uint8_t)(st.div + 1);
+    pos++;
+  } else {
+    a.e_src[pos] = i;
+    a.e_grp[pos] = g - 1;
+    int pj = prev_in_group(i);
+    if (pos > 0) sp[pos - 1] = pj >= 0 ? (uint8_t)(a.u_lcp[i] + 1) : 0;
+    pos++;
+  }
+  if (e.vafter) {
+    a.e_src[pos] = 0x80000000u | ref_idx(gi.split);
+    a.e_grp[pos] = g - 1;
+    sp[pos - 1] = (uint8_t)(st.div + 1);
+    pos++;
+  }
+  if (i + 1 == m) a.scal[SC_NE] = pos;
+}
+
+// ------------------------------------------------------------- 4. build --
+
+__global__ __launch_bounds__(256) void k_st_level(NuArgs a, int k) {
+  uint32_t q = grid_tid();
+  uint32_t ne = a.scal[SC_NE];
+  uint32_t nsp = ne > 0 ? ne - 1 : 0;
+  uint32_t half = 1u << (k - 1);
+  if (q + 2 * half > nsp) return;
+  const uint8_t *lo = a.st + (size_t)(k - 1) * a.st_stride;
+  uint8_t *hi = a.st + (size_t)k * a.st_stride;
+  hi[q] = min(lo[q], lo[q + half]);
+}
+
+struct ST {
+  const uint8_t *t;
+  size_t stride;
+  int levels;
+  uint32_t nsp;
+  __device__ __forceinline__ uint8_t at(int k, uint32_t q) const { return t[(size_t)k * stride + q]; }
+  // largest q < j with SP[q] < L, or -1
+  __device__ int prev_less(uint32_t j, uint32_t L) const {
+    int64_t pos = j;
+    for (int k = levels - 1; k >= 0; k--) {
+      int64_t s = 1ll << k;
+      if (pos - s >= 0 && at(k, (uint32_t)(pos - s)) >= L) pos -= s;
+    }
+    return (int)(pos - 1);
+  }
+  // smallest q > j with SP[q] < L, or nsp
+  __device__ uint32_t next_less(uint32_t j, uint32_t L) const {
+    uint64_t pos = (uint64_t)j + 1;
+    for (int k = levels - 1; k >= 0; k--) {
+      uint64_t s = 1ull << k;
+      if (pos + s <= nsp && at(k, (uint32_t)pos) >= L) pos += s;
+    }
+    return (uint32_t)pos;
+  }
+  // smallest q >= j with SP[q] <= L, or nsp
+  __device__ uint32_t next_leq(uint32_t j, uint32_t L) const {
+    uint64_t pos = j;
+    for (int k = levels - 1; k >= 0; k--) {
+      uint64_t s = 1ull << k;
+      if (pos + s <= nsp && at(k, (uint32_t)pos) > L) pos += s;
+    }
+    return (uint32_t)pos;
+  }
+};
+
+__device__ __forceinline__ ST st_of(const NuArgs &a) {
+  ST s;
+  s.t = a.st;
+  s.stride = a.st_stride;
+  s.levels = a.st_levels;
+  uint32_t ne = a.scal[SC_NE];
+  s.nsp = ne > 0 ? ne - 1 : 0;
+  return s;
+}
+
+// Per split point: find its branch (the representative is the leftmost split
+// point with the same LCP inside the maximal interval whose LCPs are >= it).
+__global__ __launch_bounds__(256) void k_split_reps(NuArgs a) {
+  uint32_t j = grid_tid();
+  ST s = st_of(a);
+  bool rep = false;
+  if (j < s.nsp) {
+    uint32_t L = s.at(0, j);
+    uint32_t r = ~0u;
+    if (L != 0) {
+      int lo = s.prev_less(j, L);
+      r = s.next_leq((uint32_t)(lo + 1), L);
+      rep = r == j;
+    }
+    a.repof[j] = r;
+  }
+  uint32_t id = wave_alloc(&a.P.cnt->n_branch, rep);
+  if (rep) {
+    if (id >= a.P.cap_branch) {
+      raise_error(a.P.cnt, ERR_POOL, 1);
+      id = 0;
+    }
+    a.bid[j] = id;
+    Branch &B = a.P.branch[id];
+    uint64_t *w = (uint64_t *)&B;
+#pragma unroll
+    for (int q = 0; q < 16; q++) w[q] = 0;
+    B.depth = (uint16_t)(s.at(0, j) - 1);
+    B.dirty = 1;
+  }
+  mark_new_dirty(a.P, make_ref(T_BRANCH, rep ? a.bid[j] : 0), rep);
+}
+
+// Source of an E element's key nibbles.
+struct ElemKey {
+  uint64_t off;       // key arena offset of the source key
+  uint32_t start;     // source nibble index of absolute position `origin`
+  uint32_t origin;
+  uint32_t end;       // absolute end of the element's path (exclusive)
+  uint8_t term;       // path ends with the terminator
+  bool virt;
+  uint32_t uidx;      // unique index (new key) or short node idx (virtual)
+};
+
+__device__ __forceinline__ ElemKey elem_key(const NuArgs &a, uint32_t e) {
+  ElemKey k;
+  uint32_t src = a.e_src[e];
+  if (src & 0x80000000u) {
+    const Short &S = a.P.shrt[src & 0x7fffffffu];
+    k.off = S.key_off;
+    k.start = S.key_start;
+    k.origin = S.depth;
+    k.end = S.depth + S.key_len;
+    k.term = S.term;
+    k.virt = true;
+    k.uidx = src & 0x7fffffffu;
+  } else {
+    uint32_t idx = a.u_idx[src];
+    k.off = a.kbase + a.kpoff[idx];
+    k.start = 0;
+    k.origin = 0;
+    k.end = a.klen[idx];
+    k.term = 1;
+    k.virt = false;
+    k.uidx = src;
+  }
+  return k;
+}
+
+__device__ __forceinline__ uint32_t elem_nib(const Pools &P, const ElemKey &k, uint32_t j) {
+  if (k.term && j + 1 == k.end) return kNibbleT;
+  return packed_nibble(P.keys + k.off, k.start + (j - k.origin));
+}
+
+__device__ __forceinline__ void init_short(Short &S, uint64_t key_off, uint32_t key_start, uint32_t key_len,
+                                           uint32_t depth, uint8_t term, ref_t child) {
+  uint64_t *w = (uint64_t *)&S;
+#pragma unroll
+  for (int q = 0; q < 8; q++) w[q] = 0;
+  S.key_off = key_off;
+  S.child = child;
+  S.key_start = (uint16_t)key_start;
+  S.key_len = (uint16_t)key_len;
+  S.depth = (uint16_t)depth;
+  S.term = term;
+  S.dirty = 1;
+}
+
+__device__ __forceinline__ void install(const NuArgs &a, const GroupInfo &gi, ref_t r) {
+  if (gi.attach == REF_NULL)
+    a.P.tries[gi.trie].root = r;
+  else
+    a.P.branch[ref_idx(gi.attach)].child[gi.slot] = r;
+}
+
+// Branches: link each to its parent (through an extension when depths skip).
+__global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
+  uint32_t j = grid_tid();
+  ST s = st_of(a);
+  bool rep = false, need_ext = false;
+  uint32_t L = 0, lo_depth = 0;
+  int lo = -1;
+  uint32_t hi = 0;
+  ElemKey ek;
+  GroupInfo gi;
+  uint32_t parent_bid = 0, slot = 0;
+  bool is_root = false;
+  if (j < s.nsp) {
+    L = s.at(0, j);
+    rep = L != 0 && a.repof[j] == j;
+  }
+  if (rep) {
+    lo = s.prev_less(j, L);
+    hi = s.next_less(j, L);
+    uint32_t La = lo >= 0 ? s.at(0, (uint32_t)lo) : 0u;
+    uint32_t Lb = hi < s.nsp ? s.at(0, hi) : 0u;
+    uint32_t e0 = (uint32_t)(lo + 1);
+    ek = elem_key(a, e0);
+    uint32_t depth = L - 1;
+    if (La == 0 && Lb == 0) {
+      is_root = true;
+      gi = a.ginfo[a.e_grp[e0]];
+      lo_depth = gi.base;
+    } else {
+      uint32_t p = La >= Lb ? (uint32_t)lo : hi;
+      uint32_t Lp = max(La, Lb) - 1;
+      parent_bid = a.bid[a.repof[p]];
+      slot = elem_nib(a.P, ek, Lp);
+      lo_depth = Lp + 1;
+    }
+    need_ext = depth > lo_depth;
+  }
+  uint32_t sid = wave_alloc(&a.P.cnt->n_short, need_ext);
+  ref_t me = REF_NULL;
+  if (rep) {
+    ref_t bref = make_ref(T_BRANCH, a.bid[j]);
+    me = bref;
+    if (need_ext) {
+      if (sid >= a.P.cap_short) {
+        raise_error(a.P.cnt, ERR_POOL, 2);
+        need_ext = false;
+      } else {
+        uint32_t depth = L - 1;
+        init_short(a.P.shrt[sid], ek.off, ek.start + (lo_depth - ek.origin), depth - lo_depth, lo_depth, 0, bref);
+        me = make_ref(T_SHORT, sid);
+      }
+    }
+    if (is_root)
+      install(a, gi, me);
+    else
+      a.P.branch[parent_bid].child[slot] = me;
+  }
+  mark_new_dirty(a.P, me, rep && need_ext);
+}
+
+// Elements: new keys become a leaf short node + value node; a split short
+// node's remainder is re-attached below the new branch.
+__global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
+  uint32_t e = grid_tid();
+  ST s = st_of(a);
+  uint32_t ne = a.scal[SC_NE];
+  bool active = e < ne;
+  ElemKey ek;
+  uint32_t start = 0, slot = 0, parent_bid = 0;
+  bool is_root = false, need_short = false, need_value = false;
+  GroupInfo gi;
+  if (active) {
+    ek = elem_key(a, e);
+    uint32_t La = e > 0 ? s.at(0, e - 1) : 0u;
+    uint32_t Lb = e < s.nsp ? s.at(0, e) : 0u;
+    if (La == 0 && Lb == 0) {
+      is_root = true;
+      gi = a.ginfo[a.e_grp[e]];
+      start = gi.base;
+    } else {
+      uint32_t p = La >= Lb ? e - 1 : e;
+      uint32_t Lp = max(La, Lb) - 1;
+      parent_bid = a.bid[a.repof[p]];
+      slot = elem_nib(a.P, ek, Lp);
+      start = Lp + 1;
+    }
+    need_short = start < ek.end;
+    need_value = !ek.virt;
+  }
+  uint32_t vid = wave_alloc(&a.P.cnt->n_value, need_value);
+  uint32_t sid = wave_alloc(&a.P.cnt->n_short, need_short);
+  ref_t me = REF_NULL;
+  if (active) {
+    ref_t child;
+    if (ek.virt) {
+      child = a.P.shrt[ek.uidx].child;
+    } else {
+      if (vid >= a.P.cap_value) {
+        raise_error(a.P.cnt, ERR_POOL, 3);
+        vid = 0;
+      }
+      uint32_t idx = a.u_idx[ek.uidx];
+      Value &V = a.P.value[vid];
+      int64_t vs = a.vidx[2 * idx], ve = a.vidx[2 * idx + 1];
+      V.addr = a.vbase + (uint64_t)vs;
+      V.hp = a.hps ? a.hps[idx] : 0ull;
+      V.size = (uint32_t)(ve - vs + 1);
+      V.pad = 0;
+      V.seq = 0;
+      child = make_ref(T_VALUE, vid);
+    }
+    me = child;
+    if (need_short) {
+      if (sid >= a.P.cap_short) {
+        raise_error(a.P.cnt, ERR_POOL, 2);
+        need_short = false;
+      } else {
+        init_short(a.P.shrt[sid], ek.off, ek.start + (start - ek.origin), ek.end - start, start, ek.term, child);
+        me = make_ref(T_SHORT, sid);
+      }
+    }
+    if (is_root)
+      install(a, gi, me);
+    else
+      a.P.branch[parent_bid].child[slot] = me;
+  }
+  mark_new_dirty(a.P, me, active && need_short);
+}
+
+// Existing keys: swap in a new value node (runs before the links, so a leaf
+// that is also split carries the new value into its remainder).
+__global__ __launch_bounds__(256) void k_updates(NuArgs a) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  bool upd = false;
+  Stop st;
+  if (i < m) {
+    st = a.stop[i];
+    upd = is_update(st.kind);
+  }
+  uint32_t vid = wave_alloc(&a.P.cnt->n_value, upd);
+  if (!upd) return;
+  if (vid >= a.P.cap_value) {
+    raise_error(a.P.cnt, ERR_POOL, 3);
+    return;
+  }
+  uint32_t idx = a.u_idx[i];
+  Value &V = a.P.value[vid];
+  int64_t vs = a.vidx[2 * idx], ve = a.vidx[2 * idx + 1];
+  V.addr = a.vbase + (uint64_t)vs;
+  V.hp = a.hps ? a.hps[idx] : 0ull;
+  V.size = (uint32_t)(ve - vs + 1);
+  V.pad = 0;
+  V.seq = 0;
+  ref_t vr = make_ref(T_VALUE, vid);
+  if (st.kind == STOP_LEAF_UPDATE)
+    a.P.shrt[ref_idx(st.stop)].child = vr;
+  else
+    a.P.branch[ref_idx(st.stop)].child[16] = vr;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- host side --
+
+hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s) {
+  k_keys_scan<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u64(a.kpoff, a.kpoff, a.n, (uint64_t *)(a.scal + SC_KTOT), temp, s);
+  if (e != hipSuccess) return e;
+  k_keys_pack<<<nblk(a.n), kB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
+                   uint32_t **sv) {
+  uint64_t *k[2] = {a.sk0, a.sk1};
+  uint32_t *v[2] = {a.sv0, a.sv1};
+  int cur = 0, which = 0;
+  hipError_t e;
+  if (!full) {
+    e = radix_sort_u64(k[0], v[0], k[1], v[1], a.n, 0, 64, &which, temp, s);
+    if (e != hipSuccess) return e;
+    cur = which;
+  } else {
+    // LSD over (words..., length): length first, then the words from last to first
+    int nwords = full;
+    for (int w = -1; w < nwords; w++) {
+      int word = w < 0 ? -1 : nwords - 1 - w;
+      k_gather_word<<<nblk(a.n), kB, 0, s>>>(a, v[cur], k[cur], word);
+      GMPT_LAUNCH_CHECK();
+      e = radix_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, 0, word < 0 ? 16 : 64, &which, temp, s);
+      if (e != hipSuccess) return e;
+      cur ^= which;
+    }
+  }
+  if (trie_bits > 0) {
+    // stable sort of the current permutation by trie id, then regather the
+    // windows in the new order
+    uint32_t *t[2] = {a.tk0, a.tk1};
+    k_gather_trie<<<nblk(a.n), kB, 0, s>>>(a, v[cur], t[0]);
+    GMPT_LAUNCH_CHECK();
+    e = radix_sort_u32(t[0], v[cur], t[1], v[cur ^ 1], a.n, 0, (int)((trie_bits + 7) & ~7u), &which, temp, s);
+    if (e != hipSuccess) return e;
+    if (which) cur ^= 1;
+    if (!full) {
+      k_gather_word<<<nblk(a.n), kB, 0, s>>>(a, v[cur], k[cur], 0);
+      GMPT_LAUNCH_CHECK();
+    }
+  }
+  *sk = k[cur];
+  *sv = v[cur];
+  return hipSuccess;
+}
+
+hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s) {
+  k_adjacent<<<nblk(a.n), kB, 0, s>>>(a, sk, sv, full);
+  return hipGetLastError();
+}
+
+hipError_t nu_unique(const NuArgs &a, const uint32_t *sv, void *temp, hipStream_t s) {
+  // run[] holds run-start flags; exclusive scan gives each start its run id
+  hipError_t e = scan_exclusive_u32(a.run, a.run, a.n, nullptr, temp, s);
+  if (e != hipSuccess) return e;
+  k_unique<<<nblk(a.n), kB, 0, s>>>(a, sv, a.run);
+  return hipGetLastError();
+}
+
+static hipError_t nu_groups(const NuArgs &a, void *temp, hipStream_t s) {
+  k_group_flags<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u32(a.ecount, a.grp, a.n, nullptr, temp, s);
+  if (e != hipSuccess) return e;
+  k_group_fix<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  k_group_info<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  k_group_split<<<nblk(a.n), kB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s) {
+  k_descend<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = nu_groups(a, temp, s);
+  if (e != hipSuccess) return e;
+  k_group_merge<<<nblk(a.n), kB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t nu_regroup(NuArgs &a, uint32_t m, uint32_t nruns, void *temp, hipStream_t s) {
+  k_gkey<<<nblk(m), kB, 0, s>>>(a, m);
+  GMPT_LAUNCH_CHECK();
+  int bits = 8;
+  while (bits < 32 && (1ull << bits) < nruns) bits += 8;
+  int which = 0;
+  hipError_t e = radix_sort_u32(a.gkey0, a.perm0, a.gkey1, a.perm1, m, 0, bits, &which, temp, s);
+  if (e != hipSuccess) return e;
+  k_permute<<<nblk(m), kB, 0, s>>>(a, which ? a.perm1 : a.perm0, m);
+  GMPT_LAUNCH_CHECK();
+  uint32_t *t;
+  t = a.u_idx; a.u_idx = a.u_idx2; a.u_idx2 = t;
+  t = a.u_trie; a.u_trie = a.u_trie2; a.u_trie2 = t;
+  uint8_t *l = a.u_lcp; a.u_lcp = a.u_lcp2; a.u_lcp2 = l;
+  Stop *st = a.stop; a.stop = a.stop2; a.stop2 = st;
+  return nu_groups(a, temp, s);
+}
+
+hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s) {
+  k_emit_count<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u32(a.ecount, a.ecount, a.n, nullptr, temp, s);
+  if (e != hipSuccess) return e;
+  k_emit_write<<<nblk(a.n), kB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t nu_build(const NuArgs &a, hipStream_t s) {
+  size_t ne_max = 2 * (size_t)a.n;
+  for (int k = 1; k < a.st_levels; k++) {
+    k_st_level<<<nblk(ne_max), kB, 0, s>>>(a, k);
+    GMPT_LAUNCH_CHECK();
+  }
+  k_updates<<<nblk(a.n), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  k_link_branches<<<nblk(ne_max), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  k_link_elems<<<nblk(ne_max), kB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace gmpt

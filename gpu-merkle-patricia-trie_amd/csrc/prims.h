@@ -1,0 +1,62 @@
+// prims.h -- device-wide scan and LSD radix sort (hand-written, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gmpt {
+
+// Bytes of temporary storage needed by the primitives below for n elements.
+size_t scan_temp_bytes(size_t n);
+size_t radix_temp_bytes(size_t n);
+
+// out[i] = sum(in[0..i)) ; total (if not null) receives the sum of all n
+// (device pointer).  `in` and `out` may alias.
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, void *temp,
+                              hipStream_t s);
+hipError_t scan_exclusive_u64(const uint64_t *in, uint64_t *out, size_t n, uint64_t *total, void *temp,
+                              hipStream_t s);
+
+// Stable LSD sort of (keys, vals) by key bits [begin_bit, end_bit), 8 bits per
+// pass.  Buffers ping-pong between (k0,v0) and (k1,v1); returns in *which the
+// buffer index (0 or 1) holding the result.
+hipError_t radix_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int begin_bit,
+                          int end_bit, int *which, void *temp, hipStream_t s);
+hipError_t radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int begin_bit,
+                          int end_bit, int *which, void *temp, hipStream_t s);
+
+// ---------------------------------------------------------------- device --
+
+// Inclusive scan of one value per lane over a wave64.
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread (blockDim = 256).
+// `sh` must hold 4 elements of T.  Returns the exclusive prefix; *total gets
+// the block sum.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan_256(T v, T *sh, T *total) {
+  const int lane = __lane_id();
+  const int w = threadIdx.x >> 6;
+  T inc = wave_inclusive_scan(v);
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  T base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (i < w) base += sh[i];
+    tot += sh[i];
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return base + inc - v;
+}
+
+}  // namespace gmpt

@@ -1,0 +1,262 @@
+// prims.hip -- device-wide exclusive scan and stable LSD radix sort.
+//
+// Radix sort: reduce-then-scan with 8-bit digits.  Each block owns a tile of
+// 1024 items (256 threads x 4 items, striped so loads are coalesced).  The
+// scatter kernel ranks the tile stably in LDS (per-wave digit match via 8
+// ballots), sorts it locally, then writes runs of equal digits contiguously.
+#include "prims.h"
+#include "common.h"
+
+namespace gmpt {
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, size_t n, T *__restrict__ sums) {
+  __shared__ T sh[4];
+  size_t base = (size_t)blockIdx.x * kScanTile;
+  T acc = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
+    if (j < n) acc += in[j];
+  }
+  T tot;
+  block_exclusive_scan_256<T>(acc, sh, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *__restrict__ out, size_t n,
+                                                   const T *__restrict__ offsets, T *__restrict__ total) {
+  __shared__ T tile[kScanTile];
+  __shared__ T sh[4];
+  size_t base = (size_t)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
+    tile[i * kScanBlock + threadIdx.x] = j < n ? in[j] : T(0);
+  }
+  __syncthreads();
+  T v[kScanItems];
+  T acc = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    v[i] = tile[threadIdx.x * kScanItems + i];
+    acc += v[i];
+  }
+  T tot;
+  T pre = block_exclusive_scan_256<T>(acc, sh, &tot);
+  T off = offsets ? offsets[blockIdx.x] : T(0);
+  pre += off;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    tile[threadIdx.x * kScanItems + i] = pre;
+    pre += v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
+    if (j < n) out[j] = tile[i * kScanBlock + threadIdx.x];
+  }
+  if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = off + tot;
+}
+
+template <typename T>
+size_t scan_temp_elems(size_t n) {
+  size_t e = 0;
+  while (n > (size_t)kScanTile) {
+    n = (n + kScanTile - 1) / kScanTile;
+    e += n + 1;
+  }
+  return e + 1;
+}
+
+template <typename T>
+hipError_t scan_impl(const T *in, T *out, size_t n, T *total, T *temp, hipStream_t s) {
+  if (n == 0) {
+    if (total) return hipMemsetAsync(total, 0, sizeof(T), s);
+    return hipSuccess;
+  }
+  size_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 1) {
+    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, total);
+    return hipGetLastError();
+  }
+  T *sums = temp;
+  T *rest = temp + nb + 1;
+  k_scan_reduce<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, sums);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_impl<T>(sums, sums, nb, nullptr, rest, s);
+  if (e != hipSuccess) return e;
+  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, total);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ radix sort --
+
+constexpr int kRsBlock = 256;
+constexpr int kRsItems = 4;
+constexpr int kRsTile = kRsBlock * kRsItems;
+constexpr int kRsWaves = kRsBlock / 64;
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_rs_hist(const K *__restrict__ keys, size_t n, int shift,
+                                                 uint32_t *__restrict__ hist, uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  size_t base = (size_t)blockIdx.x * kRsTile;
+#pragma unroll
+  for (int r = 0; r < kRsItems; r++) {
+    size_t j = base + (size_t)r * kRsBlock + threadIdx.x;
+    if (j < n) atomicAdd(&h[(uint32_t)(keys[j] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_rs_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                    K *__restrict__ kout, uint32_t *__restrict__ vout, size_t n,
+                                                    int shift, const uint32_t *__restrict__ goff,
+                                                    uint32_t nblocks) {
+  __shared__ uint16_t cnt[kRsItems][kRsWaves][256];
+  __shared__ K sk[kRsTile];
+  __shared__ uint32_t sv[kRsTile];
+  __shared__ uint32_t tstart[256];
+  __shared__ uint32_t shs[4];
+
+  const int t = threadIdx.x;
+  const int lane = __lane_id();
+  const int w = t >> 6;
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  size_t base = (size_t)blockIdx.x * kRsTile;
+
+  for (int i = t; i < kRsItems * kRsWaves * 256; i += kRsBlock) (&cnt[0][0][0])[i] = 0;
+  __syncthreads();
+
+  K key[kRsItems];
+  uint32_t val[kRsItems], dig[kRsItems], wrank[kRsItems];
+  bool valid[kRsItems];
+#pragma unroll
+  for (int r = 0; r < kRsItems; r++) {
+    size_t j = base + (size_t)r * kRsBlock + t;
+    valid[r] = j < n;
+    key[r] = valid[r] ? kin[j] : K(0);
+    val[r] = valid[r] ? vin[j] : 0u;
+    dig[r] = (uint32_t)(key[r] >> shift) & 255u;
+    uint64_t m = __ballot(valid[r]);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t bb = __ballot((dig[r] >> b) & 1u);
+      m &= ((dig[r] >> b) & 1u) ? bb : ~bb;
+    }
+    wrank[r] = (uint32_t)__popcll(m & lt_mask);
+    // the lowest lane of each digit group records the group's size
+    if (valid[r] && (m & lt_mask) == 0) cnt[r][w][dig[r]] = (uint16_t)__popcll(m);
+  }
+  __syncthreads();
+  // per digit: exclusive prefix over (round, wave) in stable order
+  {
+    const int d = t;
+    uint32_t run = 0;
+#pragma unroll
+    for (int r = 0; r < kRsItems; r++)
+#pragma unroll
+      for (int ww = 0; ww < kRsWaves; ww++) {
+        uint32_t c = cnt[r][ww][d];
+        cnt[r][ww][d] = (uint16_t)run;
+        run += c;
+      }
+    uint32_t tot;
+    uint32_t st = block_exclusive_scan_256<uint32_t>(run, shs, &tot);
+    tstart[d] = st;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRsItems; r++) {
+    if (valid[r]) {
+      uint32_t pos = tstart[dig[r]] + cnt[r][w][dig[r]] + wrank[r];
+      sk[pos] = key[r];
+      sv[pos] = val[r];
+    }
+  }
+  __syncthreads();
+  size_t valid_n = n - base < (size_t)kRsTile ? n - base : (size_t)kRsTile;
+#pragma unroll
+  for (int r = 0; r < kRsItems; r++) {
+    uint32_t i = r * kRsBlock + t;
+    if (i < valid_n) {
+      K k = sk[i];
+      uint32_t d = (uint32_t)(k >> shift) & 255u;
+      size_t dst = (size_t)goff[(size_t)d * nblocks + blockIdx.x] + (i - tstart[d]);
+      kout[dst] = k;
+      vout[dst] = sv[i];
+    }
+  }
+}
+
+template <typename K>
+hipError_t radix_impl(K *k0, uint32_t *v0, K *k1, uint32_t *v1, size_t n, int begin_bit, int end_bit, int *which,
+                      void *temp, hipStream_t s) {
+  *which = 0;
+  if (n <= 1) return hipSuccess;
+  uint32_t nb = (uint32_t)((n + kRsTile - 1) / kRsTile);
+  uint32_t *hist = (uint32_t *)temp;
+  uint32_t *scan_tmp = hist + (size_t)256 * nb + 16;
+  K *kin = k0, *kout = k1;
+  uint32_t *vin = v0, *vout = v1;
+  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+    k_rs_hist<K><<<nb, kRsBlock, 0, s>>>(kin, n, shift, hist, nb);
+    GMPT_LAUNCH_CHECK();
+    hipError_t e = scan_impl<uint32_t>(hist, hist, (size_t)256 * nb, nullptr, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    k_rs_scatter<K><<<nb, kRsBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
+    GMPT_LAUNCH_CHECK();
+    K *tk = kin;
+    kin = kout;
+    kout = tk;
+    uint32_t *tv = vin;
+    vin = vout;
+    vout = tv;
+    *which ^= 1;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+size_t scan_temp_bytes(size_t n) { return scan_temp_elems<uint64_t>(n) * sizeof(uint64_t) + 64; }
+
+size_t radix_temp_bytes(size_t n) {
+  size_t nb = (n + kRsTile - 1) / kRsTile + 1;
+  return (256 * nb + 16) * sizeof(uint32_t) + scan_temp_bytes(256 * nb) + 256;
+}
+
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, void *temp,
+                              hipStream_t s) {
+  return scan_impl<uint32_t>(in, out, n, total, (uint32_t *)temp, s);
+}
+
+hipError_t scan_exclusive_u64(const uint64_t *in, uint64_t *out, size_t n, uint64_t *total, void *temp,
+                              hipStream_t s) {
+  return scan_impl<uint64_t>(in, out, n, total, (uint64_t *)temp, s);
+}
+
+hipError_t radix_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int begin_bit,
+                          int end_bit, int *which, void *temp, hipStream_t s) {
+  return radix_impl<uint64_t>(k0, v0, k1, v1, n, begin_bit, end_bit, which, temp, s);
+}
+
+hipError_t radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int begin_bit,
+                          int end_bit, int *which, void *temp, hipStream_t s) {
+  return radix_impl<uint32_t>(k0, v0, k1, v1, n, begin_bit, end_bit, which, temp, s);
+}
+
+}  // namespace gmpt

@@ -1,0 +1,322 @@
+"""ctypes binding of libgmpt_amd.so (include/gmpt_amd.h, include/libgmpt.h).
+
+``MPT`` mirrors the reference's host class GpuMPT::Compress::MPT
+(mpt-with-compress/include/mpt/gpu_mpt.cuh:18-192): the same method names and
+argument meaning, with batches passed as ``workloads.Batch``.  Every call goes
+through the C-ABI into the HIP kernels; there is no CPU path, and loading
+fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgmpt_amd.so")
+_lib = None
+
+
+class GmptError(RuntimeError):
+    pass
+
+
+class _Batch(ctypes.Structure):
+    _fields_ = [("keys_hexs", ctypes.c_void_p), ("keys_indexs", ctypes.c_void_p),
+                ("values_bytes", ctypes.c_void_p), ("values_indexs", ctypes.c_void_p),
+                ("values_hps", ctypes.c_void_p), ("trie_ids", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("keys_bytes", ctypes.c_int64), ("values_bytes_total", ctypes.c_int64),
+                ("on_device", ctypes.c_int), ("adopt_values", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("last_hash_ms", ctypes.c_double), ("hash_launches", ctypes.c_uint32),
+                ("hashed_nodes", ctypes.c_uint32), ("restarts", ctypes.c_uint32), ("lock_fails", ctypes.c_uint32),
+                ("fallback_sorts", ctypes.c_uint32), ("regroups", ctypes.c_uint32), ("perms", ctypes.c_uint64),
+                ("enc_bytes", ctypes.c_uint64), ("pool_bytes_used", ctypes.c_uint64),
+                ("pool_bytes_reserved", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol the headers in include/ declare
+GMPT_SYMBOLS = ["gmpt_forest_create", "gmpt_forest_destroy", "gmpt_last_error", "gmpt_puts_2phase",
+                "gmpt_puts_latching", "gmpt_hash_onepass", "gmpt_get_root_hash", "gmpt_gets", "gmpt_get_values",
+                "gmpt_export_nodes", "gmpt_free_host", "gmpt_reset_trie", "gmpt_forest_clear", "gmpt_synchronize", "gmpt_get_stats",
+                "gmpt_device_count", "gmpt_device_malloc", "gmpt_device_free", "gmpt_memcpy_htod",
+                "gmpt_memcpy_dtoh", "gmpt_timer_start", "gmpt_timer_stop", "gmpt_keccak256_batch"]
+LIBGMPT_SYMBOLS = ["build_mpt_2phase", "build_mpt_olc", "preprocess", "get_all_nodes", "init_mpt",
+                   "insert_mpt_2phase", "insert_mpt_olc", "free_mpt"]
+
+
+class _NodeSet(ctypes.Structure):
+    _fields_ = [("hashs", ctypes.c_void_p), ("encs", ctypes.c_void_p), ("encs_indexs", ctypes.c_void_p),
+                ("num", ctypes.c_ulonglong)]
+
+
+def lib():
+    """Load libgmpt_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GmptError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, u32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_int64
+    sig = {
+        "gmpt_forest_create": ([i32, u32, ctypes.POINTER(vp)], i32),
+        "gmpt_forest_destroy": ([vp], None),
+        "gmpt_last_error": ([], ctypes.c_char_p),
+        "gmpt_puts_2phase": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(i32)], i32),
+        "gmpt_puts_latching": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(i32)], i32),
+        "gmpt_hash_onepass": ([vp, vp, i32], i32),
+        "gmpt_get_root_hash": ([vp, u32, vp, ctypes.POINTER(i32)], i32),
+        "gmpt_gets": ([vp, ctypes.POINTER(_Batch), vp, vp], i32),
+        "gmpt_get_values": ([vp, ctypes.POINTER(_Batch), vp, i64, vp, vp], i32),
+        "gmpt_export_nodes": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                               ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64)], i32),
+        "gmpt_free_host": ([vp], None),
+        "gmpt_reset_trie": ([vp, u32], i32),
+        "gmpt_forest_clear": ([vp], i32),
+        "gmpt_synchronize": ([vp], i32),
+        "gmpt_get_stats": ([vp, ctypes.POINTER(Stats)], i32),
+        "gmpt_device_count": ([ctypes.POINTER(i32)], i32),
+        "gmpt_device_malloc": ([ctypes.POINTER(vp), ctypes.c_size_t], i32),
+        "gmpt_device_free": ([vp], i32),
+        "gmpt_memcpy_htod": ([vp, vp, ctypes.c_size_t], i32),
+        "gmpt_memcpy_dtoh": ([vp, vp, ctypes.c_size_t], i32),
+        "gmpt_timer_start": ([vp], i32),
+        "gmpt_timer_stop": ([vp, ctypes.POINTER(ctypes.c_float)], i32),
+        "gmpt_keccak256_batch": ([vp, vp, vp, u32, vp], i32),
+        "build_mpt_2phase": ([i32, vp, vp, vp, vp, vp, i32], vp),
+        "build_mpt_olc": ([i32, vp, vp, vp, vp, vp, i32], vp),
+        "preprocess": ([], i32),
+        "get_all_nodes": ([i32, vp, vp, i32], _NodeSet),
+        "init_mpt": ([], vp),
+        "insert_mpt_2phase": ([vp, vp, vp, vp, vp, vp, i32], vp),
+        "insert_mpt_olc": ([vp, vp, vp, vp, vp, vp, i32], vp),
+        "free_mpt": ([vp], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise GmptError(lib().gmpt_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def host_batch(b, hps=None):
+    """A gmpt_batch struct over a workloads.Batch held in host memory."""
+    s = _Batch()
+    s.keys_hexs, s.keys_indexs = _p(b.keys_hexs), _p(b.keys_indexs)
+    s.values_bytes, s.values_indexs = _p(b.values_bytes), _p(b.values_indexs)
+    s.values_hps = _p(hps)
+    s.trie_ids = _p(b.trie_ids)
+    s.n = b.n
+    s._keep = (b, hps)
+    return s
+
+
+class DeviceBuffer:
+    """A device allocation made through the library (no other runtime needed)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().gmpt_device_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes:
+            _check(lib().gmpt_memcpy_htod(d.ptr, _p(a), a.nbytes))
+        return d
+
+    def to_array(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(lib().gmpt_memcpy_dtoh(_p(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gmpt_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceBatch:
+    """A workloads.Batch uploaded to HBM; .struct is the device gmpt_batch."""
+
+    def __init__(self, b, adopt_values=False):
+        self.bufs = [DeviceBuffer.from_array(x) for x in (b.keys_hexs, b.keys_indexs, b.values_bytes,
+                                                         b.values_indexs)]
+        self.tid = DeviceBuffer.from_array(b.trie_ids) if b.trie_ids is not None else None
+        s = _Batch()
+        s.keys_hexs, s.keys_indexs, s.values_bytes, s.values_indexs = [x.ptr for x in self.bufs]
+        s.values_hps = None
+        s.trie_ids = self.tid.ptr if self.tid else None
+        s.n = b.n
+        s.keys_bytes = b.key_bytes
+        s.values_bytes_total = b.value_bytes
+        s.on_device = 1
+        s.adopt_values = 1 if adopt_values else 0
+        self.struct = s
+        self.n = b.n
+
+    def free(self):
+        for x in self.bufs:
+            x.free()
+        if self.tid:
+            self.tid.free()
+
+
+class MPT:
+    """GpuMPT::Compress::MPT over a forest of `num_tries` tries on one device."""
+
+    def __init__(self, device=0, num_tries=1):
+        self._f = ctypes.c_void_p()
+        _check(lib().gmpt_forest_create(device, num_tries, ctypes.byref(self._f)))
+        self.num_tries = num_tries
+
+    def close(self):
+        if self._f and self._f.value:
+            lib().gmpt_forest_destroy(self._f)
+            self._f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- inserts (gpu_mpt.cuh:69-71, 95-97) -----------------------------------
+    def _puts(self, fn, batch, hps=None):
+        s = batch.struct if isinstance(batch, DeviceBatch) else host_batch(batch, hps)
+        t = ctypes.c_void_p()
+        n = ctypes.c_int()
+        _check(fn(self._f, ctypes.byref(s), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def puts_2phase_with_valuehp(self, batch, values_hps=None):
+        """PhaseNU: returns (hash targets, count) for hash_onepass_v2."""
+        return self._puts(lib().gmpt_puts_2phase, batch, values_hps)
+
+    def puts_latching_with_valuehp_v2(self, batch, values_hps=None):
+        """LockNU: returns (hash targets, count) for hash_onepass_v2."""
+        return self._puts(lib().gmpt_puts_latching, batch, values_hps)
+
+    # --- hashing (gpu_mpt.cuh:131) --------------------------------------------
+    def hash_onepass_v2(self, targets, n):
+        _check(lib().gmpt_hash_onepass(self._f, targets, n))
+
+    def get_root_hash(self, trie=0):
+        """Root hash bytes (32, or the raw root encoding when < 32 B, b'' if empty)."""
+        out = np.zeros(32, np.uint8)
+        sz = ctypes.c_int()
+        _check(lib().gmpt_get_root_hash(self._f, trie, _p(out), ctypes.byref(sz)))
+        return out[:sz.value].tobytes()
+
+    # --- gets (gpu_mpt.cuh:139-140) -------------------------------------------
+    def gets_parallel(self, batch):
+        """(host pointers recorded at insert, sizes); absent keys give (0, 0)."""
+        n = batch.n
+        hps = np.zeros(max(n, 1), np.uint64)
+        sizes = np.zeros(max(n, 1), np.int32)
+        _check(lib().gmpt_gets(self._f, ctypes.byref(host_batch(batch)), _p(hps), _p(sizes)))
+        return hps[:n], sizes[:n]
+
+    def get_values(self, batch):
+        """Value bytes per key (None when absent), read from HBM."""
+        n = batch.n
+        offs = np.zeros(max(n, 1), np.int64)
+        sizes = np.zeros(max(n, 1), np.int32)
+        cap = 1 << 20
+        while True:
+            out = np.zeros(cap, np.uint8)
+            rc = lib().gmpt_get_values(self._f, ctypes.byref(host_batch(batch)), _p(out), cap, _p(offs), _p(sizes))
+            if rc == 0:
+                break
+            msg = lib().gmpt_last_error().decode()
+            if "too small" not in msg:
+                raise GmptError(msg)
+            cap *= 4
+        return [None if sizes[i] < 0 else out[offs[i]:offs[i] + sizes[i]].tobytes() for i in range(n)]
+
+    # --- node export (gpu_mpt.cuh:146-149) ------------------------------------
+    def flush_dirty_nodes(self, batch):
+        """[(32-byte hash slot, RLP encoding)] of every node on the keys' paths."""
+        h, e, ix = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        num = ctypes.c_uint64()
+        _check(lib().gmpt_export_nodes(self._f, ctypes.byref(host_batch(batch)), ctypes.byref(h), ctypes.byref(e),
+                                       ctypes.byref(ix), ctypes.byref(num)))
+        n = num.value
+        out = []
+        if n:
+            hashes = np.ctypeslib.as_array(ctypes.cast(h, ctypes.POINTER(ctypes.c_uint8)), shape=(32 * n,)).copy()
+            idx = np.ctypeslib.as_array(ctypes.cast(ix, ctypes.POINTER(ctypes.c_uint64)), shape=(2 * n,)).copy()
+            tot = int(idx[-1]) + 1
+            encs = np.ctypeslib.as_array(ctypes.cast(e, ctypes.POINTER(ctypes.c_uint8)), shape=(tot,)).copy()
+            for i in range(n):
+                out.append((hashes[32 * i:32 * i + 32].tobytes(), encs[idx[2 * i]:idx[2 * i + 1] + 1].tobytes()))
+        for p in (h, e, ix):
+            if p.value:
+                lib().gmpt_free_host(p)
+        return out
+
+    def reset_trie(self, trie):
+        _check(lib().gmpt_reset_trie(self._f, trie))
+
+    def clear(self):
+        _check(lib().gmpt_forest_clear(self._f))
+
+    def synchronize(self):
+        _check(lib().gmpt_synchronize(self._f))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().gmpt_get_stats(self._f, ctypes.byref(s)))
+        return s.as_dict()
+
+    def timer_start(self):
+        _check(lib().gmpt_timer_start(self._f))
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        _check(lib().gmpt_timer_stop(self._f, ctypes.byref(ms)))
+        return ms.value
+
+
+def keccak256_batch(messages):
+    """Keccak-256 of each message, computed on the GPU."""
+    n = len(messages)
+    lens = np.array([len(m) for m in messages], np.uint32)
+    offs = np.zeros(n, np.uint64)
+    if n:
+        offs[1:] = np.cumsum(lens[:-1].astype(np.uint64))
+    data = np.frombuffer(b"".join(messages) + b"\0" * 8, np.uint8).copy()
+    out = np.zeros(32 * max(n, 1), np.uint8)
+    _check(lib().gmpt_keccak256_batch(_p(data), _p(offs), _p(lens), n, _p(out)))
+    return [out[32 * i:32 * i + 32].tobytes() for i in range(n)]
+
+
+def device_count():
+    c = ctypes.c_int()
+    _check(lib().gmpt_device_count(ctypes.byref(c)))
+    return c.value

@@ -1,0 +1,249 @@
+"""GPU parity: PhaseNU / LockNU / PhaseHC / get / node export through the C-ABI,
+compared bit-for-bit with the CPU oracle on the same seeded inputs.
+
+Model: the reference's relative CPU==GPU root checks (mpt-with-compress/test/
+trie.cu:1451, 4093), its get tests (trie.cu:153-197) and override test
+(trie.cu:293-328, where this build fixes last-writer-wins on the GPU too).
+"""
+import ctypes
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+MODES = ["2phase", "latching"]
+
+
+def _put(t, b, mode, hps=None):
+    fn = t.puts_2phase_with_valuehp if mode == "2phase" else t.puts_latching_with_valuehp_v2
+    targets, n = fn(b, hps)
+    t.hash_onepass_v2(targets, n)
+
+
+def _build(gmpt, b, mode):
+    t = gmpt.MPT(device=0)
+    _put(t, b, mode)
+    return t
+
+
+def _rand_batch(workloads, seed, n, vlo, vhi, key_len=32):
+    rng = np.random.default_rng(seed)
+    keys = workloads.unique_random_keys(rng, n, key_len)
+    flat, lens = workloads.random_values(rng, n, vlo, vhi)
+    return workloads.batch_from_fixed(keys, flat, lens)
+
+
+def test_keccak_matches_oracle(gmpt, oracle):
+    with open(os.path.join(GOLDEN, "keccak256.json")) as f:
+        vecs = json.load(f)
+    msgs = [bytes.fromhex(v["msg"]) for v in vecs]
+    got = gmpt.keccak256_batch(msgs)
+    assert [g.hex() for g in got] == [v["digest"] for v in vecs]
+    rng = random.Random(3)
+    msgs = [bytes(rng.getrandbits(8) for _ in range(L)) for L in list(range(0, 420)) + [543, 544, 1087, 1088, 2000]]
+    got = gmpt.keccak256_batch(msgs)
+    for m, g in zip(msgs, got):
+        assert g == oracle.keccak256(m), len(m)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_doe_dog_dogglesworth(gmpt, workloads, mode):
+    with open(os.path.join(GOLDEN, "roots.json")) as f:
+        g = json.load(f)
+    b = workloads.batch_from_bytes([b"doe", b"dog", b"dogglesworth"], [b"reindeer", b"puppy", b"cat"])
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash().hex() == g["doe_dog_dogglesworth"]
+    assert t.get_values(b) == [b"reindeer", b"puppy", b"cat"]
+    miss = workloads.batch_from_bytes([b"do", b"dogg", b"cat"], [b"", b"", b""])
+    assert t.get_values(miss) == [None, None, None]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_gets_return_host_pointers(gmpt, workloads, mode):
+    # gets_parallel returns the host pointer handed over at insert
+    # (gpu_mpt_kernels.cuh:2285-2290); read the bytes back through it.
+    b = workloads.batch_from_bytes([b"doe", b"dog", b"dogglesworth"], [b"reindeer", b"puppy", b"cat"])
+    base = b.values_bytes.ctypes.data
+    hps = np.array([base + int(b.values_indexs[2 * i]) for i in range(b.n)], np.uint64)
+    t = gmpt.MPT(device=0)
+    _put(t, b, mode, hps)
+    ptrs, sizes = t.gets_parallel(b)
+    for i in range(b.n):
+        assert ctypes.string_at(int(ptrs[i]), int(sizes[i])) == b.value(i)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_override_last_writer_wins(gmpt, workloads, mode):
+    b = workloads.batch_from_bytes([b"dog", b"dog", b"dogglesworth"], [b"reindeer", b"puppy", b"cat"])
+    t = _build(gmpt, b, mode)
+    assert t.get_values(b) == [b"puppy", b"puppy", b"cat"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_config1_golden_root(gmpt, workloads, mode):
+    with open(os.path.join(GOLDEN, "roots.json")) as f:
+        g = json.load(f)
+    t = _build(gmpt, workloads.config1_oracle(), mode)
+    assert t.get_root_hash().hex() == g["config1"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,vlo,vhi", [(1, 40, 40), (2, 1, 3), (3, 33, 300), (17, 1, 40), (1000, 0, 8),
+                                       (5000, 70, 110), (3000, 130, 560), (20000, 1, 200)])
+def test_random_vs_oracle(gmpt, workloads, oracle, mode, n, vlo, vhi):
+    b = _rand_batch(workloads, 100 + n, n, vlo, vhi)
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash() == oracle.root_of(b)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_variable_length_keys(gmpt, workloads, oracle, mode):
+    rng = random.Random(17)
+    keys = set()
+    while len(keys) < 1500:
+        keys.add(bytes(rng.choice([0, 1, 2, 0x10, 0xff]) for _ in range(rng.randint(1, 6))))
+    keys = list(keys)
+    rng.shuffle(keys)
+    vals = [bytes([rng.getrandbits(8)]) * rng.randint(1, 70) for _ in keys]
+    b = workloads.batch_from_bytes(keys, vals)
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash() == oracle.root_of(b)
+    assert t.get_values(b) == vals
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_duplicates_in_batch(gmpt, workloads, oracle, mode):
+    rng = np.random.default_rng(9)
+    keys = workloads.unique_random_keys(rng, 400, 32)
+    idx = rng.integers(0, 400, size=3000)
+    flat, lens = workloads.random_values(rng, 3000, 30, 90)
+    b = workloads.batch_from_fixed(keys[idx], flat, lens)
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash() == oracle.root_of(b)
+    last = {}
+    for i in range(b.n):
+        last[b.key(i)] = b.value(i)
+    assert t.get_values(b) == [last[b.key(i)] for i in range(b.n)]
+
+
+@pytest.mark.parametrize("modes", [("2phase", "2phase"), ("latching", "latching"), ("2phase", "latching"),
+                                   ("latching", "2phase")])
+def test_incremental_batches(gmpt, workloads, oracle, modes):
+    rng = np.random.default_rng(5)
+    base = _rand_batch(workloads, 5, 6000, 30, 120)
+    # second batch: 1500 overwrites of existing keys + 2500 new keys, shuffled
+    upd = base.subset(rng.choice(6000, 1500, replace=False))
+    fresh = _rand_batch(workloads, 55, 2500, 1, 200)
+    flat, lens = workloads.random_values(rng, 1500, 40, 80)
+    upd = workloads.batch_from_hex([upd.key(i) for i in range(upd.n)],
+                                   [flat[int(lens[:i].sum()):int(lens[:i + 1].sum())].tobytes() for i in range(1500)])
+    second = workloads.concat_batches([upd, fresh])
+    second = second.subset(rng.permutation(second.n))
+    t = gmpt.MPT(device=0)
+    _put(t, base, modes[0])
+    o = oracle.OracleMPT()
+    o.puts(base)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    _put(t, second, modes[1])
+    o.puts(second)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    sample = second.subset(range(0, second.n, 37))
+    assert t.get_values(sample) == o.gets(sample)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_many_small_batches_split_shorts(gmpt, workloads, oracle, mode):
+    # keys sharing long prefixes split existing short nodes at every depth
+    rng = random.Random(23)
+    keys = sorted({bytes([0xAB, 0xCD]) + bytes(rng.choice([0, 1, 0x11]) for _ in range(rng.randint(1, 8)))
+                   for _ in range(600)})
+    rng.shuffle(keys)
+    vals = [bytes([i % 251]) * (33 + i % 40) for i in range(len(keys))]
+    t = gmpt.MPT(device=0)
+    o = oracle.OracleMPT()
+    for lo in range(0, len(keys), 50):
+        b = workloads.batch_from_bytes(keys[lo:lo + 50], vals[lo:lo + 50])
+        _put(t, b, mode)
+        o.puts(b)
+        o.hash()
+        assert t.get_root_hash() == o.root(), lo
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_node_export_matches_oracle(gmpt, workloads, oracle, mode):
+    b = _rand_batch(workloads, 77, 3000, 20, 150)
+    t = _build(gmpt, b, mode)
+    got = t.flush_dirty_nodes(b)
+    o = oracle.OracleMPT()
+    o.puts(b)
+    o.hash()
+    want = [(h + bytes(32 - len(h)), e) for h, e in o.export()]
+    assert sorted(got) == sorted(want)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_multi_trie_batch(gmpt, workloads, oracle, mode):
+    rng = np.random.default_rng(8)
+    batches = [_rand_batch(workloads, 800 + i, int(s), 30, 100) for i, s in enumerate(rng.integers(1, 700, 8))]
+    allb = workloads.concat_batches(batches, trie_ids=list(range(8)))
+    allb = allb.subset(rng.permutation(allb.n))
+    t = gmpt.MPT(device=0, num_tries=8)
+    _put(t, allb, mode)
+    for i, b in enumerate(batches):
+        assert t.get_root_hash(i) == oracle.root_of(b), i
+
+
+def test_device_resident_batch(gmpt, workloads, oracle):
+    b = _rand_batch(workloads, 31, 4000, 70, 110)
+    db = gmpt.DeviceBatch(b, adopt_values=True)
+    t = gmpt.MPT(device=0)
+    targets, n = t.puts_2phase_with_valuehp(db)
+    t.hash_onepass_v2(targets, n)
+    assert t.get_root_hash() == oracle.root_of(b)
+    assert t.get_values(b.subset(range(0, 4000, 101))) == [b.value(i) for i in range(0, 4000, 101)]
+    t.close()
+    db.free()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_malformed_key_is_rejected(gmpt, workloads, oracle, mode):
+    t = gmpt.MPT(device=0)
+    bad = workloads.batch_from_hex([bytes([1, 2, 16]), bytes([1, 17, 16])], [b"x" * 40, b"y" * 40])
+    with pytest.raises(gmpt.GmptError):
+        _put(t, bad, mode)
+    no_term = workloads.batch_from_hex([bytes([1, 2, 3])], [b"z" * 40])
+    with pytest.raises(gmpt.GmptError):
+        _put(t, no_term, mode)
+    good = _rand_batch(workloads, 3, 100, 40, 60)
+    _put(t, good, mode)
+    assert t.get_root_hash() == oracle.root_of(good)
+
+
+def test_reference_c_abi(gmpt, workloads, oracle):
+    L = gmpt.gmpt.lib()
+    b = _rand_batch(workloads, 44, 2000, 40, 90)
+    args = [b.keys_hexs.ctypes.data, b.keys_indexs.ctypes.data, b.values_bytes.ctypes.data,
+            b.values_indexs.ctypes.data, None, b.n]
+    assert L.preprocess() == 1
+    want = oracle.root_of(b)
+    h = L.build_mpt_2phase(0, *args)
+    assert ctypes.string_at(h, 32) == want
+    h2 = L.build_mpt_olc(2, *args)
+    assert ctypes.string_at(h2, 32) == want
+    ns = L.get_all_nodes(0, b.keys_hexs.ctypes.data, b.keys_indexs.ctypes.data, b.n)
+    assert ns.num > b.n
+    m = L.init_mpt()
+    h3 = L.insert_mpt_2phase(m, *args)
+    assert ctypes.string_at(h3, 32) == want
+    h4 = L.insert_mpt_olc(m, *args)
+    assert ctypes.string_at(h4, 32) == want
+    L.free_mpt(m)
+    for p in (h, h2, h3, h4, ns.hashs, ns.encs, ns.encs_indexs):
+        L.gmpt_free_host(p)
