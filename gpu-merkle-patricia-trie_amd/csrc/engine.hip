@@ -179,8 +179,9 @@ int Forest::ensure_pools(uint64_t nb, uint64_t ns, uint64_t nv, uint64_t key_byt
   if ((r = grow(shrt_, (cnt_.n_short + ns) * sizeof(Short), cnt_.n_short * sizeof(Short)))) return r;
   if ((r = grow(value_, (cnt_.n_value + nv) * sizeof(Value), cnt_.n_value * sizeof(Value)))) return r;
   if ((r = grow(keys_, key_used_ + key_bytes + 64, key_used_))) return r;
-  // every branch / short node is listed at most once
-  size_t dcap = (branch_.cap / sizeof(Branch) + shrt_.cap / sizeof(Short)) * sizeof(ref_t);
+  // every existing node is listed at most once, a build adds at most
+  // 3 entries (holes included) per slot it reserved
+  size_t dcap = ((size_t)cnt_.n_dirty + cnt_.n_branch + cnt_.n_short + 2 * nb + ns + 64) * sizeof(ref_t);
   if ((r = grow(dirty_, dcap, cnt_.n_dirty * sizeof(ref_t)))) return r;
   if ((uint64_t)(cnt_.n_branch + nb) > 0x3fffffffull || (uint64_t)(cnt_.n_short + ns) > 0x3fffffffull ||
       (uint64_t)(cnt_.n_value + nv) > 0x3fffffffull) {
@@ -302,7 +303,8 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     uint64_t vbase = 0;
     if ((r = stage_batch(b, &d, &vbase, true))) return r;
     const uint32_t n = b.n;
-    if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, n, d.keys_bytes / 2 + 8ull * n))) return r;
+    // slots: branches <= 2n+2, shorts <= 2(2n+2), values: 2n+2 for elements + n for updates
+    if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, 3ull * n + 2, d.keys_bytes / 2 + 8ull * n))) return r;
     const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
     if ((r = ensure_ws(ws_, nu_ws_bytes(n, has_tid)))) return r;
     Carver c(ws_.p);
@@ -320,6 +322,9 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     a.v_bytes = d.values_total;
     a.kbase = key_used_;
     a.vbase = vbase;
+    a.base_branch = cnt_.n_branch;
+    a.base_short = cnt_.n_short;
+    a.base_value = cnt_.n_value;
     size_t ne = 2 * (size_t)n + 2;
     a.klen = c.take<uint32_t>(n);
     a.kpoff = c.take<uint64_t>(n);
@@ -398,7 +403,7 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
       stats_.regroups++;
     }
     HIP_TRY(nu_emit(a, temp, stream_));
-    HIP_TRY(nu_build(a, stream_));
+    HIP_TRY(nu_build(a, temp, stream_));
     HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
     if ((r = sync_counters())) return r;
     key_used_ += *(uint64_t *)(h_scal_ + SC_KTOT);
@@ -419,7 +424,7 @@ int Forest::hash(void *targets, int n_targets) {
   (void)n_targets;  // the whole dirty list is hashed: a subset would leave stale ancestors
   uint32_t n = cnt_.n_dirty;
   stats_.hash_launches = 0;
-  stats_.hashed_nodes = n;
+  stats_.hashed_nodes = 0;
   stats_.last_hash_ms = 0;
   if (n > 0) {
     Pools P = pools();
@@ -440,6 +445,7 @@ int Forest::hash(void *targets, int n_targets) {
       if (cnt) {
         HIP_TRY(hc_hash_range(P, sorted + start, cnt, stream_));
         stats_.hash_launches++;
+        stats_.hashed_nodes += cnt;
       }
       start += cnt;
     }
@@ -608,7 +614,7 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
     if ((r = stage_batch(b, &d, &vbase, true))) return r;
     const uint32_t n = b.n;
     if ((r = ensure_pools(n, 2ull * n, n, d.keys_bytes / 2 + 8ull * n))) return r;
-    if ((r = ensure_ws(ws_, 12ull * n + 4096 + scan_temp_bytes(n)))) return r;
+    if ((r = ensure_ws(ws_, 20ull * n + 8192 + scan_temp_bytes(n)))) return r;
     HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
     Carver c(ws_.p);
     NuArgs a;
@@ -628,6 +634,8 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
     a.klen = c.take<uint32_t>(n);
     a.kpoff = c.take<uint64_t>(n);
     a.scal = c.take<uint32_t>(16);
+    a.ecount = c.take<uint32_t>(n);  // per-block partials of the key scan
+    a.grp = c.take<uint32_t>(n);
     a.seq_base = lock_seq_;
     a.base_branch = cnt_.n_branch;
     a.base_short = cnt_.n_short;

@@ -54,31 +54,94 @@ static_assert(kKeccak.rc[0] == 1ull, "rc[0]");
 static_assert(kKeccak.rc[1] == 0x8082ull, "rc[1]");
 static_assert(kKeccak.rho[1] == 1 && kKeccak.rho[2] == 62, "rho");
 
+// 64-bit rotate as two 32-bit funnel shifts (v_alignbit_b32); r is a
+// compile-time constant after unrolling.
 __device__ __forceinline__ uint64_t rotl64(uint64_t v, int r) {
-  return r == 0 ? v : ((v << r) | (v >> (64 - r)));
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  if (r == 0) return v;
+  if (r >= 32) {
+    uint32_t t = lo;
+    lo = hi;
+    hi = t;
+    r -= 32;
+    if (r == 0) return ((uint64_t)hi << 32) | lo;
+  }
+  uint32_t nlo = __builtin_amdgcn_alignbit(lo, hi, 32 - r);
+  uint32_t nhi = __builtin_amdgcn_alignbit(hi, lo, 32 - r);
+  return ((uint64_t)nhi << 32) | nlo;
+}
+
+// gfx950 three-input boolean op: result bit = imm bit (4 a + 2 b + c).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+// chi: a ^ (~b & c)
+__device__ __forceinline__ uint32_t chi3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);
+}
+
+// rotate-left of the 64-bit lane (lo, hi) by a compile-time r
+__device__ __forceinline__ void rotl2(uint32_t lo, uint32_t hi, int r, uint32_t &olo, uint32_t &ohi) {
+  if (r >= 32) {
+    uint32_t t = lo;
+    lo = hi;
+    hi = t;
+    r -= 32;
+  }
+  if (r == 0) {
+    olo = lo;
+    ohi = hi;
+    return;
+  }
+  olo = __builtin_amdgcn_alignbit(lo, hi, 32 - r);
+  ohi = __builtin_amdgcn_alignbit(hi, lo, 32 - r);
 }
 
 // Keccak-f[1600] on a state held in registers; lane A[x,y] is s[x + 5y].
+// Per round and 32-bit half: theta's column parities with two xor3 each, the
+// theta update fused into one xor3 per lane (A ^ C[x-1] ^ rot(C[x+1], 1)),
+// rho as alignbit pairs, chi as one bitop3 per lane: 180 VALU ops per round.
 __device__ __forceinline__ void keccak_f1600(uint64_t s[25]) {
+  uint32_t L[25], H[25];
+#pragma unroll
+  for (int i = 0; i < 25; i++) {
+    L[i] = (uint32_t)s[i];
+    H[i] = (uint32_t)(s[i] >> 32);
+  }
 #pragma unroll 1
   for (int ir = 0; ir < 24; ir++) {
-    uint64_t C[5], D[5], B[25];
+    uint32_t CL[5], CH[5], RL[5], RH[5], BL[25], BH[25];
 #pragma unroll
-    for (int x = 0; x < 5; x++) C[x] = s[x] ^ s[x + 5] ^ s[x + 10] ^ s[x + 15] ^ s[x + 20];
+    for (int x = 0; x < 5; x++) {
+      CL[x] = xor3(xor3(L[x], L[x + 5], L[x + 10]), L[x + 15], L[x + 20]);
+      CH[x] = xor3(xor3(H[x], H[x + 5], H[x + 10]), H[x + 15], H[x + 20]);
+    }
 #pragma unroll
-    for (int x = 0; x < 5; x++) D[x] = C[(x + 4) % 5] ^ rotl64(C[(x + 1) % 5], 1);
+    for (int x = 0; x < 5; x++) rotl2(CL[(x + 1) % 5], CH[(x + 1) % 5], 1, RL[x], RH[x]);
 #pragma unroll
     for (int x = 0; x < 5; x++)
 #pragma unroll
-      for (int y = 0; y < 5; y++)
-        B[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(s[x + 5 * y] ^ D[x], kKeccak.rho[x + 5 * y]);
+      for (int y = 0; y < 5; y++) {
+        int i = x + 5 * y;
+        uint32_t al = xor3(L[i], CL[(x + 4) % 5], RL[x]);
+        uint32_t ah = xor3(H[i], CH[(x + 4) % 5], RH[x]);
+        int j = y + 5 * ((2 * x + 3 * y) % 5);
+        rotl2(al, ah, kKeccak.rho[i], BL[j], BH[j]);
+      }
 #pragma unroll
     for (int y = 0; y < 5; y++)
 #pragma unroll
-      for (int x = 0; x < 5; x++)
-        s[x + 5 * y] = B[x + 5 * y] ^ ((~B[(x + 1) % 5 + 5 * y]) & B[(x + 2) % 5 + 5 * y]);
-    s[0] ^= kKeccak.rc[ir];
+      for (int x = 0; x < 5; x++) {
+        int i = x + 5 * y, i1 = (x + 1) % 5 + 5 * y, i2 = (x + 2) % 5 + 5 * y;
+        L[i] = chi3(BL[i], BL[i1], BL[i2]);
+        H[i] = chi3(BH[i], BH[i1], BH[i2]);
+      }
+    uint64_t rc = kKeccak.rc[ir];
+    L[0] ^= (uint32_t)rc;
+    H[0] ^= (uint32_t)(rc >> 32);
   }
+#pragma unroll
+  for (int i = 0; i < 25; i++) s[i] = ((uint64_t)H[i] << 32) | L[i];
 }
 
 }  // namespace gmpt

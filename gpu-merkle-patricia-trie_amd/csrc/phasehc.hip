@@ -114,8 +114,10 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *list, uin
   __shared__ uint32_t h[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
-  for (uint32_t i = blockIdx.x * kHcTile + threadIdx.x; i < min(n, (blockIdx.x + 1) * kHcTile); i += blockDim.x)
-    atomicAdd(&h[bin_of(P, list[i])], 1u);
+  for (uint32_t i = blockIdx.x * kHcTile + threadIdx.x; i < min(n, (blockIdx.x + 1) * kHcTile); i += blockDim.x) {
+    ref_t r = list[i];
+    if (r != REF_NULL) atomicAdd(&h[bin_of(P, r)], 1u);  // builds leave null holes
+  }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x)
     if (h[b]) atomicAdd(&hist[b], h[b]);
@@ -133,9 +135,8 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *list, 
 #pragma unroll
   for (int q = 0; q < kPer; q++) {
     uint32_t i = blockIdx.x * kHcTile + q * 256 + threadIdx.x;
-    r[q] = REF_NULL;
-    if (i < n) {
-      r[q] = list[i];
+    r[q] = i < n ? list[i] : REF_NULL;
+    if (r[q] != REF_NULL) {
       bin[q] = bin_of(P, r[q]);
       rank[q] = atomicAdd(&cnt[bin[q]], 1u);
     }
@@ -224,6 +225,131 @@ struct Stage {
       }
     }
     pos += len;
+  }
+};
+
+// Unaligned 8-byte read as aligned loads plus a byte funnel shift.
+__device__ __forceinline__ uint64_t load8(const uint8_t *p) {
+  uintptr_t a = (uintptr_t)p;
+  uint32_t sh = (uint32_t)(a & 7);
+  const uint64_t *q = (const uint64_t *)(a - sh);
+  uint64_t w0 = q[0];
+  if (sh == 0) return w0;
+  return (w0 >> (8 * sh)) | (q[1] << (64 - 8 * sh));
+}
+// The first c (1..7) bytes at p, upper bytes zero; reads only words holding them.
+__device__ __forceinline__ uint64_t load_part(const uint8_t *p, uint32_t c) {
+  uintptr_t a = (uintptr_t)p;
+  uint32_t sh = (uint32_t)(a & 7);
+  const uint64_t *q = (const uint64_t *)(a - sh);
+  uint64_t v = q[0] >> (8 * sh);
+  if (sh + c > 8) v |= q[1] << (64 - 8 * sh);
+  return v & ((1ull << (8 * c)) - 1);
+}
+
+// One Keccak block of a node's message, assembled 8 bytes at a time: parts
+// are emitted in message order, bytes outside [lo, lo + 136) are skipped and
+// completed 64-bit words go to the lane's LDS words (word w at lane + 512 w).
+struct Blk {
+  uint8_t *lane;
+  uint32_t lo;
+  uint32_t pos = 0;   // message offset of the next part
+  uint64_t acc = 0;   // pending bytes
+  uint32_t accn = 0;
+  uint32_t nw = 0;    // words written
+  __device__ __forceinline__ Blk(uint8_t *l, uint32_t block_lo) : lane(l), lo(block_lo) {}
+  __device__ __forceinline__ void push(uint64_t v, uint32_t n) {  // n in 1..8, v's upper bytes zero
+    acc |= v << (8 * accn);
+    uint32_t tot = accn + n;
+    if (tot >= 8) {
+      *(uint64_t *)(lane + nw * 512) = acc;
+      nw++;
+      uint32_t used = 8 - accn;
+      acc = used >= 8 ? 0ull : (v >> (8 * used));
+      accn = tot - 8;
+    } else {
+      accn = tot;
+    }
+  }
+  __device__ __forceinline__ void window(uint32_t n, uint32_t *a, uint32_t *e) const {
+    *a = pos > lo ? pos : lo;
+    *e = min(pos + n, lo + 136);
+  }
+  __device__ __forceinline__ void inline_bytes(uint64_t b, uint32_t len) {  // len <= 8
+    uint32_t a, e;
+    window(len, &a, &e);
+    if (a < e) {
+      uint32_t c = e - a;
+      uint64_t v = b >> (8 * (a - pos));
+      if (c < 8) v &= (1ull << (8 * c)) - 1;
+      push(v, c);
+    }
+    pos += len;
+  }
+  __device__ __forceinline__ void mem_bytes(const uint8_t *src, uint32_t len) {
+    uint32_t a, e;
+    window(len, &a, &e);
+    if (a < e) {
+      const uint8_t *sp = src + (a - pos);
+      uint32_t c = e - a;
+      for (; c >= 8; c -= 8, sp += 8) push(load8(sp), 8);
+      if (c) push(load_part(sp, c), c);
+    }
+    pos += len;
+  }
+  // write_bytes (rlp.cuh:132-155) of a memory string
+  __device__ __forceinline__ void rlp_mem(const uint8_t *src, uint32_t len) {
+    if (len == 1 && src[0] <= 0x7f) {
+      inline_bytes(src[0], 1);
+      return;
+    }
+    uint32_t hn;
+    uint64_t h = rlp_hdr(len, 0x80, 0xB7, &hn);
+    inline_bytes(h, hn);
+    mem_bytes(src, len);
+  }
+  // write_bytes of the hex-prefix form of nibbles [s, s + cnt) of a packed
+  // key with the leaf flag (utils.cuh:372-396)
+  __device__ __forceinline__ void rlp_compact(const uint8_t *key, uint32_t s, uint32_t cnt, uint32_t term) {
+    uint32_t kc = cnt / 2 + 1;
+    uint32_t flag = (term << 5) | ((cnt & 1) << 4) | ((cnt & 1) ? packed_nibble(key, s) : 0u);
+    if (kc == 1) {  // single byte <= 0x3f: written raw
+      inline_bytes(flag, 1);
+      return;
+    }
+    uint32_t hn;
+    uint64_t h = rlp_hdr(kc, 0x80, 0xB7, &hn);
+    inline_bytes(h, hn);
+    inline_bytes(flag, 1);
+    uint32_t first = s + (cnt & 1);
+    if ((first & 1) == 0) {
+      mem_bytes(key + first / 2, kc - 1);
+      return;
+    }
+    // nibble pairs straddle bytes: out byte i = low nibble of b[i], high of b[i+1]
+    uint32_t a, e;
+    window(kc - 1, &a, &e);
+    if (a < e) {
+      const uint8_t *sp = key + first / 2 + (a - pos);
+      uint32_t c = e - a;
+      for (; c > 0; sp += 8) {
+        uint64_t x = load8(sp), y = load8(sp + 1);
+        uint64_t v = ((x << 4) & 0xF0F0F0F0F0F0F0F0ull) | ((y >> 4) & 0x0F0F0F0F0F0F0F0Full);
+        uint32_t t = c < 8 ? c : 8;
+        if (t < 8) v &= (1ull << (8 * t)) - 1;
+        push(v, t);
+        c -= t;
+      }
+    }
+    pos += kc - 1;
+  }
+  // flush the pending word and zero the block's remaining words
+  __device__ __forceinline__ void finish() {
+    if (accn) {
+      *(uint64_t *)(lane + nw * 512) = acc;
+      nw++;
+    }
+    for (uint32_t w = nw; w < 17; w++) *(uint64_t *)(lane + w * 512) = 0;
   }
 };
 
@@ -328,13 +454,9 @@ __global__ __launch_bounds__(256) void k_hc_hash(Pools P, const ref_t *list, uin
   for (uint32_t b = 0; b < wave_blocks; b++) {
     bool mine_b = b < nblocks;
     if (mine_b) {
-#pragma unroll
-      for (int q = 0; q < 17; q++) *(uint64_t *)(mine + q * 512) = 0;
-      Stage<LdsSink, 136> st;
-      st.sink.lane = mine;
-      st.lo = b * 136;
-      st.pos = 0;
+      Blk st(mine, b * 136);
       emit_node(P, r, payload, st);
+      st.finish();
       if (enc >= 32 && b + 1 == nblocks) {
         // Keccak padding: 0x01 after the message, 0x80 on the block's last byte
         uint32_t k = enc - st.lo;
@@ -414,13 +536,9 @@ __global__ __launch_bounds__(256) void k_keccak_batch(const uint8_t *data, const
   for (uint32_t b = 0; b < wb; b++) {
     bool mb = b < nblocks;
     if (mb) {
-#pragma unroll
-      for (int q = 0; q < 17; q++) *(uint64_t *)(mine + q * 512) = 0;
-      Stage<LdsSink, 136> st;
-      st.sink.lane = mine;
-      st.lo = b * 136;
-      st.pos = 0;
+      Blk st(mine, b * 136);
       st.mem_bytes(src, L);
+      st.finish();
       if (b + 1 == nblocks) {
         uint32_t k = L - st.lo;
         mine[(k >> 3) * 512 + (k & 7)] ^= 0x01;

@@ -86,8 +86,10 @@ struct NuArgs {
 };
 
 // device scalar slots in NuArgs::scal
-enum { SC_W = 0, SC_MAXLEN = 1, SC_M_UNIQ = 2, SC_NGROUP = 3, SC_NE = 4, SC_KTOT = 6 /* u64 */, SC_MERGE = 8,
-       SC_COUNT = 16 };
+enum {
+  SC_W = 0, SC_MAXLEN = 1, SC_M_UNIQ = 2, SC_NGROUP = 3, SC_NE = 4, SC_KTOT = 6 /* u64 */, SC_MERGE = 8,
+  SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11, SC_COUNT = 16
+};
 
 hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s);
 hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
@@ -102,6 +104,8 @@ hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s);
 hipError_t nu_regroup(NuArgs &a, uint32_t m, uint32_t nruns, void *temp, hipStream_t s);
 // Emits the element sequence E of every group.
 hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s);
-hipError_t nu_build(const NuArgs &a, hipStream_t s);
+// Builds every group's subtree; node and dirty-list slots are fixed by
+// position from base_branch / base_short / base_value (see phasenu.hip).
+hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s);
 
 }  // namespace gmpt

@@ -56,7 +56,26 @@ __device__ __forceinline__ bool key_range_ok(const NuArgs &a, uint32_t i) {
   return s >= 0 && e >= s && (uint64_t)e < a.kh_bytes && e - s + 1 <= kMaxKeyNibbles;
 }
 
-__global__ __launch_bounds__(256) void k_keys_scan(NuArgs a) {
+// the first c (0..8) bytes at p (upper bytes zero)
+__device__ __forceinline__ uint64_t ldn(const uint8_t *p, uint32_t c) {
+  if (c == 0) return 0;
+  uintptr_t a = (uintptr_t)p;
+  uint32_t sh = (uint32_t)(a & 7);
+  const uint64_t *q = (const uint64_t *)(a - sh);
+  uint64_t v = q[0] >> (8 * sh);
+  if (sh + c > 8) v |= q[1] << (64 - 8 * sh);
+  return c >= 8 ? v : v & ((1ull << (8 * c)) - 1);
+}
+// 8 hex nibble bytes -> 4 packed bytes (high nibble first), memory order
+__device__ __forceinline__ uint32_t pack8(uint64_t h) {
+  uint64_t t = ((h & 0x0F000F000F000F00ull) >> 8) | ((h & 0x000F000F000F000Full) << 4);
+  return (uint32_t)((t & 0xFFull) | ((t >> 8) & 0xFF00ull) | ((t >> 16) & 0xFF0000ull) | ((t >> 24) & 0xFF000000ull));
+}
+
+// Validates every key, and reduces per block the common prefix with key 0
+// and the longest key (no global atomics: one slot per block).
+__global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, uint32_t *blk_len) {
+  __shared__ uint32_t sw[4], sl[4];
   uint32_t i = grid_tid();
   uint32_t lcp0 = UINT_MAX, len = 0;
   if (i < a.n) {
@@ -66,16 +85,27 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a) {
       int32_t s = a.kidx[2 * i], e = a.kidx[2 * i + 1];
       L = e - s + 1;
       const uint8_t *k = a.kh + s;
-      ok = k[L - 1] == kNibbleT;
-      for (int j = 0; ok && j < L - 1; j++) ok = k[j] < 16;
-      // common prefix with key 0 (terminator counts as a distinct symbol)
+      uint32_t nn = (uint32_t)L - 1;  // nibbles before the terminator
+      ok = k[nn] == kNibbleT;
+      for (uint32_t o = 0; ok && o < nn; o += 8) {
+        uint64_t w = ldn(k + o, min(8u, nn - o));
+        ok = (w & 0xF0F0F0F0F0F0F0F0ull) == 0;
+      }
+      // common prefix with key 0 (the terminator is a symbol of its own)
       if (ok && key_range_ok(a, 0)) {
         int32_t s0 = a.kidx[0], e0 = a.kidx[1];
         const uint8_t *k0 = a.kh + s0;
-        int32_t L0 = e0 - s0 + 1;
+        uint32_t mn = min((uint32_t)L, (uint32_t)(e0 - s0 + 1));
         uint32_t p = 0;
-        while ((int32_t)p < L && (int32_t)p < L0 && k[p] == k0[p]) p++;
-        lcp0 = p;
+        for (; p < mn; p += 8) {
+          uint32_t c = min(8u, mn - p);
+          uint64_t x = ldn(k + p, c) ^ ldn(k0 + p, c);
+          if (x) {
+            p += (uint32_t)(__ffsll((unsigned long long)x) - 1) / 8;
+            break;
+          }
+        }
+        lcp0 = min(p, mn);
       } else {
         lcp0 = 0;
       }
@@ -94,14 +124,41 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a) {
     a.klen[i] = len;
     a.kpoff[i] = (uint64_t)((len / 2 + 8) & ~7u);  // packed slot bytes (>= 8)
   }
-  // wave-reduce then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     lcp0 = min(lcp0, (uint32_t)__shfl_xor(lcp0, off));
     len = max(len, (uint32_t)__shfl_xor(len, off));
   }
   if (__lane_id() == 0) {
-    atomicMin(&a.scal[SC_W], lcp0);
-    atomicMax(&a.scal[SC_MAXLEN], len);
+    sw[threadIdx.x >> 6] = lcp0;
+    sl[threadIdx.x >> 6] = len;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    blk_w[blockIdx.x] = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
+    blk_len[blockIdx.x] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_keys_reduce(NuArgs a, const uint32_t *blk_w, const uint32_t *blk_len,
+                                                     uint32_t nb) {
+  __shared__ uint32_t sw[4], sl[4];
+  uint32_t w = UINT_MAX, l = 0;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    w = min(w, blk_w[b]);
+    l = max(l, blk_len[b]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    w = min(w, (uint32_t)__shfl_xor(w, off));
+    l = max(l, (uint32_t)__shfl_xor(l, off));
+  }
+  if (__lane_id() == 0) {
+    sw[threadIdx.x >> 6] = w;
+    sl[threadIdx.x >> 6] = l;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.scal[SC_W] = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
+    a.scal[SC_MAXLEN] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
   }
 }
 
@@ -112,30 +169,20 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
   if (i >= a.n) return;
   uint32_t L = a.klen[i];
   const uint8_t *k = a.kh + (key_range_ok(a, i) ? a.kidx[2 * i] : 0);
-  uint8_t *dst = a.P.keys + a.kbase + a.kpoff[i];
+  uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + a.kpoff[i]);
   uint32_t nb = (L / 2 + 8) & ~7u;
   uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
   for (uint32_t b = 0; b < nb; b += 8) {
-    uint64_t w = 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      uint32_t j = 2 * (b + q);
-      uint32_t hi = j < nn ? k[j] : 0u;
-      uint32_t lo = j + 1 < nn ? k[j + 1] : 0u;
-      w |= (uint64_t)((hi << 4) | lo) << (8 * q);
-    }
-    *(uint64_t *)(dst + b) = w;
+    uint32_t o = 2 * b;  // first nibble of this 8-byte word
+    uint64_t h0 = o < nn ? ldn(k + o, min(8u, nn - o)) : 0ull;
+    uint64_t h1 = o + 8 < nn ? ldn(k + o + 8, min(8u, nn - o - 8)) : 0ull;
+    dst[b / 8] = (uint64_t)pack8(h0) | ((uint64_t)pack8(h1) << 32);
   }
   if (!a.sk0) return;  // LockNU packs keys without sorting them
   uint32_t W = a.scal[SC_W];
-  uint64_t win = 0;
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    uint32_t j = W + q;
-    uint64_t v = j < nn ? (uint64_t)k[j] : 0ull;
-    win |= v << (60 - 4 * q);
-  }
-  a.sk0[i] = win;
+  uint64_t h0 = W < nn ? ldn(k + W, min(8u, nn - W)) : 0ull;
+  uint64_t h1 = W + 8 < nn ? ldn(k + W + 8, min(8u, nn - W - 8)) : 0ull;
+  a.sk0[i] = ((uint64_t)__builtin_bswap32(pack8(h0)) << 32) | __builtin_bswap32(pack8(h1));
   a.sv0[i] = i;
 }
 
@@ -647,8 +694,11 @@ __device__ __forceinline__ ST st_of(const NuArgs &a) {
 
 // Per split point: find its branch (the representative is the leftmost split
 // point with the same LCP inside the maximal interval whose LCPs are >= it).
-__global__ __launch_bounds__(256) void k_split_reps(NuArgs a) {
+// bid[j] = 1 for representatives; an exclusive scan turns it into the
+// branch's rank.  Entries up to the launch bound are written (0 past nsp).
+__global__ __launch_bounds__(256) void k_split_reps(NuArgs a, uint32_t ne_max) {
   uint32_t j = grid_tid();
+  if (j >= ne_max) return;
   ST s = st_of(a);
   bool rep = false;
   if (j < s.nsp) {
@@ -661,21 +711,53 @@ __global__ __launch_bounds__(256) void k_split_reps(NuArgs a) {
     }
     a.repof[j] = r;
   }
-  uint32_t id = wave_alloc(&a.P.cnt->n_branch, rep);
-  if (rep) {
-    if (id >= a.P.cap_branch) {
-      raise_error(a.P.cnt, ERR_POOL, 1);
-      id = 0;
-    }
-    a.bid[j] = id;
-    Branch &B = a.P.branch[id];
-    uint64_t *w = (uint64_t *)&B;
-#pragma unroll
-    for (int q = 0; q < 16; q++) w[q] = 0;
-    B.depth = (uint16_t)(s.at(0, j) - 1);
-    B.dirty = 1;
+  a.bid[j] = rep ? 1u : 0u;
+}
+
+// Node slots of this build, all fixed by position (no allocator atomics):
+//   branch of representative rank r   : base_branch + r
+//   leaf / remainder short of element e: base_short + e
+//   extension short of branch rank r   : base_short + NE + r
+//   value of new-key element e         : base_value + e
+//   value of update i (unique index)   : base_value + upd_off + i
+// Dirty-list entries: dbase + e (element short), dbase + NE + r (branch),
+// dbase + NE + R + r (extension); REF_NULL where no node was made.
+struct Slots {
+  uint32_t ne, nrep, dbase;
+};
+__device__ __forceinline__ Slots slots_of(const NuArgs &a) {
+  Slots q;
+  q.ne = a.scal[SC_NE];
+  q.nrep = a.scal[SC_NREP];
+  q.dbase = a.scal[SC_DBASE];
+  return q;
+}
+__device__ __forceinline__ void put_dirty(const NuArgs &a, uint32_t slot, ref_t r) {
+  if (slot < a.P.cap_dirty)
+    a.P.dirty[slot] = r;
+  else
+    raise_error(a.P.cnt, ERR_POOL, 4);
+}
+
+__global__ __launch_bounds__(256) void k_init_branches(NuArgs a) {
+  uint32_t j = grid_tid();
+  ST s = st_of(a);
+  if (j >= s.nsp) return;
+  uint32_t L = s.at(0, j);
+  if (L == 0 || a.repof[j] != j) return;
+  Slots q = slots_of(a);
+  uint32_t id = a.base_branch + a.bid[j];
+  if (id >= a.P.cap_branch) {
+    raise_error(a.P.cnt, ERR_POOL, 1);
+    return;
   }
-  mark_new_dirty(a.P, make_ref(T_BRANCH, rep ? a.bid[j] : 0), rep);
+  Branch &B = a.P.branch[id];
+  uint64_t *w = (uint64_t *)&B;
+#pragma unroll
+  for (int k = 0; k < 16; k++) w[k] = 0;
+  B.depth = (uint16_t)(L - 1);
+  B.dirty = 1;
+  put_dirty(a, q.dbase + q.ne + a.bid[j], make_ref(T_BRANCH, id));
 }
 
 // Source of an E element's key nibbles.
@@ -744,149 +826,52 @@ __device__ __forceinline__ void install(const NuArgs &a, const GroupInfo &gi, re
 __global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
   uint32_t j = grid_tid();
   ST s = st_of(a);
-  bool rep = false, need_ext = false;
-  uint32_t L = 0, lo_depth = 0;
-  int lo = -1;
-  uint32_t hi = 0;
-  ElemKey ek;
+  if (j >= s.nsp) return;
+  uint32_t L = s.at(0, j);
+  if (L == 0 || a.repof[j] != j) return;
+  Slots q = slots_of(a);
+  int lo = s.prev_less(j, L);
+  uint32_t hi = s.next_less(j, L);
+  uint32_t La = lo >= 0 ? s.at(0, (uint32_t)lo) : 0u;
+  uint32_t Lb = hi < s.nsp ? s.at(0, hi) : 0u;
+  uint32_t e0 = (uint32_t)(lo + 1);
+  ElemKey ek = elem_key(a, e0);
+  uint32_t depth = L - 1, lo_depth, parent_bid = 0, slot = 0;
+  bool is_root = La == 0 && Lb == 0;
   GroupInfo gi;
-  uint32_t parent_bid = 0, slot = 0;
-  bool is_root = false;
-  if (j < s.nsp) {
-    L = s.at(0, j);
-    rep = L != 0 && a.repof[j] == j;
+  if (is_root) {
+    gi = a.ginfo[a.e_grp[e0]];
+    lo_depth = gi.base;
+  } else {
+    uint32_t p = La >= Lb ? (uint32_t)lo : hi;
+    uint32_t Lp = max(La, Lb) - 1;
+    parent_bid = a.base_branch + a.bid[a.repof[p]];
+    slot = elem_nib(a.P, ek, Lp);
+    lo_depth = Lp + 1;
   }
-  if (rep) {
-    lo = s.prev_less(j, L);
-    hi = s.next_less(j, L);
-    uint32_t La = lo >= 0 ? s.at(0, (uint32_t)lo) : 0u;
-    uint32_t Lb = hi < s.nsp ? s.at(0, hi) : 0u;
-    uint32_t e0 = (uint32_t)(lo + 1);
-    ek = elem_key(a, e0);
-    uint32_t depth = L - 1;
-    if (La == 0 && Lb == 0) {
-      is_root = true;
-      gi = a.ginfo[a.e_grp[e0]];
-      lo_depth = gi.base;
+  uint32_t rank = a.bid[j];
+  ref_t me = make_ref(T_BRANCH, a.base_branch + rank);
+  ref_t ext = REF_NULL;
+  if (depth > lo_depth) {
+    uint32_t sid = a.base_short + q.ne + rank;
+    if (sid >= a.P.cap_short) {
+      raise_error(a.P.cnt, ERR_POOL, 2);
     } else {
-      uint32_t p = La >= Lb ? (uint32_t)lo : hi;
-      uint32_t Lp = max(La, Lb) - 1;
-      parent_bid = a.bid[a.repof[p]];
-      slot = elem_nib(a.P, ek, Lp);
-      lo_depth = Lp + 1;
+      init_short(a.P.shrt[sid], ek.off, ek.start + (lo_depth - ek.origin), depth - lo_depth, lo_depth, 0, me);
+      ext = make_ref(T_SHORT, sid);
+      me = ext;
     }
-    need_ext = depth > lo_depth;
   }
-  uint32_t sid = wave_alloc(&a.P.cnt->n_short, need_ext);
-  ref_t me = REF_NULL;
-  if (rep) {
-    ref_t bref = make_ref(T_BRANCH, a.bid[j]);
-    me = bref;
-    if (need_ext) {
-      if (sid >= a.P.cap_short) {
-        raise_error(a.P.cnt, ERR_POOL, 2);
-        need_ext = false;
-      } else {
-        uint32_t depth = L - 1;
-        init_short(a.P.shrt[sid], ek.off, ek.start + (lo_depth - ek.origin), depth - lo_depth, lo_depth, 0, bref);
-        me = make_ref(T_SHORT, sid);
-      }
-    }
-    if (is_root)
-      install(a, gi, me);
-    else
-      a.P.branch[parent_bid].child[slot] = me;
-  }
-  mark_new_dirty(a.P, me, rep && need_ext);
+  put_dirty(a, q.dbase + q.ne + q.nrep + rank, ext);
+  if (is_root)
+    install(a, gi, me);
+  else
+    a.P.branch[parent_bid].child[slot] = me;
 }
 
 // Elements: new keys become a leaf short node + value node; a split short
 // node's remainder is re-attached below the new branch.
-__global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
-  uint32_t e = grid_tid();
-  ST s = st_of(a);
-  uint32_t ne = a.scal[SC_NE];
-  bool active = e < ne;
-  ElemKey ek;
-  uint32_t start = 0, slot = 0, parent_bid = 0;
-  bool is_root = false, need_short = false, need_value = false;
-  GroupInfo gi;
-  if (active) {
-    ek = elem_key(a, e);
-    uint32_t La = e > 0 ? s.at(0, e - 1) : 0u;
-    uint32_t Lb = e < s.nsp ? s.at(0, e) : 0u;
-    if (La == 0 && Lb == 0) {
-      is_root = true;
-      gi = a.ginfo[a.e_grp[e]];
-      start = gi.base;
-    } else {
-      uint32_t p = La >= Lb ? e - 1 : e;
-      uint32_t Lp = max(La, Lb) - 1;
-      parent_bid = a.bid[a.repof[p]];
-      slot = elem_nib(a.P, ek, Lp);
-      start = Lp + 1;
-    }
-    need_short = start < ek.end;
-    need_value = !ek.virt;
-  }
-  uint32_t vid = wave_alloc(&a.P.cnt->n_value, need_value);
-  uint32_t sid = wave_alloc(&a.P.cnt->n_short, need_short);
-  ref_t me = REF_NULL;
-  if (active) {
-    ref_t child;
-    if (ek.virt) {
-      child = a.P.shrt[ek.uidx].child;
-    } else {
-      if (vid >= a.P.cap_value) {
-        raise_error(a.P.cnt, ERR_POOL, 3);
-        vid = 0;
-      }
-      uint32_t idx = a.u_idx[ek.uidx];
-      Value &V = a.P.value[vid];
-      int64_t vs = a.vidx[2 * idx], ve = a.vidx[2 * idx + 1];
-      V.addr = a.vbase + (uint64_t)vs;
-      V.hp = a.hps ? a.hps[idx] : 0ull;
-      V.size = (uint32_t)(ve - vs + 1);
-      V.pad = 0;
-      V.seq = 0;
-      child = make_ref(T_VALUE, vid);
-    }
-    me = child;
-    if (need_short) {
-      if (sid >= a.P.cap_short) {
-        raise_error(a.P.cnt, ERR_POOL, 2);
-        need_short = false;
-      } else {
-        init_short(a.P.shrt[sid], ek.off, ek.start + (start - ek.origin), ek.end - start, start, ek.term, child);
-        me = make_ref(T_SHORT, sid);
-      }
-    }
-    if (is_root)
-      install(a, gi, me);
-    else
-      a.P.branch[parent_bid].child[slot] = me;
-  }
-  mark_new_dirty(a.P, me, active && need_short);
-}
-
-// Existing keys: swap in a new value node (runs before the links, so a leaf
-// that is also split carries the new value into its remainder).
-__global__ __launch_bounds__(256) void k_updates(NuArgs a) {
-  uint32_t i = grid_tid();
-  uint32_t m = a.scal[SC_M_UNIQ];
-  bool upd = false;
-  Stop st;
-  if (i < m) {
-    st = a.stop[i];
-    upd = is_update(st.kind);
-  }
-  uint32_t vid = wave_alloc(&a.P.cnt->n_value, upd);
-  if (!upd) return;
-  if (vid >= a.P.cap_value) {
-    raise_error(a.P.cnt, ERR_POOL, 3);
-    return;
-  }
-  uint32_t idx = a.u_idx[i];
+__device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32_t idx) {
   Value &V = a.P.value[vid];
   int64_t vs = a.vidx[2 * idx], ve = a.vidx[2 * idx + 1];
   V.addr = a.vbase + (uint64_t)vs;
@@ -894,6 +879,73 @@ __global__ __launch_bounds__(256) void k_updates(NuArgs a) {
   V.size = (uint32_t)(ve - vs + 1);
   V.pad = 0;
   V.seq = 0;
+}
+
+__global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
+  uint32_t e = grid_tid();
+  ST s = st_of(a);
+  Slots q = slots_of(a);
+  if (e >= q.ne) return;
+  ElemKey ek = elem_key(a, e);
+  uint32_t La = e > 0 ? s.at(0, e - 1) : 0u;
+  uint32_t Lb = e < s.nsp ? s.at(0, e) : 0u;
+  uint32_t start, slot = 0, parent_bid = 0;
+  bool is_root = La == 0 && Lb == 0;
+  GroupInfo gi;
+  if (is_root) {
+    gi = a.ginfo[a.e_grp[e]];
+    start = gi.base;
+  } else {
+    uint32_t p = La >= Lb ? e - 1 : e;
+    uint32_t Lp = max(La, Lb) - 1;
+    parent_bid = a.base_branch + a.bid[a.repof[p]];
+    slot = elem_nib(a.P, ek, Lp);
+    start = Lp + 1;
+  }
+  ref_t child;
+  if (ek.virt) {
+    child = a.P.shrt[ek.uidx].child;
+  } else {
+    uint32_t vid = a.base_value + e;
+    if (vid >= a.P.cap_value) {
+      raise_error(a.P.cnt, ERR_POOL, 3);
+      return;
+    }
+    init_value(a, vid, a.u_idx[ek.uidx]);
+    child = make_ref(T_VALUE, vid);
+  }
+  ref_t me = child, sh = REF_NULL;
+  if (start < ek.end) {
+    uint32_t sid = a.base_short + e;
+    if (sid >= a.P.cap_short) {
+      raise_error(a.P.cnt, ERR_POOL, 2);
+    } else {
+      init_short(a.P.shrt[sid], ek.off, ek.start + (start - ek.origin), ek.end - start, start, ek.term, child);
+      sh = make_ref(T_SHORT, sid);
+      me = sh;
+    }
+  }
+  put_dirty(a, q.dbase + e, sh);
+  if (is_root)
+    install(a, gi, me);
+  else
+    a.P.branch[parent_bid].child[slot] = me;
+}
+
+// Existing keys: swap in a new value node (runs before the links, so a leaf
+// that is also split carries the new value into its remainder).
+__global__ __launch_bounds__(256) void k_updates(NuArgs a, uint32_t upd_off) {
+  uint32_t i = grid_tid();
+  uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= m) return;
+  Stop st = a.stop[i];
+  if (!is_update(st.kind)) return;
+  uint32_t vid = a.base_value + upd_off + i;
+  if (vid >= a.P.cap_value) {
+    raise_error(a.P.cnt, ERR_POOL, 3);
+    return;
+  }
+  init_value(a, vid, a.u_idx[i]);
   ref_t vr = make_ref(T_VALUE, vid);
   if (st.kind == STOP_LEAF_UPDATE)
     a.P.shrt[ref_idx(st.stop)].child = vr;
@@ -901,12 +953,31 @@ __global__ __launch_bounds__(256) void k_updates(NuArgs a) {
     a.P.branch[ref_idx(st.stop)].child[16] = vr;
 }
 
+// Records this build's slot bases before it starts and advances the pool and
+// dirty-list counters after it.
+__global__ void k_build_begin(NuArgs a, const uint32_t *nrep_total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  a.scal[SC_NREP] = *nrep_total;
+  a.scal[SC_DBASE] = a.P.cnt->n_dirty;
+}
+__global__ void k_build_end(NuArgs a, uint32_t upd_off) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t ne = a.scal[SC_NE], nrep = a.scal[SC_NREP], m = a.scal[SC_M_UNIQ];
+  a.P.cnt->n_branch = a.base_branch + nrep;
+  a.P.cnt->n_short = a.base_short + ne + nrep;
+  a.P.cnt->n_value = a.base_value + upd_off + m;
+  a.P.cnt->n_dirty = a.scal[SC_DBASE] + ne + 2 * nrep;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- host side --
 
 hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s) {
-  k_keys_scan<<<nblk(a.n), kB, 0, s>>>(a);
+  // per-block partial results live in ecount / grp, unused until grouping
+  k_keys_scan<<<nblk(a.n), kB, 0, s>>>(a, a.ecount, a.grp);
+  GMPT_LAUNCH_CHECK();
+  k_keys_reduce<<<1, kB, 0, s>>>(a, a.ecount, a.grp, nblk(a.n));
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u64(a.kpoff, a.kpoff, a.n, (uint64_t *)(a.scal + SC_KTOT), temp, s);
   if (e != hipSuccess) return e;
@@ -1017,19 +1088,27 @@ hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t nu_build(const NuArgs &a, hipStream_t s) {
-  size_t ne_max = 2 * (size_t)a.n;
+hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
+  uint32_t ne_max = 2 * a.n + 2;
   for (int k = 1; k < a.st_levels; k++) {
     k_st_level<<<nblk(ne_max), kB, 0, s>>>(a, k);
     GMPT_LAUNCH_CHECK();
   }
-  k_updates<<<nblk(a.n), kB, 0, s>>>(a);
+  k_updates<<<nblk(a.n), kB, 0, s>>>(a, ne_max);
   GMPT_LAUNCH_CHECK();
-  k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a);
+  k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a, ne_max);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u32(a.bid, a.bid, ne_max, a.scal + SC_NREP_TOT, temp, s);
+  if (e != hipSuccess) return e;
+  k_build_begin<<<1, 64, 0, s>>>(a, a.scal + SC_NREP_TOT);
+  GMPT_LAUNCH_CHECK();
+  k_init_branches<<<nblk(ne_max), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   k_link_branches<<<nblk(ne_max), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   k_link_elems<<<nblk(ne_max), kB, 0, s>>>(a);
+  GMPT_LAUNCH_CHECK();
+  k_build_end<<<1, 64, 0, s>>>(a, ne_max);
   return hipGetLastError();
 }
 
