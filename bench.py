@@ -19,6 +19,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "gpu-merkle-patricia-trie_amd")
 
@@ -141,12 +143,350 @@ def bench_phase(pkg, args):
     return out
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    return oracle
+
+
+def bench_batched(pkg, args):
+    """4,096 uneven tries: one multi-trie PhaseNU insert + one PhaseHC pass for
+    all of them, against one plain insert + hash + root call per trie."""
+    wl = pkg.workloads
+    t0 = time.time()
+    b, sizes = wl.config4_batched()
+    gen_s = time.time() - t0
+    ntries = len(sizes)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    db = pkg.DeviceBatch(b, adopt_values=True)
+    views = [db.view(int(s), int(c)) for s, c in zip(starts, sizes)]
+    fb = pkg.MPT(device=0, num_tries=ntries)
+    fp = pkg.MPT(device=0)
+
+    def batched_step():
+        fb.clear()
+        targets, nt = fb.puts_2phase_with_valuehp(db)
+        fb.hash_onepass_v2(targets, nt)
+        return fb.get_root_hashes()
+
+    def plain_step():
+        roots = []
+        for v in views:
+            fp.clear()
+            targets, nt = fp.puts_2phase_with_valuehp(v)
+            fp.hash_onepass_v2(targets, nt)
+            roots.append(fp.get_root_hash())
+        return roots
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            out = fn()
+        fb.synchronize()
+        fp.synchronize()
+        w0 = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        fb.synchronize()
+        fp.synchronize()
+        return (time.perf_counter() - w0) / steps, out
+
+    bt, broots = timed(batched_step, args.steps, args.warmup)
+    st = fb.stats()
+    pt, proots = timed(plain_step, max(1, args.steps // 5), 1)
+    if broots != proots:
+        bad = [i for i in range(ntries) if broots[i] != proots[i]][:5]
+        raise SystemExit(f"bench: batched and per-trie roots differ for tries {bad}")
+    oracle = _oracle()
+    rng = np.random.default_rng(0)
+    sample = sorted(set(rng.choice(ntries, 48, replace=False).tolist()) | {int(np.argmax(sizes))})
+    for i in sample:
+        want = oracle.root_of(b.subset(range(int(starts[i]), int(starts[i] + sizes[i]))))
+        if broots[i] != want:
+            raise SystemExit(f"bench: trie {i} root {broots[i].hex()} != oracle {want.hex()}")
+    out = {
+        "metric": "batched tries/s (4,096 uneven tries, insert + root hash)", "value": ntries / bt,
+        "unit": "tries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": bt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/u64 (integer)",
+        "data": "synthetic (seeded lognormal trie sizes, uniform 32-byte keys, 32-byte random values)",
+        "config": {"workload": "4,096 tries, sizes lognormal(median 256, sigma 2.046) clipped to 1..200,000, "
+                               "one multi-trie PhaseNU batch + one PhaseHC pass, inputs resident in HBM",
+                   "tries": ntries, "keys": int(b.n)},
+        "plain": {"tries_per_s": ntries / pt, "ms_per_pass": pt * 1e3,
+                  "how": "one insert + hash + root call per trie on a reused single-trie forest"},
+        "speedup_vs_plain": pt / bt,
+        "mputs_per_s": b.n / bt / 1e6,
+        "per_step": {"permutations": st["perms"], "nodes_hashed": st["hashed_nodes"],
+                     "hash_ms": st["last_hash_ms"], "launches": st["hash_launches"]},
+        "roots_checked_vs_oracle": len(sample), "gen_s": gen_s,
+    }
+    if not args.no_cpu_baseline:
+        # sequential oracle build + hash of whole tries, on one core, ~10 s sample
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except (AttributeError, OSError):
+            pass
+        order = rng.permutation(ntries)
+        done, keys, secs = 0, 0, 0.0
+        for i in order:
+            sub = b.subset(range(int(starts[i]), int(starts[i] + sizes[i])))
+            dt, _ = oracle.time_build(sub)
+            secs += dt
+            done += 1
+            keys += int(sizes[i])
+            if secs > 10.0:
+                break
+        out["cpu_baseline"] = {"value": done / secs, "unit": "tries/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} random whole tries ({keys} keys), puts + hash each, {secs:.1f} s",
+                               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    else:
+        out["cpu_baseline"] = None
+    fb.close()
+    fp.close()
+    db.free()
+    return out
+
+
+def bench_lock(pkg, args):
+    """LockNU under Zipf(0.99) skew: 4M preloaded keys, then 1M overwrite/insert
+    ops with 1,000-byte values; the timed region is the ops' insert + hash."""
+    wl = pkg.workloads
+    t0 = time.time()
+    pre, ops = wl.config3_lock()
+    gen_s = time.time() - t0
+    dpre = pkg.DeviceBatch(pre, adopt_values=True)
+    dops = pkg.DeviceBatch(ops, adopt_values=True)
+    t = pkg.MPT(device=0)
+    roots, times, stats = [], [], []
+    for k in range(args.warmup + args.steps):
+        t.clear()
+        targets, nt = t.puts_2phase_with_valuehp(dpre)
+        t.hash_onepass_v2(targets, nt)
+        t.synchronize()
+        w0 = time.perf_counter()
+        targets, nt = t.puts_latching_with_valuehp_v2(dops)
+        s_ins = t.stats()
+        t.hash_onepass_v2(targets, nt)
+        roots.append(t.get_root_hash())
+        dt = time.perf_counter() - w0
+        if k >= args.warmup:
+            times.append(dt)
+            st = t.stats()
+            st["restarts"], st["lock_fails"] = s_ins["restarts"], s_ins["lock_fails"]
+            stats.append(st)
+    if len(set(roots)) != 1:
+        raise SystemExit("bench: LockNU roots differ between steps")
+    sec = float(np.mean(times))
+    out = {
+        "metric": "LockNU puts+root-hash ops/s (Zipf 0.99, 50% overwrite)", "value": ops.n / sec,
+        "unit": "puts+root-hash/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/u64 (integer)",
+        "data": "synthetic ('user'+19-digit Zipf ids over 10M, 1,000-byte random values)",
+        "config": {"workload": "4M preloaded keys (untimed), then 1M LockNU ops, 50% overwrite / 50% insert, "
+                               "Zipf(0.99) ids, 1,000-byte values, then one-pass root hash",
+                   "preload": pre.n, "ops": ops.n},
+        "per_step": {"restarts": stats[-1]["restarts"], "lock_fails": stats[-1]["lock_fails"],
+                     "permutations": stats[-1]["perms"], "nodes_hashed": stats[-1]["hashed_nodes"],
+                     "hash_ms": stats[-1]["last_hash_ms"]},
+        "root": roots[-1].hex(), "gen_s": gen_s,
+    }
+    if not args.no_cpu_baseline:
+        oracle = _oracle()
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except (AttributeError, OSError):
+            pass
+        o = oracle.OracleMPT()
+        o.puts(pre)
+        o.hash()
+        w0 = time.perf_counter()
+        o.puts(ops)
+        o.hash()
+        secs = time.perf_counter() - w0
+        if o.root() != roots[-1]:
+            raise SystemExit(f"bench: GPU root {roots[-1].hex()} != oracle root {o.root().hex()}")
+        out["root_matches_oracle"] = True
+        out["cpu_baseline"] = {"value": ops.n / secs, "unit": "puts+root-hash/s", "cores": 1, "kind": "port",
+                               "sample": f"the full 1M ops applied sequentially to the preloaded trie, {secs:.1f} s",
+                               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    else:
+        out["cpu_baseline"] = None
+    t.close()
+    dpre.free()
+    dops.free()
+    return out
+
+
+def _poisson_run(submit, reqs, rate, duration, rng):
+    """Submits reqs cyclically at Poisson arrivals of `rate` per second for
+    `duration` seconds; returns the number submitted."""
+    t0 = time.perf_counter()
+    t_next = t0
+    k = 0
+    while True:
+        t_next += rng.exponential(1.0 / rate)
+        if t_next - t0 > duration:
+            return k
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        submit(*reqs[k % len(reqs)])
+        k += 1
+
+
+def _pct(lat):
+    lat = np.asarray(lat, np.float64)
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "requests": int(lat.size)}
+
+
+def bench_served(pkg, args):
+    """Continuous batching: Poisson arrivals of insert+hash requests over 1,024
+    resident tries (Zipf 0.8), against one plain insert+hash call per request."""
+    import queue
+    import threading
+    wl = pkg.workloads
+    gmpt = pkg.gmpt
+    tries = 1024
+    t0 = time.time()
+    pool = wl.served_requests(n_requests=args.pool)
+    pre = wl.served_requests(seed=55, n_requests=tries)
+    gen_s = time.time() - t0
+    rng = np.random.default_rng(5)
+    srv = gmpt.Server(device=0, num_tries=tries, lanes=args.lanes, max_batch_keys=args.max_batch_keys)
+    for t in range(tries):  # resident tries, pre-populated (untimed)
+        srv.submit(t, pre[t][1])
+    srv.drain()
+
+    # saturation: a backlog of requests submitted at once
+    n0 = srv.stats()["completed"]
+    w0 = time.perf_counter()
+    for t, b in pool[:args.backlog]:
+        srv.submit(t, b)
+    srv.drain()
+    sat_s = time.perf_counter() - w0
+    sat_rps = args.backlog / sat_s
+    sat_keys = sum(b.n for _, b in pool[:args.backlog])
+
+    # offered load: Poisson arrivals at a fraction of saturation
+    runs = []
+    for frac in args.rates:
+        base = len(srv.latencies())
+        st0 = srv.stats()
+        w0 = time.perf_counter()
+        k = _poisson_run(lambda t, b: srv.submit(t, b), pool, frac * sat_rps, args.duration, rng)
+        srv.drain()
+        wall = time.perf_counter() - w0
+        st1 = srv.stats()
+        lat = srv.latencies()[base:]
+        keys = st1["batched_keys"] - st0["batched_keys"]
+        batches = st1["batches"] - st0["batches"]
+        runs.append({"offered_frac_of_saturation": frac, "offered_rps": frac * sat_rps, "requests": k,
+                     "puts_per_s": keys / wall, "requests_per_s": k / wall,
+                     "requests_per_batch": (st1["batched_requests"] - st0["batched_requests"]) / max(batches, 1),
+                     "deferred_by_memory": st1["deferred_mem"] - st0["deferred_mem"], **_pct(lat)})
+    sst = srv.stats()
+    srv.close()
+
+    # plain: one insert + hash + root call per request, FIFO, one worker
+    f = gmpt.MPT(device=0, num_tries=tries)
+    for t in range(tries):
+        b = pre[t][1]
+        b.trie_ids = np.full(b.n, t, np.int32)
+        tg, n = f.puts_2phase_with_valuehp(b)
+        f.hash_onepass_v2(tg, n)
+    for t, b in pool:
+        b.trie_ids = np.full(b.n, t, np.int32)
+
+    def plain_one(t, b):
+        tg, n = f.puts_2phase_with_valuehp(b)
+        f.hash_onepass_v2(tg, n)
+        return f.get_root_hash(t)
+
+    w0 = time.perf_counter()
+    for t, b in pool[:args.backlog]:
+        plain_one(t, b)
+    plain_sat_s = time.perf_counter() - w0
+    plain_rps = args.backlog / plain_sat_s
+    q = queue.Queue()
+    lat = []
+
+    def worker():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            ta, t, b = item
+            plain_one(t, b)
+            lat.append((time.perf_counter() - ta) * 1e3)
+
+    th = threading.Thread(target=worker)
+    th.start()
+    prate = args.rates[0] * plain_rps
+    w0 = time.perf_counter()
+    k = _poisson_run(lambda t, b: q.put((time.perf_counter(), t, b)), pool, prate, args.plain_duration, rng)
+    q.put(None)
+    th.join()
+    pwall = time.perf_counter() - w0
+    f.close()
+    main_run = runs[0]
+    out = {
+        "metric": "served puts/s with p50/p99 latency (continuous batching, 1,024 resident tries)",
+        "value": main_run["puts_per_s"], "unit": "puts/s", "n_gpus": 1, "steps": len(runs), "warmup": 0,
+        "ms_per_step": args.duration * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/u64 (integer)",
+        "data": "synthetic (Zipf(0.8) trie choice, lognormal(median 2,000) request sizes, uniform 32-byte keys, "
+                "70-110-byte values)",
+        "config": {"workload": f"Poisson arrivals at {args.rates} x measured saturation for {args.duration:.0f} s "
+                               "each, 1,024 HBM-resident tries, PhaseNU insert + PhaseHC hash per batch",
+                   "lanes": args.lanes, "max_batch_keys": args.max_batch_keys, "request_pool": len(pool)},
+        "latency": {k2: main_run[k2] for k2 in ("p50_ms", "p99_ms", "mean_ms")},
+        "saturation": {"requests_per_s": sat_rps, "puts_per_s": sat_keys / sat_s},
+        "runs": runs,
+        "plain": {"saturation_requests_per_s": plain_rps, "offered_rps": prate, "requests": k,
+                  "requests_per_s": k / pwall, **_pct(lat),
+                  "how": "one insert + hash + root call per request, FIFO, one worker thread"},
+        "speedup_vs_plain_saturation": sat_rps / plain_rps,
+        "server_stats": sst, "gen_s": gen_s, "cpu_baseline": None,
+    }
+    return out
+
+
+def bench_hbm_copy(pkg, args):
+    """Calibration: a plain 16-byte-per-lane copy of 1 GiB, the known-traffic
+    kernel that scales the FETCH_SIZE / WRITE_SIZE counters (k_copy16)."""
+    gmpt = pkg.gmpt
+    nbytes = 1 << 30
+    src, dst = gmpt.DeviceBuffer(nbytes), gmpt.DeviceBuffer(nbytes)
+    for _ in range(args.warmup):
+        gmpt.lib().gmpt_device_copy(dst.ptr, src.ptr, nbytes)
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        if gmpt.lib().gmpt_device_copy(dst.ptr, src.ptr, nbytes) != 0:
+            raise SystemExit("bench: copy failed")
+    sec = (time.perf_counter() - w0) / args.steps
+    src.free()
+    dst.free()
+    return {"metric": "HBM copy GB/s (read + write)", "value": 2 * nbytes / sec / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "uninitialised device buffers", "config": {"workload": "k_copy16, 1 GiB -> 1 GiB"},
+            "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="phase", choices=["phase"])
+    ap.add_argument("--config", default="phase", choices=["phase", "batched", "lock", "served", "hbm-copy"])
+    ap.add_argument("--duration", type=float, default=60.0, help="served: seconds per offered rate")
+    ap.add_argument("--plain-duration", type=float, default=20.0, help="served: seconds of the plain run")
+    ap.add_argument("--rates", type=float, nargs="+", default=[0.7], help="served: fractions of saturation")
+    ap.add_argument("--lanes", type=int, default=4, help="served: concurrent lanes (forests)")
+    ap.add_argument("--max-batch-keys", type=int, default=1 << 20, help="served: keys per lane batch")
+    ap.add_argument("--pool", type=int, default=1500, help="served: distinct pre-generated requests")
+    ap.add_argument("--backlog", type=int, default=400, help="served: requests of the saturation run")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -155,7 +495,8 @@ def main():
               file=sys.stderr)
         raise SystemExit(2)
     pkg = _import_package()
-    out = bench_phase(pkg, args)
+    out = {"phase": bench_phase, "batched": bench_batched, "lock": bench_lock,
+           "served": bench_served, "hbm-copy": bench_hbm_copy}[args.config](pkg, args)
     print(json.dumps(out), flush=True)
 
 

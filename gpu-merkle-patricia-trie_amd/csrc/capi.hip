@@ -9,6 +9,8 @@
 #include "../../include/libgmpt.h"
 #include "engine.h"
 #include "phasehc.h"
+#include "prims.h"
+#include "server.h"
 
 using gmpt::BatchDesc;
 using gmpt::Forest;
@@ -16,6 +18,10 @@ using gmpt::Forest;
 struct gmpt_forest {
   Forest *f = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+struct gmpt_server {
+  gmpt::Server *s = nullptr;
 };
 
 namespace {
@@ -109,6 +115,11 @@ int gmpt_get_root_hash(gmpt_forest *f, uint32_t trie, uint8_t out[32], int *size
   return f->f->root_hash(trie, out, size);
 }
 
+int gmpt_get_root_hashes(gmpt_forest *f, uint32_t first, uint32_t count, uint8_t *out, int *sizes) {
+  if (!f || (count && (!out || !sizes))) return fail("null argument");
+  return f->f->root_hashes(first, count, out, sizes);
+}
+
 int gmpt_gets(gmpt_forest *f, const gmpt_batch *keys, const uint8_t **values_hps, int *values_sizes) {
   if (!f) return fail("null forest");
   BatchDesc d;
@@ -160,6 +171,25 @@ int gmpt_reset_trie(gmpt_forest *f, uint32_t trie) {
 int gmpt_forest_clear(gmpt_forest *f) {
   if (!f) return fail("null forest");
   return f->f->clear();
+}
+
+int gmpt_set_option(gmpt_forest *f, const char *name, int64_t value) {
+  if (!f || !name) return fail("null argument");
+  return f->f->set_option(name, value);
+}
+
+int gmpt_get_option(gmpt_forest *f, const char *name, int64_t *value) {
+  if (!f || !name || !value) return fail("null argument");
+  return f->f->get_option(name, value);
+}
+
+int gmpt_autotune(gmpt_forest *f, const gmpt_batch *sample, int mode, int64_t *chosen, double *best_ms) {
+  if (!f) return fail("null forest");
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (PhaseNU) or 1 (LockNU)");
+  BatchDesc d;
+  int r;
+  if ((r = to_desc(sample, &d)) || (r = check_put_batch(sample))) return r;
+  return f->f->autotune(d, mode, chosen, best_ms);
 }
 
 int gmpt_synchronize(gmpt_forest *f) {
@@ -215,6 +245,13 @@ int gmpt_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   return 0;
 }
 
+int gmpt_device_copy(void *dst, const void *src, size_t bytes) {
+  if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return fail("copy needs 16-byte aligned sizes");
+  CAPI_TRY(gmpt::device_copy16(dst, src, bytes, nullptr));
+  CAPI_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
 int gmpt_timer_start(gmpt_forest *f) {
   if (!f) return fail("null forest");
   CAPI_TRY(hipEventRecord(f->t0, f->f->stream()));
@@ -229,34 +266,115 @@ int gmpt_timer_stop(gmpt_forest *f, float *ms) {
   return 0;
 }
 
-int gmpt_keccak256_batch(const uint8_t *data, const uint64_t *offsets, const uint32_t *lengths, uint32_t n,
-                         uint8_t *out) {
+int gmpt_keccak256_batch_mode(const uint8_t *data, const uint64_t *offsets, const uint32_t *lengths, uint32_t n,
+                              uint8_t *out, int mode) {
   if (n == 0) return 0;
+  if (!data || !offsets || !lengths || !out) return fail("null argument");
+  if (mode != 0 && mode != 1) return fail("keccak mode must be 0 (lane) or 1 (spread)");
   uint64_t total = 0;
   for (uint32_t i = 0; i < n; i++) {
     uint64_t e = offsets[i] + lengths[i];
     if (e > total) total = e;
   }
+  uint64_t sbytes = gmpt::keccak_scratch_bytes(lengths, n);
   uint8_t *d_data = nullptr, *d_out = nullptr;
-  uint64_t *d_off = nullptr;
+  uint64_t *d_off = nullptr, *d_slot = nullptr, *d_scratch = nullptr;
   uint32_t *d_len = nullptr;
+  void *d_temp = nullptr;
   int rc = 0;
   hipError_t e;
-  if ((e = hipMalloc(&d_data, total + 8)) != hipSuccess || (e = hipMalloc(&d_off, 8ull * n)) != hipSuccess ||
+  if ((e = hipMalloc(&d_data, total + 16)) != hipSuccess || (e = hipMalloc(&d_off, 8ull * n)) != hipSuccess ||
       (e = hipMalloc(&d_len, 4ull * n)) != hipSuccess || (e = hipMalloc(&d_out, 32ull * n)) != hipSuccess ||
+      (e = hipMalloc(&d_slot, 8ull * n)) != hipSuccess || (e = hipMalloc(&d_scratch, sbytes + 64)) != hipSuccess ||
+      (e = hipMalloc(&d_temp, gmpt::scan_temp_bytes(n))) != hipSuccess ||
       (e = hipMemcpy(d_data, data, total, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d_off, offsets, 8ull * n, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d_len, lengths, 4ull * n, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = gmpt::keccak_batch(d_data, d_off, d_len, n, d_out, nullptr)) != hipSuccess ||
+      (e = gmpt::keccak_batch(d_data, d_off, d_len, n, d_out, d_slot, d_scratch, d_temp, mode, nullptr)) !=
+          hipSuccess ||
       (e = hipMemcpy(out, d_out, 32ull * n, hipMemcpyDeviceToHost)) != hipSuccess) {
     gmpt::set_error(std::string("keccak batch: ") + hipGetErrorString(e));
     rc = -1;
   }
-  if (d_data) (void)hipFree(d_data);
-  if (d_off) (void)hipFree(d_off);
-  if (d_len) (void)hipFree(d_len);
-  if (d_out) (void)hipFree(d_out);
+  void *bufs[] = {d_data, d_off, d_len, d_out, d_slot, d_scratch, d_temp};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
   return rc;
+}
+
+int gmpt_keccak256_batch(const uint8_t *data, const uint64_t *offsets, const uint32_t *lengths, uint32_t n,
+                         uint8_t *out) {
+  return gmpt_keccak256_batch_mode(data, offsets, lengths, n, out, 0);
+}
+
+// ------------------------------------------------------------------ server --
+
+int gmpt_server_create(int device, uint32_t num_tries, uint32_t lanes, uint64_t mem_budget, uint32_t max_batch_keys,
+                       int mode, gmpt_server **out) {
+  if (!out) return fail("null output");
+  *out = nullptr;
+  std::unique_ptr<gmpt::Server> s(new (std::nothrow) gmpt::Server(device, num_tries, lanes, mem_budget,
+                                                                   max_batch_keys, mode));
+  if (!s) return fail("out of host memory");
+  if (!s->ok()) return -1;
+  gmpt_server *h = new (std::nothrow) gmpt_server;
+  if (!h) return fail("out of host memory");
+  h->s = s.release();
+  *out = h;
+  return 0;
+}
+
+void gmpt_server_destroy(gmpt_server *s) {
+  if (!s) return;
+  delete s->s;
+  delete s;
+}
+
+int gmpt_server_submit(gmpt_server *s, uint32_t trie, const gmpt_batch *b, uint64_t *req_id) {
+  if (!s || !req_id) return fail("null argument");
+  BatchDesc d;
+  int r;
+  if ((r = to_desc(b, &d)) || (r = check_put_batch(b))) return r;
+  return s->s->submit(trie, d, req_id);
+}
+
+int gmpt_server_poll(gmpt_server *s, uint64_t req_id, int *state, uint8_t root[32], int *root_size,
+                     double *latency_ms) {
+  if (!s || !state) return fail("null argument");
+  return s->s->poll(req_id, state, root, root_size, latency_ms);
+}
+
+int gmpt_server_wait(gmpt_server *s, uint64_t req_id, int *state, uint8_t root[32], int *root_size,
+                     double *latency_ms) {
+  if (!s || !state) return fail("null argument");
+  return s->s->wait(req_id, state, root, root_size, latency_ms);
+}
+
+int gmpt_server_drain(gmpt_server *s) {
+  if (!s) return fail("null server");
+  return s->s->drain();
+}
+
+int gmpt_server_get_stats(gmpt_server *s, gmpt_server_stats *out) {
+  if (!s || !out) return fail("null argument");
+  gmpt::SrvStats st = s->s->stats();
+  out->submitted = st.submitted;
+  out->completed = st.completed;
+  out->failed = st.failed;
+  out->batches = st.batches;
+  out->batched_requests = st.batched_requests;
+  out->batched_keys = st.batched_keys;
+  out->deferred_mem = st.deferred_mem;
+  out->busy_s = st.busy_s;
+  return 0;
+}
+
+int gmpt_server_latencies(gmpt_server *s, double *out, uint64_t cap, uint64_t *count) {
+  if (!s || !count) return fail("null argument");
+  std::vector<double> v = s->s->latencies();
+  *count = v.size();
+  if (out) memcpy(out, v.data(), sizeof(double) * std::min<uint64_t>(cap, v.size()));
+  return 0;
 }
 
 // ------------------------------------------------------------- libgmpt.h --

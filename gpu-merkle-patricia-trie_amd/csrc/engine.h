@@ -58,6 +58,9 @@ class Forest {
   // PhaseHC over the dirty list, then every trie's root hash is refreshed.
   int hash(void *targets, int n_targets);
   int root_hash(uint32_t trie, uint8_t *out, int *size);
+  // Root hashes of tries [first, first + count) in one copy: 32-byte slots
+  // (zero padded) and sizes.
+  int root_hashes(uint32_t first, uint32_t count, uint8_t *out, int32_t *sizes);
   // get: value size (-1 absent) and the caller's host pointer per key.
   int gets(const BatchDesc &keys, uint64_t *hps, int32_t *sizes);
   // get with value bytes copied out (offsets per key into out).
@@ -70,6 +73,13 @@ class Forest {
   // Empties every trie and recycles all pool memory (no reallocation).
   int clear();
   int synchronize();
+  // Tunables by name ("hc_spread_max": levels with at most this many dirty
+  // nodes use the spread Keccak layout).
+  int set_option(const std::string &name, int64_t value);
+  int get_option(const std::string &name, int64_t *value) const;
+  // Times PhaseHC on `sample` (inserted into a scratch forest on the same
+  // device) for each candidate spread threshold and keeps the fastest.
+  int autotune(const BatchDesc &sample, int mode, int64_t *chosen, double *best_ms);
   uint32_t ntries() const { return ntries_; }
   uint64_t pool_bytes_used() const;
   uint64_t pool_bytes_reserved() const;
@@ -102,8 +112,9 @@ class Forest {
   size_t vchunk_used_ = 0;
   DevBuf st_keys_, st_kidx_, st_vidx_, st_hps_, st_tid_;
   DevBuf ws_, ws2_;
-  DevBuf hc_sorted_, hc_hist_, hc_cursor_;
-  std::vector<uint32_t> h_hist_;
+  DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_;
+  void *h_meta_ = nullptr;       // pinned HcMeta mirror
+  uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   OpStats stats_;
 };

@@ -1,6 +1,7 @@
 // engine.hip -- host runtime of the forest: pools, staging, PhaseNU / PhaseHC.
 #include "engine.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -78,7 +79,9 @@ Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
   if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return;
   if (grow(tries_, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1), 0) != 0) return;
   if (hipMemset(tries_.p, 0, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1)) != hipSuccess) return;
-  h_hist_.resize(kHcBins);
+  if (hipHostMalloc(&h_meta_, sizeof(HcMeta)) != hipSuccess) return;
+  hc_spread_max_ = kHcSpreadMaxDefault;
+  if (const char *e = getenv("GMPT_HC_SPREAD_MAX")) hc_spread_max_ = (uint32_t)strtoul(e, nullptr, 10);
   init_ok_ = true;
 }
 
@@ -86,7 +89,7 @@ Forest::~Forest() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
-                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_hist_, &hc_cursor_};
+                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (DevBuf &b : vchunks_)
@@ -94,6 +97,7 @@ Forest::~Forest() {
   if (d_cnt_) (void)hipFree(d_cnt_);
   if (h_cnt_) (void)hipHostFree(h_cnt_);
   if (h_scal_) (void)hipHostFree(h_scal_);
+  if (h_meta_) (void)hipHostFree(h_meta_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -429,21 +433,29 @@ int Forest::hash(void *targets, int n_targets) {
   if (n > 0) {
     Pools P = pools();
     if ((r = ensure_ws(hc_sorted_, sizeof(ref_t) * (size_t)n))) return r;
-    if ((r = ensure_ws(hc_hist_, sizeof(uint32_t) * kHcBins))) return r;
+    if ((r = ensure_ws(hc_slot_, sizeof(uint64_t) * (size_t)n))) return r;
+    if ((r = ensure_ws(hc_enc_, sizeof(uint32_t) * (size_t)n))) return r;
+    if ((r = ensure_ws(hc_meta_, sizeof(HcMeta)))) return r;
     if ((r = ensure_ws(hc_cursor_, sizeof(uint32_t) * kHcBins))) return r;
-    if ((r = ensure_ws(ws2_, scan_temp_bytes(kHcBins)))) return r;
+    if ((r = ensure_ws(ws2_, hc_temp_bytes(n)))) return r;
     ref_t *sorted = (ref_t *)hc_sorted_.p;
-    HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (uint32_t *)hc_hist_.p, (uint32_t *)hc_cursor_.p, sorted,
+    uint64_t *slot = (uint64_t *)hc_slot_.p;
+    uint32_t *encl = (uint32_t *)hc_enc_.p;
+    HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
                         ws2_.p, stream_));
-    HIP_TRY(hipMemcpyAsync(h_hist_.data(), hc_hist_.p, sizeof(uint32_t) * kHcBins, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
+    const HcMeta &M = *(const HcMeta *)h_meta_;
+    if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
+    uint64_t *scratch = (uint64_t *)hc_scratch_.p;
     HIP_TRY(hipEventRecord(ev0_, stream_));
     uint32_t start = 0;
     for (uint32_t lv = 0; lv < kHcLevels; lv++) {
       uint32_t cnt = 0;
-      for (uint32_t c = 0; c < kHcClasses; c++) cnt += h_hist_[lv * kHcClasses + c];
+      for (uint32_t c = 0; c < kHcClasses; c++) cnt += M.hist[lv * kHcClasses + c];
       if (cnt) {
-        HIP_TRY(hc_hash_range(P, sorted + start, cnt, stream_));
+        HIP_TRY(hc_hash_range(P, sorted + start, slot + start, cnt, scratch, encl + start, lv == 0, hc_spread_max_,
+                              stream_));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
       }
@@ -480,6 +492,25 @@ int Forest::root_hash(uint32_t trie, uint8_t *out, int *size) {
   return 0;
 }
 
+int Forest::root_hashes(uint32_t first, uint32_t count, uint8_t *out, int32_t *sizes) {
+  int r;
+  if ((r = set_device())) return r;
+  if (first > ntries_ || count > ntries_ - first) {
+    set_error("trie range out of range");
+    return -3;
+  }
+  if (count == 0) return 0;
+  std::vector<TrieHdr> h(count);
+  HIP_TRY(hipMemcpyAsync(h.data(), (TrieHdr *)tries_.p + first, sizeof(TrieHdr) * count, hipMemcpyDeviceToHost,
+                         stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  for (uint32_t i = 0; i < count; i++) {
+    sizes[i] = (int32_t)h[i].root_hash_size;
+    memcpy(out + 32 * (size_t)i, h[i].root_hash, 32);
+  }
+  return 0;
+}
+
 int Forest::reset_trie(uint32_t trie) {
   int r;
   if ((r = set_device())) return r;
@@ -504,6 +535,58 @@ int Forest::clear() {
   for (size_t i = 1; i < vchunks_.size(); i++) HIP_TRY(hipFree(vchunks_[i].p));
   if (vchunks_.size() > 1) vchunks_.resize(1);
   vchunk_used_ = 0;
+  return 0;
+}
+
+int Forest::set_option(const std::string &name, int64_t value) {
+  if (name == "hc_spread_max") {
+    if (value < 0 || value > 0xffffffffll) {
+      set_error("hc_spread_max out of range");
+      return -3;
+    }
+    hc_spread_max_ = (uint32_t)value;
+    return 0;
+  }
+  set_error("unknown option: " + name);
+  return -3;
+}
+
+int Forest::get_option(const std::string &name, int64_t *value) const {
+  if (name == "hc_spread_max") {
+    *value = hc_spread_max_;
+    return 0;
+  }
+  set_error("unknown option: " + name);
+  return -3;
+}
+
+int Forest::autotune(const BatchDesc &sample, int mode, int64_t *chosen, double *best_ms) {
+  static const uint32_t kCand[] = {0, 256, 1024, 4096, 16384, 65536};
+  Forest t(device_, ntries_);
+  if (!t.ok()) return -1;
+  double best = 1e30;
+  uint32_t arg = hc_spread_max_;
+  for (uint32_t c : kCand) {
+    t.hc_spread_max_ = c;
+    double ms = 0;
+    for (int rep = 0; rep < 2; rep++) {  // the second run is timed (pools warm)
+      int r;
+      void *tg = nullptr;
+      int nt = 0;
+      if ((r = t.clear())) return r;
+      r = mode == 0 ? t.puts_2phase(sample, &tg, &nt) : t.puts_latching(sample, &tg, &nt);
+      if (r) return r;
+      if ((r = t.hash(tg, nt))) return r;
+      ms = t.stats().last_hash_ms;
+    }
+    if (ms < best) {
+      best = ms;
+      arg = c;
+    }
+  }
+  hc_spread_max_ = arg;
+  if (chosen) *chosen = arg;
+  if (best_ms) *best_ms = best;
   return 0;
 }
 

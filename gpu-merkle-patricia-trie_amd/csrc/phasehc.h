@@ -8,21 +8,38 @@ constexpr uint32_t kHcClasses = 8;     // Keccak block-count classes per level
 constexpr uint32_t kHcMaxDepth = 255;  // levels: 0 = leaves, 1 + (255 - depth) otherwise
 constexpr uint32_t kHcLevels = kHcMaxDepth + 2;
 constexpr uint32_t kHcBins = kHcLevels * kHcClasses;
+// Levels with at most this many nodes spread each Keccak state over 25 lanes.
+constexpr uint32_t kHcSpreadMaxDefault = 8192;
 
-// Bins dirty nodes by (level, block class): hist[bin] = count, out = the list
-// ordered by bin.
-hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, uint32_t *hist, uint32_t *cursor,
-                       ref_t *out, void *temp, hipStream_t s);
-// Encodes and hashes list[0..n) (one level: no node depends on another).
-hipError_t hc_hash_range(const Pools &P, const ref_t *list, uint32_t n, hipStream_t s);
+// Classification results, read back by the host in one copy.
+struct HcMeta {
+  uint32_t hist[kHcBins];   // nodes per (level, class) bin
+  uint32_t nvalid;          // binned nodes (the dirty list minus null holes)
+  uint32_t pad;
+  uint64_t scratch_bytes;   // sum of the nodes' scratch slot sizes
+};
+
+size_t hc_temp_bytes(uint32_t n);
+// Bins dirty nodes by (level, block class): out = the list ordered by bin,
+// slot[i] = byte offset of sorted node i's message slot in the scratch buffer.
+hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
+                       uint64_t *slot, void *temp, hipStream_t s);
+// Encodes and hashes list[0..n) (one level: no node depends on another);
+// slot and enc_len are the level's slices of the classification arrays;
+// leaves = the leaf level (level 0: only leaf short nodes).
+hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
+                         uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s);
 // Copies every trie's root hash into its header.
 hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s);
 // Node export: encoding lengths, then encodings at offs and 32-byte hash slots.
 hipError_t hc_enc_len(const Pools &P, const ref_t *refs, uint32_t m, uint64_t *len, hipStream_t s);
 hipError_t hc_enc_write(const Pools &P, const ref_t *refs, uint32_t m, const uint64_t *offs, uint8_t *encs,
                         uint8_t *hashes, hipStream_t s);
-// Keccak-256 of n messages (test hook for the permutation and sponge).
+// Keccak-256 of n messages (test hook for the sponge): scratch holds
+// keccak_scratch_bytes(len, n) bytes, slot n entries, temp scan workspace.
+// spread = 1 runs the 25-lane layout, 0 one state per lane.
+uint64_t keccak_scratch_bytes(const uint32_t *len, uint32_t n);
 hipError_t keccak_batch(const uint8_t *data, const uint64_t *off, const uint32_t *len, uint32_t n, uint8_t *out,
-                        hipStream_t s);
+                        uint64_t *slot, uint64_t *scratch, void *temp, int spread, hipStream_t s);
 
 }  // namespace gmpt

@@ -6,8 +6,19 @@
 // :2885-3241).  Here every dirty node carries its nibble depth, so the hash
 // order is fixed without counters: all leaf short nodes first (they depend on
 // nothing), then non-leaf nodes by decreasing depth (a child is always deeper
-// than its parent).  Inside a level, nodes are grouped by their Keccak block
-// count so a wave's lanes run the same number of permutations.
+// than its parent).  Nodes are binned by (level, estimated Keccak blocks), so
+// the lanes of a wave run similar numbers of permutations.
+//
+// Per level two kernels run:
+//  - encode: one thread per node writes the node's RLP encoding, Keccak
+//    padding included, into its slot of an HBM scratch buffer (slot sizes are
+//    the binning's upper bound).  An encoding shorter than 32 bytes becomes
+//    the node's "hash" directly.  The kernel holds few registers, so many
+//    waves overlap the pointer chasing to children and values.
+//  - permute: Keccak-f[1600] over the slots.  Large levels keep one 25-word
+//    state per GPU lane (VGPRs, no communication); small levels are latency
+//    bound, so there one state is spread over 25 lanes of a 32-lane group
+//    and theta, pi and chi exchange words with cross-lane permutes.
 //
 // Node encoding follows the reference's encoders exactly
 // (FullNode::encode node.cuh:414-452, ShortNode::encode node.cuh:526-559,
@@ -56,10 +67,14 @@ __device__ __forceinline__ uint64_t rlp_hdr(uint32_t L, uint32_t small, uint32_t
   return h;
 }
 
+// Keccak-256 blocks (rate 136 bytes) of a message of len bytes plus padding.
+__device__ __forceinline__ uint32_t keccak_blocks(uint32_t len) { return len / 136 + 1; }
+
 // ---------------------------------------------------------- classification --
 
-// Number of Keccak blocks a node's message needs, estimated before its
-// children are hashed (children assumed to carry 32-byte digests).
+// Keccak blocks of a node's message, bounded from above before its children
+// are hashed: a hashed child counts as a 32-byte digest, and an embedded
+// child (encoding < 32 bytes) is never longer than that.
 __device__ uint32_t estimate_blocks(const Pools &P, ref_t r, bool *leaf) {
   uint32_t t = ref_type(r), i = ref_idx(r);
   uint32_t len;
@@ -90,13 +105,14 @@ __device__ uint32_t estimate_blocks(const Pools &P, ref_t r, bool *leaf) {
     }
     len = rlp_hdr_len(pay) + pay;
   }
-  return len / 136 + 1;
+  return keccak_blocks(len);
 }
 
-__device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r) {
+__device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r, uint32_t *blocks) {
   bool leaf;
-  uint32_t blocks = estimate_blocks(P, r, &leaf);
-  uint32_t cls = blocks > kHcClasses ? kHcClasses - 1 : blocks - 1;
+  uint32_t nb = estimate_blocks(P, r, &leaf);
+  *blocks = nb;
+  uint32_t cls = nb > kHcClasses ? kHcClasses - 1 : nb - 1;
   uint32_t level;
   if (leaf) {
     level = 0;
@@ -105,6 +121,11 @@ __device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r) {
     level = 1 + (kHcMaxDepth - min(depth, (uint32_t)kHcMaxDepth));
   }
   return level * kHcClasses + cls;
+}
+
+__global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
+  uint32_t i = grid_tid();
+  if (i < n) p[i] = 0;
 }
 
 // Counting sort of the dirty list by bin.  Counts are aggregated per block in
@@ -116,28 +137,31 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *list, uin
   __syncthreads();
   for (uint32_t i = blockIdx.x * kHcTile + threadIdx.x; i < min(n, (blockIdx.x + 1) * kHcTile); i += blockDim.x) {
     ref_t r = list[i];
-    if (r != REF_NULL) atomicAdd(&h[bin_of(P, r)], 1u);  // builds leave null holes
+    uint32_t nb;
+    if (r != REF_NULL) atomicAdd(&h[bin_of(P, r, &nb)], 1u);  // builds leave null holes
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x)
     if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
+// Scatters each node to its bin and records its scratch slot size
+// (136 bytes per estimated block) at the same sorted position.
 __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *list, uint32_t n, uint32_t *cursor,
-                                                    ref_t *out) {
+                                                    ref_t *out, uint64_t *slot) {
   __shared__ uint32_t cnt[kHcBins];
   __shared__ uint32_t base[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
   constexpr int kPer = kHcTile / 256;
-  uint32_t bin[kPer], rank[kPer];
+  uint32_t bin[kPer], rank[kPer], nb[kPer];
   ref_t r[kPer];
 #pragma unroll
   for (int q = 0; q < kPer; q++) {
     uint32_t i = blockIdx.x * kHcTile + q * 256 + threadIdx.x;
     r[q] = i < n ? list[i] : REF_NULL;
     if (r[q] != REF_NULL) {
-      bin[q] = bin_of(P, r[q]);
+      bin[q] = bin_of(P, r[q], &nb[q]);
       rank[q] = atomicAdd(&cnt[bin[q]], 1u);
     }
   }
@@ -147,86 +171,21 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *list, 
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kPer; q++)
-    if (r[q] != REF_NULL) out[base[bin[q]] + rank[q]] = r[q];
+    if (r[q] != REF_NULL) {
+      uint32_t pos = base[bin[q]] + rank[q];
+      out[pos] = r[q];
+      slot[pos] = 136ull * nb[q];
+    }
 }
 
-// ------------------------------------------------------------------ hashing --
+// Slot sizes of the positions past the binned nodes (null holes of the dirty
+// list) are zero, so one scan over the whole list gives every slot's offset.
+__global__ __launch_bounds__(256) void k_hc_slot_tail(const uint32_t *nvalid, uint32_t n, uint64_t *slot) {
+  uint32_t i = grid_tid();
+  if (i < n && i >= *nvalid) slot[i] = 0;
+}
 
-// One lane's staging block in LDS: message byte k of the current block sits
-// at word k/8 of the lane; words are interleaved across the wave's 64 lanes so
-// the absorb reads (ds_read_b64) are bank-conflict free.
-struct LdsSink {
-  uint8_t *lane;   // wave region + lane * 8
-  __device__ __forceinline__ void put(uint32_t k, uint32_t v) { lane[(k >> 3) * 512 + (k & 7)] = (uint8_t)v; }
-};
-// Whole encoding written to global memory (node export).
-struct MemSink {
-  uint8_t *out;
-  __device__ __forceinline__ void put(uint32_t k, uint32_t v) { out[k] = (uint8_t)v; }
-};
-
-// Emits message bytes [lo, lo + W) of a node encoding into a sink.
-template <typename Sink, uint32_t W>
-struct Stage {
-  Sink sink;
-  uint32_t lo;     // message offset of this block
-  uint32_t pos;    // message offset of the next part to emit
-  __device__ __forceinline__ void put(uint32_t m, uint32_t v) { sink.put(m - lo, v); }
-  __device__ __forceinline__ bool overlaps(uint32_t len) const { return pos < lo + W && pos + len > lo; }
-  __device__ __forceinline__ void inline_bytes(uint64_t b, uint32_t len) {
-    if (overlaps(len)) {
-      for (uint32_t q = 0; q < len; q++) {
-        uint32_t m = pos + q;
-        if (m >= lo && m < lo + W) put(m, (uint32_t)(b >> (8 * q)) & 255u);
-      }
-    }
-    pos += len;
-  }
-  __device__ __forceinline__ void mem_bytes(const uint8_t *src, uint32_t len) {
-    if (overlaps(len)) {
-      uint32_t a = pos > lo ? pos : lo;
-      uint32_t b = min(pos + len, lo + W);
-      for (uint32_t m = a; m < b; m++) put(m, src[m - pos]);
-    }
-    pos += len;
-  }
-  // write_bytes (rlp.cuh:132-155) of a memory string
-  __device__ __forceinline__ void rlp_mem(const uint8_t *src, uint32_t len) {
-    if (len == 1 && src[0] <= 0x7f) {
-      inline_bytes(src[0], 1);
-      return;
-    }
-    uint32_t hn;
-    uint64_t h = rlp_hdr(len, 0x80, 0xB7, &hn);
-    inline_bytes(h, hn);
-    mem_bytes(src, len);
-  }
-  // write_bytes of the hex-prefix (compact) form of nibbles [s, s+cnt) of a
-  // packed key, with the leaf flag (utils.cuh:372-396)
-  __device__ __forceinline__ void rlp_compact(const uint8_t *key, uint32_t s, uint32_t cnt, uint32_t term) {
-    uint32_t kc = cnt / 2 + 1;
-    uint32_t flag = (term << 5) | ((cnt & 1) << 4) | ((cnt & 1) ? packed_nibble(key, s) : 0u);
-    if (kc == 1) {  // single byte <= 0x3f: written raw
-      inline_bytes(flag, 1);
-      return;
-    }
-    uint32_t hn;
-    uint64_t h = rlp_hdr(kc, 0x80, 0xB7, &hn);
-    inline_bytes(h, hn);
-    inline_bytes(flag, 1);
-    uint32_t first = s + (cnt & 1);
-    uint32_t len = kc - 1;
-    if (overlaps(len)) {
-      uint32_t a = pos > lo ? pos : lo;
-      uint32_t b = min(pos + len, lo + W);
-      for (uint32_t m = a; m < b; m++) {
-        uint32_t q = first + 2 * (m - pos);
-        put(m, (packed_nibble(key, q) << 4) | packed_nibble(key, q + 1));
-      }
-    }
-    pos += len;
-  }
-};
+// ------------------------------------------------------------------ encode --
 
 // Unaligned 8-byte read as aligned loads plus a byte funnel shift.
 __device__ __forceinline__ uint64_t load8(const uint8_t *p) {
@@ -247,23 +206,19 @@ __device__ __forceinline__ uint64_t load_part(const uint8_t *p, uint32_t c) {
   return v & ((1ull << (8 * c)) - 1);
 }
 
-// One Keccak block of a node's message, assembled 8 bytes at a time: parts
-// are emitted in message order, bytes outside [lo, lo + 136) are skipped and
-// completed 64-bit words go to the lane's LDS words (word w at lane + 512 w).
-struct Blk {
-  uint8_t *lane;
-  uint32_t lo;
-  uint32_t pos = 0;   // message offset of the next part
+// Writes a byte stream as aligned little-endian 64-bit words.
+struct WordWriter {
+  uint64_t *out;
   uint64_t acc = 0;   // pending bytes
-  uint32_t accn = 0;
+  uint32_t accn = 0;  // number of pending bytes (0..7)
   uint32_t nw = 0;    // words written
-  __device__ __forceinline__ Blk(uint8_t *l, uint32_t block_lo) : lane(l), lo(block_lo) {}
-  __device__ __forceinline__ void push(uint64_t v, uint32_t n) {  // n in 1..8, v's upper bytes zero
+  __device__ __forceinline__ explicit WordWriter(uint64_t *o) : out(o) {}
+  // appends the low n (1..8) bytes of v; v's bytes above n must be zero
+  __device__ __forceinline__ void push(uint64_t v, uint32_t n) {
     acc |= v << (8 * accn);
     uint32_t tot = accn + n;
     if (tot >= 8) {
-      *(uint64_t *)(lane + nw * 512) = acc;
-      nw++;
+      out[nw++] = acc;
       uint32_t used = 8 - accn;
       acc = used >= 8 ? 0ull : (v >> (8 * used));
       accn = tot - 8;
@@ -271,31 +226,12 @@ struct Blk {
       accn = tot;
     }
   }
-  __device__ __forceinline__ void window(uint32_t n, uint32_t *a, uint32_t *e) const {
-    *a = pos > lo ? pos : lo;
-    *e = min(pos + n, lo + 136);
-  }
-  __device__ __forceinline__ void inline_bytes(uint64_t b, uint32_t len) {  // len <= 8
-    uint32_t a, e;
-    window(len, &a, &e);
-    if (a < e) {
-      uint32_t c = e - a;
-      uint64_t v = b >> (8 * (a - pos));
-      if (c < 8) v &= (1ull << (8 * c)) - 1;
-      push(v, c);
-    }
-    pos += len;
+  __device__ __forceinline__ void inline_bytes(uint64_t b, uint32_t len) {  // len 1..8
+    push(len < 8 ? b & ((1ull << (8 * len)) - 1) : b, len);
   }
   __device__ __forceinline__ void mem_bytes(const uint8_t *src, uint32_t len) {
-    uint32_t a, e;
-    window(len, &a, &e);
-    if (a < e) {
-      const uint8_t *sp = src + (a - pos);
-      uint32_t c = e - a;
-      for (; c >= 8; c -= 8, sp += 8) push(load8(sp), 8);
-      if (c) push(load_part(sp, c), c);
-    }
-    pos += len;
+    for (; len >= 8; len -= 8, src += 8) push(load8(src), 8);
+    if (len) push(load_part(src, len), len);
   }
   // write_bytes (rlp.cuh:132-155) of a memory string
   __device__ __forceinline__ void rlp_mem(const uint8_t *src, uint32_t len) {
@@ -308,12 +244,12 @@ struct Blk {
     inline_bytes(h, hn);
     mem_bytes(src, len);
   }
-  // write_bytes of the hex-prefix form of nibbles [s, s + cnt) of a packed
-  // key with the leaf flag (utils.cuh:372-396)
+  // write_bytes of the hex-prefix form (utils.cuh:372-396) of nibbles
+  // [s, s + cnt) of a packed key (high nibble first) with the leaf flag term
   __device__ __forceinline__ void rlp_compact(const uint8_t *key, uint32_t s, uint32_t cnt, uint32_t term) {
     uint32_t kc = cnt / 2 + 1;
     uint32_t flag = (term << 5) | ((cnt & 1) << 4) | ((cnt & 1) ? packed_nibble(key, s) : 0u);
-    if (kc == 1) {  // single byte <= 0x3f: written raw
+    if (kc == 1) {  // a single byte <= 0x3f is written raw
       inline_bytes(flag, 1);
       return;
     }
@@ -322,34 +258,39 @@ struct Blk {
     inline_bytes(h, hn);
     inline_bytes(flag, 1);
     uint32_t first = s + (cnt & 1);
+    uint32_t c = kc - 1;
     if ((first & 1) == 0) {
-      mem_bytes(key + first / 2, kc - 1);
+      mem_bytes(key + first / 2, c);
       return;
     }
-    // nibble pairs straddle bytes: out byte i = low nibble of b[i], high of b[i+1]
-    uint32_t a, e;
-    window(kc - 1, &a, &e);
-    if (a < e) {
-      const uint8_t *sp = key + first / 2 + (a - pos);
-      uint32_t c = e - a;
-      for (; c > 0; sp += 8) {
-        uint64_t x = load8(sp), y = load8(sp + 1);
-        uint64_t v = ((x << 4) & 0xF0F0F0F0F0F0F0F0ull) | ((y >> 4) & 0x0F0F0F0F0F0F0F0Full);
-        uint32_t t = c < 8 ? c : 8;
-        if (t < 8) v &= (1ull << (8 * t)) - 1;
-        push(v, t);
-        c -= t;
-      }
+    // pairs straddle bytes: out byte i = low nibble of b[i], high nibble of b[i+1]
+    const uint8_t *sp = key + first / 2;
+    for (; c > 0; sp += 8) {
+      uint64_t x = load8(sp), y = load8(sp + 1);
+      uint64_t v = ((x << 4) & 0xF0F0F0F0F0F0F0F0ull) | ((y >> 4) & 0x0F0F0F0F0F0F0F0Full);
+      uint32_t t = c < 8 ? c : 8;
+      if (t < 8) v &= (1ull << (8 * t)) - 1;
+      push(v, t);
+      c -= t;
     }
-    pos += kc - 1;
   }
-  // flush the pending word and zero the block's remaining words
-  __device__ __forceinline__ void finish() {
-    if (accn) {
-      *(uint64_t *)(lane + nw * 512) = acc;
-      nw++;
+  // Keccak padding (0x01 after the message, 0x80 in the last byte of the
+  // last block) and zero fill up to total_words.
+  __device__ __forceinline__ void finish_keccak(uint32_t total_words) {
+    push(0x01, 1);
+    if (nw == total_words) {  // the 0x01 byte was the last byte of the block
+      out[total_words - 1] |= 0x80ull << 56;
+      return;
     }
-    for (uint32_t w = nw; w < 17; w++) *(uint64_t *)(lane + w * 512) = 0;
+    for (uint32_t w = nw; w < total_words; w++) {
+      uint64_t v = w == nw ? acc : 0ull;
+      if (w + 1 == total_words) v |= 0x80ull << 56;
+      out[w] = v;
+    }
+  }
+  // flush and zero fill up to total_words
+  __device__ __forceinline__ void finish_raw(uint32_t total_words) {
+    for (uint32_t w = nw; w < total_words; w++) out[w] = w == nw ? acc : 0ull;
   }
 };
 
@@ -379,26 +320,27 @@ __device__ __forceinline__ uint32_t child_item_len(const ChildRef &c) {
   return rlp_item_len(c.len, c.len == 1 ? c.p[0] : 0x80u);
 }
 
-// Emits the node's full encoding, keeping only the bytes of the current block.
-template <typename St>
-__device__ __forceinline__ void emit_node(const Pools &P, ref_t r, uint32_t payload, St &st) {
+// Emits the node's full encoding (list header, then items) into a writer.
+template <typename W>
+__device__ __forceinline__ void emit_node(const Pools &P, ref_t r, uint32_t payload, W &w) {
   uint32_t hn;
   uint64_t h = rlp_hdr(payload, 0xC0, 0xF7, &hn);
-  st.inline_bytes(h, hn);
+  w.inline_bytes(h, hn);
   if (ref_type(r) == T_SHORT) {
     const Short &S = P.shrt[ref_idx(r)];
-    st.rlp_compact(P.keys + S.key_off, S.key_start, S.key_len - S.term, S.term);
+    w.rlp_compact(P.keys + S.key_off, S.key_start, S.key_len - S.term, S.term);
     ChildRef c = child_bytes(P, S.child);
-    st.rlp_mem(c.p, c.len);
+    w.rlp_mem(c.p, c.len);
   } else {
     const Branch &B = P.branch[ref_idx(r)];
+#pragma unroll
     for (int s = 0; s < 17; s++) {
       ref_t c = B.child[s];
       if (c == REF_NULL) {
-        st.inline_bytes(0x80, 1);
+        w.inline_bytes(0x80, 1);
       } else {
         ChildRef cb = child_bytes(P, c);
-        st.rlp_mem(cb.p, cb.len);
+        w.rlp_mem(cb.p, cb.len);
       }
     }
   }
@@ -413,6 +355,7 @@ __device__ __forceinline__ uint32_t node_payload(const Pools &P, ref_t r) {
   }
   const Branch &B = P.branch[ref_idx(r)];
   uint32_t pay = 0;
+#pragma unroll
   for (int s = 0; s < 17; s++) {
     ref_t c = B.child[s];
     pay += c == REF_NULL ? 1u : child_item_len(child_bytes(P, c));
@@ -420,75 +363,13 @@ __device__ __forceinline__ uint32_t node_payload(const Pools &P, ref_t r) {
   return pay;
 }
 
-__global__ __launch_bounds__(256) void k_hc_hash(Pools P, const ref_t *list, uint32_t n) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  const uint32_t lane = __lane_id();
-  const uint32_t w = threadIdx.x >> 6;
-  uint8_t *region = lds + (size_t)w * (17 * 512);
-  uint8_t *mine = region + lane * 8;
-  uint32_t i = grid_tid();
-  bool active = i < n;
-  ref_t r = active ? list[i] : REF_NULL;
-  uint32_t payload = 0, enc = 0, nblocks = 0;
-  if (active) {
-    payload = node_payload(P, r);
-    enc = rlp_hdr_len(payload) + payload;
-    nblocks = enc < 32 ? 1u : enc / 136 + 1;
-  }
-  uint32_t wave_blocks = nblocks;
-  uint32_t perms = enc >= 32 ? nblocks : 0u, bytes = enc;
-  for (int off = 32; off > 0; off >>= 1) {
-    wave_blocks = max(wave_blocks, (uint32_t)__shfl_xor(wave_blocks, off));
-    perms += (uint32_t)__shfl_xor(perms, off);
-    bytes += (uint32_t)__shfl_xor(bytes, off);
-  }
-  if (lane == 0) {
-    atomicAdd(&P.cnt->n_perms, (unsigned long long)perms);
-    atomicAdd(&P.cnt->enc_bytes, (unsigned long long)bytes);
-  }
-
-  uint64_t s[25];
-#pragma unroll
-  for (int q = 0; q < 25; q++) s[q] = 0;
-
-  for (uint32_t b = 0; b < wave_blocks; b++) {
-    bool mine_b = b < nblocks;
-    if (mine_b) {
-      Blk st(mine, b * 136);
-      emit_node(P, r, payload, st);
-      st.finish();
-      if (enc >= 32 && b + 1 == nblocks) {
-        // Keccak padding: 0x01 after the message, 0x80 on the block's last byte
-        uint32_t k = enc - st.lo;
-        mine[(k >> 3) * 512 + (k & 7)] ^= 0x01;
-        mine[16 * 512 + 7] ^= 0x80;
-      }
-      if (enc >= 32) {
-#pragma unroll
-        for (int q = 0; q < 17; q++) s[q] ^= *(const uint64_t *)(mine + q * 512);
-      }
-    }
-    if (mine_b && enc >= 32) keccak_f1600(s);
-  }
-  if (!active) return;
-  uint8_t *hash;
-  if (ref_type(r) == T_SHORT)
-    hash = P.shrt[ref_idx(r)].hash;
-  else
-    hash = P.branch[ref_idx(r)].hash;
-  uint32_t hs;
-  if (enc >= 32) {
-    uint64_t *h64 = (uint64_t *)hash;
-    h64[0] = s[0];
-    h64[1] = s[1];
-    h64[2] = s[2];
-    h64[3] = s[3];
-    hs = 32;
-  } else {
-    for (uint32_t q = 0; q < enc; q++) hash[q] = mine[(q >> 3) * 512 + (q & 7)];
-    for (uint32_t q = enc; q < 32; q++) hash[q] = 0;
-    hs = enc;
-  }
+__device__ __forceinline__ uint8_t *node_hash(const Pools &P, ref_t r) {
+  return ref_type(r) == T_SHORT ? P.shrt[ref_idx(r)].hash : P.branch[ref_idx(r)].hash;
+}
+__device__ __forceinline__ uint32_t node_hash_size(const Pools &P, ref_t r) {
+  return ref_type(r) == T_SHORT ? P.shrt[ref_idx(r)].hash_size : P.branch[ref_idx(r)].hash_size;
+}
+__device__ __forceinline__ void node_done(const Pools &P, ref_t r, uint32_t hs) {
   if (ref_type(r) == T_SHORT) {
     P.shrt[ref_idx(r)].hash_size = (uint8_t)hs;
     P.shrt[ref_idx(r)].dirty = 0;
@@ -498,6 +379,372 @@ __global__ __launch_bounds__(256) void k_hc_hash(Pools P, const ref_t *list, uin
   }
 }
 
+// Generic encoder of node r (loads interleaved with stores): enc_len[i], and
+// the padded message in out (>= 32 bytes) or the raw encoding as the node's
+// "hash" (< 32 bytes, cpu_mpt.cuh:413-418).  Returns the permutations needed.
+__device__ __forceinline__ uint32_t encode_generic(const Pools &P, ref_t r, uint64_t *out, uint32_t *enc_out) {
+  uint32_t payload = node_payload(P, r);
+  uint32_t enc = rlp_hdr_len(payload) + payload;
+  *enc_out = enc;
+  if (enc < 32) {
+    WordWriter w((uint64_t *)node_hash(P, r));
+    emit_node(P, r, payload, w);
+    w.finish_raw(4);
+    node_done(P, r, enc);
+    return 0;
+  }
+  uint32_t nb = keccak_blocks(enc);
+  WordWriter w(out);
+  emit_node(P, r, payload, w);
+  w.finish_keccak(nb * 17);
+  return nb;
+}
+
+// bytes [off, off + 8) of the little-endian 16-byte pair (a, b), off in 0..7
+__device__ __forceinline__ uint64_t funnel8(uint64_t a, uint64_t b, uint32_t off) {
+  return off ? (a >> (8 * off)) | (b << (64 - 8 * off)) : a;
+}
+__device__ __forceinline__ uint64_t low_bytes(uint64_t v, uint32_t n) { return n >= 8 ? v : v & ((1ull << (8 * n)) - 1); }
+
+constexpr uint32_t kLeafFastValue = 120;  // value bytes held in registers
+constexpr uint32_t kLeafFastNibbles = 64; // key nibbles held in registers
+
+// Leaf short node [compact key, value] with a value of <= 120 bytes and a key
+// remainder of <= 64 nibbles.  Every load is issued before the first store:
+// on this GPU loads and stores share one completion counter, so a load that
+// follows a store waits for the store, and the generic encoder's interleaved
+// accesses serialise.  Returns false (nothing written) when the node does not
+// qualify or its encoding is shorter than 32 bytes.
+__device__ __forceinline__ bool encode_leaf_fast(const Pools &P, ref_t r, uint64_t *__restrict__ out,
+                                                 uint32_t *enc_out, uint32_t *nb_out) {
+  const Short &S = P.shrt[ref_idx(r)];
+  const uint64_t key_off = S.key_off;
+  const ref_t ch = S.child;
+  const uint32_t ks = S.key_start, cnt = (uint32_t)S.key_len - S.term, term = S.term;
+  if (ref_type(ch) != T_VALUE || cnt > kLeafFastNibbles) return false;
+  const Value &V = P.value[ref_idx(ch)];
+  const uint64_t va = V.addr;
+  const uint32_t vsz = V.size;
+  // packed key bytes b[0..] from byte ks / 2: up to 33 bytes, 6 aligned words
+  const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
+  const uint32_t koff = (uint32_t)(ka & 7);
+  const uint64_t *kw_p = (const uint64_t *)(ka - koff);
+  const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
+  const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;  // one byte more for the straddled pairs
+  uint64_t kw[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
+  if (vsz > kLeafFastValue) return false;
+  const uint32_t voff = (uint32_t)(va & 7);
+  const uint64_t *vw_p = (const uint64_t *)(va - voff);
+  const uint32_t nvw = (voff + vsz + 7) / 8;
+  uint64_t vw[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) vw[q] = (uint32_t)q < nvw ? vw_p[q] : 0ull;
+
+  // lengths
+  const uint32_t kc = cnt / 2 + 1;
+  const uint32_t key_item = kc == 1 ? 1u : 1u + kc;
+  const uint32_t v0 = (uint32_t)(funnel8(vw[0], vw[1], voff) & 0xff);
+  const bool v_raw = vsz == 1 && v0 <= 0x7f;
+  const uint32_t val_item = v_raw ? 1u : rlp_hdr_len(vsz) + vsz;
+  const uint32_t payload = key_item + val_item;
+  const uint32_t enc = rlp_hdr_len(payload) + payload;
+  if (enc < 32) return false;
+  const uint32_t nb = keccak_blocks(enc);
+
+  WordWriter w(out);
+  uint32_t hn;
+  uint64_t h = rlp_hdr(payload, 0xC0, 0xF7, &hn);
+  w.inline_bytes(h, hn);
+  // compact key: flag byte, then the nibble pairs
+  uint64_t b0 = funnel8(kw[0], kw[1], koff);
+  uint32_t first_nib = (ks & 1) ? (uint32_t)(b0 & 15) : (uint32_t)((b0 >> 4) & 15);
+  uint32_t flag = (term << 5) | ((cnt & 1) << 4) | ((cnt & 1) ? first_nib : 0u);
+  if (kc == 1) {
+    w.inline_bytes(flag, 1);
+  } else {
+    w.inline_bytes(0x80 + kc, 1);
+    w.inline_bytes(flag, 1);
+    const uint32_t first = ks + (cnt & 1);
+    const uint32_t d = first / 2 - ks / 2;  // 0 or 1
+    uint64_t bs[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) bs[k] = funnel8(kw[k], kw[k + 1], koff);
+    uint32_t c = kc - 1;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if ((uint32_t)(8 * k) < c) {
+        uint64_t v;
+        if (first & 1) {  // pairs straddle bytes
+          uint64_t x = bs[k], y = (bs[k] >> 8) | (bs[k + 1] << 56);
+          v = ((x << 4) & 0xF0F0F0F0F0F0F0F0ull) | ((y >> 4) & 0x0F0F0F0F0F0F0F0Full);
+        } else {
+          v = d ? (bs[k] >> 8) | (bs[k + 1] << 56) : bs[k];
+        }
+        uint32_t t = min(8u, c - 8 * k);
+        w.push(low_bytes(v, t), t);
+      }
+    }
+  }
+  // value item
+  if (v_raw) {
+    w.inline_bytes(v0, 1);
+  } else {
+    h = rlp_hdr(vsz, 0x80, 0xB7, &hn);
+    w.inline_bytes(h, hn);
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      if ((uint32_t)(8 * k) < vsz) {
+        uint32_t t = min(8u, vsz - 8 * k);
+        w.push(low_bytes(funnel8(vw[k], vw[k + 1], voff), t), t);
+      }
+    }
+  }
+  w.finish_keccak(nb * 17);
+  *enc_out = enc;
+  *nb_out = nb;
+  return true;
+}
+
+__device__ __forceinline__ void count_work(const Pools &P, uint32_t perms, uint32_t bytes) {
+  for (int off = 32; off > 0; off >>= 1) {
+    perms += (uint32_t)__shfl_xor((int)perms, off);
+    bytes += (uint32_t)__shfl_xor((int)bytes, off);
+  }
+  if (__lane_id() == 0 && (perms | bytes)) {
+    atomicAdd(&P.cnt->n_perms, (unsigned long long)perms);
+    atomicAdd(&P.cnt->enc_bytes, (unsigned long long)bytes);
+  }
+}
+
+// Leaf level: one thread per leaf short node.
+__global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__restrict__ list,
+                                                        const uint64_t *__restrict__ slot, uint32_t n,
+                                                        uint64_t *__restrict__ scratch,
+                                                        uint32_t *__restrict__ enc_len) {
+  uint32_t i = grid_tid();
+  uint32_t perms = 0, bytes = 0;
+  if (i < n) {
+    ref_t r = list[i];
+    uint64_t *out = scratch + slot[i] / 8;
+    uint32_t enc, nb;
+    if (encode_leaf_fast(P, r, out, &enc, &nb)) {
+      perms = nb;
+    } else {
+      perms = encode_generic(P, r, out, &enc);
+    }
+    enc_len[i] = enc;
+    bytes = enc;
+  }
+  count_work(P, perms, bytes);
+}
+
+constexpr uint32_t kInnerGroup = 16;              // lanes per node (one per nibble slot)
+constexpr uint32_t kInnerNodes = 256 / kInnerGroup;
+constexpr uint32_t kInnerWords = 80;              // 640 LDS bytes per node >= 4 Keccak blocks
+
+// Inner levels: 16 lanes per node.  For a branch whose 16 nibble slots hold
+// only empty slots and hashed (32-byte) children, and whose slot 16 is empty
+// (every branch of fixed-length keys), lane s loads child s's digest, the
+// group prefix-sums the item lengths, each lane writes its item into an LDS
+// copy of the padded message, and the group streams the message words to the
+// scratch slot.  Other nodes (extension short nodes, branches with embedded
+// children or a value) take the generic encoder on the group's first lane.
+__global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *__restrict__ list,
+                                                         const uint64_t *__restrict__ slot, uint32_t n,
+                                                         uint64_t *__restrict__ scratch,
+                                                         uint32_t *__restrict__ enc_len) {
+  __shared__ uint64_t lds[kInnerNodes][kInnerWords];
+  const uint32_t g = threadIdx.x / kInnerGroup, s = threadIdx.x % kInnerGroup;
+  const uint32_t i = blockIdx.x * kInnerNodes + g;
+  const bool active = i < n;
+  const ref_t r = active ? list[i] : REF_NULL;
+  const bool is_branch = ref_type(r) == T_BRANCH;
+  ref_t c = REF_NULL, c16 = REF_NULL;
+  if (is_branch) {
+    const Branch &B = P.branch[ref_idx(r)];
+    c = B.child[s];
+    c16 = B.child[16];
+  }
+  const uint32_t ct = ref_type(c);
+  const bool node_child = ct == T_BRANCH || ct == T_SHORT;
+  uint64_t h[4] = {0, 0, 0, 0};
+  uint32_t hs = 0;
+  if (node_child) {
+    const uint64_t *hp = (const uint64_t *)node_hash(P, c);
+#pragma unroll
+    for (int q = 0; q < 4; q++) h[q] = hp[q];
+    hs = node_hash_size(P, c);
+  }
+  const bool ok = c == REF_NULL || (node_child && hs == 32);
+  const uint64_t bad = __ballot(!ok);
+  const uint32_t gbase = __lane_id() & ~(kInnerGroup - 1);
+  const bool fast = active && is_branch && c16 == REF_NULL && ((bad >> gbase) & 0xffffull) == 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kInnerWords / kInnerGroup; q++) lds[g][s + kInnerGroup * q] = 0;
+  __syncthreads();
+  uint32_t enc = 0, nb = 0;
+  if (fast) {
+    const uint32_t len = c == REF_NULL ? 1u : 33u;
+    uint32_t incl = len;
+#pragma unroll
+    for (uint32_t d = 1; d < kInnerGroup; d <<= 1) {
+      uint32_t t = (uint32_t)__shfl_up((int)incl, d, kInnerGroup);
+      if (s >= d) incl += t;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, kInnerGroup - 1, kInnerGroup);
+    const uint32_t payload = total + 1;  // slot 16 is empty: one 0x80 byte
+    uint32_t hn;
+    const uint64_t hdr = rlp_hdr(payload, 0xC0, 0xF7, &hn);
+    enc = hn + payload;
+    nb = keccak_blocks(enc);
+    uint8_t *b = (uint8_t *)lds[g];
+    const uint32_t pos = hn + incl - len;
+    if (c == REF_NULL) {
+      b[pos] = 0x80;
+    } else {
+      b[pos] = 0xa0;
+#pragma unroll
+      for (int q = 0; q < 32; q++) b[pos + 1 + q] = (uint8_t)(h[q >> 3] >> (8 * (q & 7)));
+    }
+    if (s == 0) {
+      for (uint32_t q = 0; q < hn; q++) b[q] = (uint8_t)(hdr >> (8 * q));
+      b[enc - 1] = 0x80;  // slot 16
+      b[enc] = 0x01;      // Keccak padding
+      b[136 * nb - 1] |= 0x80;
+    }
+  }
+  __syncthreads();
+  uint32_t perms = 0, bytes = 0;
+  if (fast) {
+    uint64_t *out = scratch + slot[i] / 8;
+    for (uint32_t w = s; w < 17 * nb; w += kInnerGroup) out[w] = lds[g][w];
+    if (s == 0) {
+      enc_len[i] = enc;
+      perms = nb;
+      bytes = enc;
+    }
+  } else if (active && s == 0) {
+    perms = encode_generic(P, r, scratch + slot[i] / 8, &enc);
+    enc_len[i] = enc;
+    bytes = enc;
+  }
+  count_work(P, perms, bytes);
+}
+
+// ----------------------------------------------------------------- permute --
+
+// Sponge over nb padded blocks with one state per lane.
+__device__ __forceinline__ void keccak_lane(const uint64_t *m, uint32_t nb, uint64_t s[25]) {
+#pragma unroll
+  for (int q = 0; q < 25; q++) s[q] = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+#pragma unroll
+    for (int q = 0; q < 17; q++) s[q] ^= m[17 * b + q];
+    keccak_f1600(s);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list, const uint64_t *slot,
+                                                      const uint32_t *enc_len, uint32_t n,
+                                                      const uint64_t *scratch) {
+  uint32_t i = grid_tid();
+  if (i >= n) return;
+  uint32_t enc = enc_len[i];
+  if (enc < 32) return;
+  uint64_t s[25];
+  keccak_lane(scratch + slot[i] / 8, keccak_blocks(enc), s);
+  ref_t r = list[i];
+  uint64_t *h = (uint64_t *)node_hash(P, r);
+  h[0] = s[0];
+  h[1] = s[1];
+  h[2] = s[2];
+  h[3] = s[3];
+  node_done(P, r, 32);
+}
+
+// Keccak-f[1600] with one state spread over lanes 0..24 of a 32-lane group:
+// lane t holds word A[t % 5, t / 5] as (l, h) halves.  Lanes 25..31 compute
+// values nobody reads.  Shuffles stay inside the 32-lane group.
+struct SpreadLane {
+  uint32_t t, x, y, rho, pi_src, n1, n2;
+  __device__ __forceinline__ explicit SpreadLane(uint32_t gl) {
+    t = gl < 25 ? gl : 0;
+    x = t % 5;
+    y = t / 5;
+    rho = (uint32_t)kKeccak.rho[t];
+    // pi moves A[x', y'] to B[y', 2x' + 3y']; this lane (x, y) receives from
+    // x' = 3 (y - 3x) mod 5, y' = x
+    pi_src = (3 * (y + 5 - (3 * x) % 5)) % 5 + 5 * x;
+    n1 = (x + 1) % 5 + 5 * y;
+    n2 = (x + 2) % 5 + 5 * y;
+  }
+};
+
+__device__ __forceinline__ uint32_t sh32(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src, 32); }
+
+__device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const SpreadLane &c) {
+#pragma unroll 1
+  for (int ir = 0; ir < 24; ir++) {
+    // theta: column parity C[x], then A ^= C[x - 1] ^ rotl(C[x + 1], 1)
+    uint32_t cl = xor3(xor3(sh32(l, c.x), sh32(l, c.x + 5), sh32(l, c.x + 10)), sh32(l, c.x + 15), sh32(l, c.x + 20));
+    uint32_t ch = xor3(xor3(sh32(h, c.x), sh32(h, c.x + 5), sh32(h, c.x + 10)), sh32(h, c.x + 15), sh32(h, c.x + 20));
+    uint32_t xm = (c.x + 4) % 5, xp = (c.x + 1) % 5;
+    uint32_t cml = sh32(cl, xm), cmh = sh32(ch, xm), cpl = sh32(cl, xp), cph = sh32(ch, xp);
+    l = xor3(l, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
+    h = xor3(h, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
+    // rho: rotate this lane's word by its own offset
+    uint32_t rr = c.rho & 31;
+    uint32_t a = c.rho >= 32 ? h : l, b = c.rho >= 32 ? l : h;
+    uint32_t nl = rr ? __builtin_amdgcn_alignbit(a, b, 32 - rr) : a;
+    uint32_t nh = rr ? __builtin_amdgcn_alignbit(b, a, 32 - rr) : b;
+    // pi
+    uint32_t bl = sh32(nl, c.pi_src), bh = sh32(nh, c.pi_src);
+    // chi
+    l = chi3(bl, sh32(bl, c.n1), sh32(bl, c.n2));
+    h = chi3(bh, sh32(bh, c.n1), sh32(bh, c.n2));
+    // iota
+    if (c.t == 0) {
+      uint64_t rc = kKeccak.rc[ir];
+      l ^= (uint32_t)rc;
+      h ^= (uint32_t)(rc >> 32);
+    }
+  }
+}
+
+// Sponge over nb padded blocks, state spread over a 32-lane group; every lane
+// of the group must call it with the same m and nb.
+__device__ __forceinline__ void sponge_spread(const uint64_t *m, uint32_t nb, uint32_t gl, const SpreadLane &c,
+                                              uint32_t &l, uint32_t &h) {
+  l = 0;
+  h = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    if (gl < 17) {
+      uint64_t w = m[17 * b + gl];
+      l ^= (uint32_t)w;
+      h ^= (uint32_t)(w >> 32);
+    }
+    keccak_spread(l, h, c);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *list, const uint64_t *slot,
+                                                        const uint32_t *enc_len, uint32_t n,
+                                                        const uint64_t *scratch) {
+  const uint32_t gl = __lane_id() & 31;
+  const uint32_t i = blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32;
+  uint32_t enc = i < n ? enc_len[i] : 0;
+  uint32_t nb = enc >= 32 ? keccak_blocks(enc) : 0;  // uniform over the group
+  SpreadLane c(gl);
+  uint32_t l, h;
+  sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, gl, c, l, h);
+  if (nb == 0) return;
+  ref_t r = list[i];
+  if (gl < 4) ((uint64_t *)node_hash(P, r))[gl] = ((uint64_t)h << 32) | l;
+  if (gl == 0) node_done(P, r, 32);
+}
+
 __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries) {
   uint32_t t = grid_tid();
   if (t >= ntries) return;
@@ -505,55 +752,91 @@ __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries) {
   ref_t r = T.root;
   uint32_t hs = 0;
   const uint8_t *h = nullptr;
-  if (ref_type(r) == T_SHORT) {
-    hs = P.shrt[ref_idx(r)].hash_size;
-    h = P.shrt[ref_idx(r)].hash;
-  } else if (ref_type(r) == T_BRANCH) {
-    hs = P.branch[ref_idx(r)].hash_size;
-    h = P.branch[ref_idx(r)].hash;
+  if (ref_type(r) == T_SHORT || ref_type(r) == T_BRANCH) {
+    hs = node_hash_size(P, r);
+    h = node_hash(P, r);
   }
   for (int q = 0; q < 32; q++) T.root_hash[q] = q < (int)hs ? h[q] : 0;
   T.root_hash_size = hs;
 }
 
-// Standalone batch Keccak-256: message i is bytes [off[i], off[i] + len[i]) of
-// data; digests to out[32 i].  Used to test the permutation and sponge.
-__global__ __launch_bounds__(256) void k_keccak_batch(const uint8_t *data, const uint64_t *off,
-                                                      const uint32_t *len, uint32_t n, uint8_t *out) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  const uint32_t lane = __lane_id();
-  uint8_t *mine = lds + (size_t)(threadIdx.x >> 6) * (17 * 512) + lane * 8;
+// ---------------------------------------------- standalone Keccak (test hook) --
+
+__global__ __launch_bounds__(256) void k_kb_slots(const uint32_t *len, uint32_t n, uint64_t *slot) {
   uint32_t i = grid_tid();
-  bool active = i < n;
-  uint32_t L = active ? len[i] : 0;
-  const uint8_t *src = active ? data + off[i] : data;
-  uint32_t nblocks = active ? L / 136 + 1 : 0;
-  uint32_t wb = nblocks;
-  for (int o = 32; o > 0; o >>= 1) wb = max(wb, (uint32_t)__shfl_xor(wb, o));
-  uint64_t s[25];
-#pragma unroll
-  for (int q = 0; q < 25; q++) s[q] = 0;
-  for (uint32_t b = 0; b < wb; b++) {
-    bool mb = b < nblocks;
-    if (mb) {
-      Blk st(mine, b * 136);
-      st.mem_bytes(src, L);
-      st.finish();
-      if (b + 1 == nblocks) {
-        uint32_t k = L - st.lo;
-        mine[(k >> 3) * 512 + (k & 7)] ^= 0x01;
-        mine[16 * 512 + 7] ^= 0x80;
-      }
-#pragma unroll
-      for (int q = 0; q < 17; q++) s[q] ^= *(const uint64_t *)(mine + q * 512);
-      keccak_f1600(s);
-    }
-  }
-  if (!active) return;
-  for (int q = 0; q < 32; q++) out[32 * (size_t)i + q] = (uint8_t)(s[q / 8] >> (8 * (q % 8)));
+  if (i < n) slot[i] = 136ull * keccak_blocks(len[i]);
 }
 
-// Node export: encoding length of each node.
+__global__ __launch_bounds__(256) void k_kb_pad(const uint8_t *data, const uint64_t *off, const uint32_t *len,
+                                                uint32_t n, const uint64_t *slot, uint64_t *scratch) {
+  uint32_t i = grid_tid();
+  if (i >= n) return;
+  uint32_t L = len[i];
+  WordWriter w(scratch + slot[i] / 8);
+  if (L) w.mem_bytes(data + off[i], L);
+  w.finish_keccak(keccak_blocks(L) * 17);
+}
+
+__global__ __launch_bounds__(256) void k_kb_lane(const uint64_t *slot, const uint32_t *len, uint32_t n,
+                                                 const uint64_t *scratch, uint8_t *out) {
+  uint32_t i = grid_tid();
+  if (i >= n) return;
+  uint64_t s[25];
+  keccak_lane(scratch + slot[i] / 8, keccak_blocks(len[i]), s);
+  for (int q = 0; q < 4; q++) ((uint64_t *)(out + 32 * (size_t)i))[q] = s[q];
+}
+
+__global__ __launch_bounds__(256) void k_kb_spread(const uint64_t *slot, const uint32_t *len, uint32_t n,
+                                                   const uint64_t *scratch, uint8_t *out) {
+  const uint32_t gl = __lane_id() & 31;
+  const uint32_t i = blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32;
+  uint32_t nb = i < n ? keccak_blocks(len[i]) : 0;
+  SpreadLane c(gl);
+  uint32_t l, h;
+  sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, gl, c, l, h);
+  if (nb && gl < 4) ((uint64_t *)(out + 32 * (size_t)i))[gl] = ((uint64_t)h << 32) | l;
+}
+
+// ------------------------------------------------------------- node export --
+
+// Byte-granular writer for the export (encodings are concatenated unaligned).
+struct ByteWriter {
+  uint8_t *out;
+  uint32_t pos = 0;
+  __device__ __forceinline__ explicit ByteWriter(uint8_t *o) : out(o) {}
+  __device__ __forceinline__ void inline_bytes(uint64_t b, uint32_t len) {
+    for (uint32_t q = 0; q < len; q++) out[pos++] = (uint8_t)(b >> (8 * q));
+  }
+  __device__ __forceinline__ void mem_bytes(const uint8_t *src, uint32_t len) {
+    for (uint32_t q = 0; q < len; q++) out[pos++] = src[q];
+  }
+  __device__ __forceinline__ void rlp_mem(const uint8_t *src, uint32_t len) {
+    if (len == 1 && src[0] <= 0x7f) {
+      inline_bytes(src[0], 1);
+      return;
+    }
+    uint32_t hn;
+    uint64_t h = rlp_hdr(len, 0x80, 0xB7, &hn);
+    inline_bytes(h, hn);
+    mem_bytes(src, len);
+  }
+  __device__ __forceinline__ void rlp_compact(const uint8_t *key, uint32_t s, uint32_t cnt, uint32_t term) {
+    uint32_t kc = cnt / 2 + 1;
+    uint32_t flag = (term << 5) | ((cnt & 1) << 4) | ((cnt & 1) ? packed_nibble(key, s) : 0u);
+    if (kc == 1) {
+      inline_bytes(flag, 1);
+      return;
+    }
+    uint32_t hn;
+    uint64_t h = rlp_hdr(kc, 0x80, 0xB7, &hn);
+    inline_bytes(h, hn);
+    inline_bytes(flag, 1);
+    uint32_t first = s + (cnt & 1);
+    for (uint32_t q = 0; q + 1 < kc; q++)
+      out[pos++] = (uint8_t)((packed_nibble(key, first + 2 * q) << 4) | packed_nibble(key, first + 2 * q + 1));
+  }
+};
+
 __global__ __launch_bounds__(256) void k_enc_len(Pools P, const ref_t *refs, uint32_t m, uint64_t *len) {
   uint32_t i = grid_tid();
   if (i >= m) return;
@@ -568,17 +851,56 @@ __global__ __launch_bounds__(256) void k_enc_write(Pools P, const ref_t *refs, u
   if (i >= m) return;
   ref_t r = refs[i];
   uint32_t pay = node_payload(P, r);
-  Stage<MemSink, 0xffffffffu> st;
-  st.sink.out = encs + offs[i];
-  st.lo = 0;
-  st.pos = 0;
-  emit_node(P, r, pay, st);
-  const uint8_t *h = ref_type(r) == T_SHORT ? P.shrt[ref_idx(r)].hash : P.branch[ref_idx(r)].hash;
-  uint32_t hs = ref_type(r) == T_SHORT ? P.shrt[ref_idx(r)].hash_size : P.branch[ref_idx(r)].hash_size;
+  ByteWriter w(encs + offs[i]);
+  emit_node(P, r, pay, w);
+  const uint8_t *h = node_hash(P, r);
+  uint32_t hs = node_hash_size(P, r);
   for (int q = 0; q < 32; q++) hashes[32 * (size_t)i + q] = q < (int)hs ? h[q] : 0;
 }
 
 }  // namespace
+
+size_t hc_temp_bytes(uint32_t n) { return scan_temp_bytes(n > kHcBins ? n : kHcBins); }
+
+hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
+                       uint64_t *slot, void *temp, hipStream_t s) {
+  k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins);
+  GMPT_LAUNCH_CHECK();
+  if (n == 0) {
+    k_hc_zero<<<1, kB, 0, s>>>(&meta->nvalid, 4);  // nvalid, pad, scratch_bytes
+    return hipGetLastError();
+  }
+  unsigned tiles = (unsigned)((n + kHcTile - 1) / kHcTile);
+  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u32(meta->hist, cursor, kHcBins, &meta->nvalid, temp, s);
+  if (e != hipSuccess) return e;
+  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, n, cursor, out, slot);
+  GMPT_LAUNCH_CHECK();
+  k_hc_slot_tail<<<nblk(n), kB, 0, s>>>(&meta->nvalid, n, slot);
+  GMPT_LAUNCH_CHECK();
+  return scan_exclusive_u64(slot, slot, n, &meta->scratch_bytes, temp, s);
+}
+
+hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
+                         uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (leaves)
+    k_hc_encode_leaf<<<nblk(n), kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+  else
+    k_hc_encode_inner<<<(n + kInnerNodes - 1) / kInnerNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+  GMPT_LAUNCH_CHECK();
+  if (n <= spread_max)
+    k_hc_perm_spread<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
+  else
+    k_hc_perm_lane<<<nblk(n), kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
+  return hipGetLastError();
+}
+
+hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s) {
+  k_hc_roots<<<nblk(ntries), kB, 0, s>>>(P, ntries);
+  return hipGetLastError();
+}
 
 hipError_t hc_enc_len(const Pools &P, const ref_t *refs, uint32_t m, uint64_t *len, hipStream_t s) {
   if (m == 0) return hipSuccess;
@@ -593,37 +915,25 @@ hipError_t hc_enc_write(const Pools &P, const ref_t *refs, uint32_t m, const uin
   return hipGetLastError();
 }
 
-size_t hc_lds_bytes() { return 4 * 17 * 512; }
-
-hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, uint32_t *hist, uint32_t *cursor,
-                       ref_t *out, void *temp, hipStream_t s) {
-  hipError_t e = hipMemsetAsync(hist, 0, sizeof(uint32_t) * kHcBins, s);
-  if (e != hipSuccess) return e;
-  if (n == 0) return hipSuccess;
-  unsigned tiles = (unsigned)((n + kHcTile - 1) / kHcTile);
-  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, hist);
-  GMPT_LAUNCH_CHECK();
-  e = scan_exclusive_u32(hist, cursor, kHcBins, nullptr, temp, s);
-  if (e != hipSuccess) return e;
-  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, n, cursor, out);
-  return hipGetLastError();
-}
-
-hipError_t hc_hash_range(const Pools &P, const ref_t *list, uint32_t n, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_hc_hash<<<nblk(n), kB, hc_lds_bytes(), s>>>(P, list, n);
-  return hipGetLastError();
-}
-
-hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s) {
-  k_hc_roots<<<nblk(ntries), kB, 0, s>>>(P, ntries);
-  return hipGetLastError();
+uint64_t keccak_scratch_bytes(const uint32_t *len, uint32_t n) {
+  uint64_t t = 0;
+  for (uint32_t i = 0; i < n; i++) t += 136ull * (len[i] / 136 + 1);
+  return t;
 }
 
 hipError_t keccak_batch(const uint8_t *data, const uint64_t *off, const uint32_t *len, uint32_t n, uint8_t *out,
-                        hipStream_t s) {
+                        uint64_t *slot, uint64_t *scratch, void *temp, int spread, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_keccak_batch<<<nblk(n), kB, hc_lds_bytes(), s>>>(data, off, len, n, out);
+  k_kb_slots<<<nblk(n), kB, 0, s>>>(len, n, slot);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u64(slot, slot, n, nullptr, temp, s);
+  if (e != hipSuccess) return e;
+  k_kb_pad<<<nblk(n), kB, 0, s>>>(data, off, len, n, slot, scratch);
+  GMPT_LAUNCH_CHECK();
+  if (spread)
+    k_kb_spread<<<(n + 7) / 8, kB, 0, s>>>(slot, len, n, scratch, out);
+  else
+    k_kb_lane<<<nblk(n), kB, 0, s>>>(slot, len, n, scratch, out);
   return hipGetLastError();
 }
 

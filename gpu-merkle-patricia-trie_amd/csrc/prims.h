@@ -59,4 +59,8 @@ __device__ __forceinline__ T block_exclusive_scan_256(T v, T *sh, T *total) {
   return base + inc - v;
 }
 
+// Plain streaming copy of bytes / 16 16-byte words (a reference kernel of
+// known traffic for calibrating memory counters).
+hipError_t device_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 }  // namespace gmpt

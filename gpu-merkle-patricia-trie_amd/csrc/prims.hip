@@ -5,6 +5,8 @@
 // scatter kernel ranks the tile stably in LDS (per-wave digit match via 8
 // ballots), sorts it locally, then writes runs of equal digits contiguously.
 #include "prims.h"
+
+#include <algorithm>
 #include "common.h"
 
 namespace gmpt {
@@ -14,6 +16,7 @@ namespace {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
+constexpr size_t kScanDirectTiles = 2048;  // up to this many tiles: two launches per scan
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, size_t n, T *__restrict__ sums) {
@@ -30,9 +33,13 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, s
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// Scans one tile; its offset is offsets[blockIdx.x] (already scanned), or,
+// with tile_sums, the sum of tile_sums[0 .. blockIdx.x) computed here (used
+// when there are few tiles: saves the launch that scans the tile sums).
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *__restrict__ out, size_t n,
-                                                   const T *__restrict__ offsets, T *__restrict__ total) {
+                                                   const T *__restrict__ offsets, const T *__restrict__ tile_sums,
+                                                   T *__restrict__ total) {
   __shared__ T tile[kScanTile];
   __shared__ T sh[4];
   size_t base = (size_t)blockIdx.x * kScanTile;
@@ -40,6 +47,14 @@ __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *
   for (int i = 0; i < kScanItems; i++) {
     size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
     tile[i * kScanBlock + threadIdx.x] = j < n ? in[j] : T(0);
+  }
+  T off = 0;
+  if (tile_sums) {
+    T part = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanBlock) part += tile_sums[j];
+    block_exclusive_scan_256<T>(part, sh, &off);
+  } else if (offsets) {
+    off = offsets[blockIdx.x];
   }
   __syncthreads();
   T v[kScanItems];
@@ -51,7 +66,6 @@ __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *
   }
   T tot;
   T pre = block_exclusive_scan_256<T>(acc, sh, &tot);
-  T off = offsets ? offsets[blockIdx.x] : T(0);
   pre += off;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
@@ -85,16 +99,20 @@ hipError_t scan_impl(const T *in, T *out, size_t n, T *total, T *temp, hipStream
   }
   size_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 1) {
-    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, total);
+    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, nullptr, total);
     return hipGetLastError();
   }
   T *sums = temp;
   T *rest = temp + nb + 1;
   k_scan_reduce<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, sums);
   GMPT_LAUNCH_CHECK();
+  if (nb <= kScanDirectTiles) {  // each tile adds up the sums before it
+    k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, nullptr, sums, total);
+    return hipGetLastError();
+  }
   hipError_t e = scan_impl<T>(sums, sums, nb, nullptr, rest, s);
   if (e != hipSuccess) return e;
-  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, total);
+  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, nullptr, total);
   return hipGetLastError();
 }
 
@@ -257,6 +275,21 @@ hipError_t radix_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1
 hipError_t radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int begin_bit,
                           int end_bit, int *which, void *temp, hipStream_t s) {
   return radix_impl<uint32_t>(k0, v0, k1, v1, n, begin_bit, end_bit, which, temp, s);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t device_copy16(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  size_t n16 = bytes / 16;
+  if (n16 == 0) return hipSuccess;
+  size_t blocks = std::min<size_t>((n16 + 255) / 256, 8192);
+  k_copy16<<<(unsigned)blocks, 256, 0, s>>>((const uint4 *)src, (uint4 *)dst, n16);
+  return hipGetLastError();
 }
 
 }  // namespace gmpt

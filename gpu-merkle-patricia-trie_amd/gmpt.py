@@ -41,12 +41,25 @@ class Stats(ctypes.Structure):
 
 # every symbol the headers in include/ declare
 GMPT_SYMBOLS = ["gmpt_forest_create", "gmpt_forest_destroy", "gmpt_last_error", "gmpt_puts_2phase",
-                "gmpt_puts_latching", "gmpt_hash_onepass", "gmpt_get_root_hash", "gmpt_gets", "gmpt_get_values",
-                "gmpt_export_nodes", "gmpt_free_host", "gmpt_reset_trie", "gmpt_forest_clear", "gmpt_synchronize", "gmpt_get_stats",
+                "gmpt_puts_latching", "gmpt_hash_onepass", "gmpt_get_root_hash", "gmpt_get_root_hashes", "gmpt_gets", "gmpt_get_values",
+                "gmpt_export_nodes", "gmpt_free_host", "gmpt_reset_trie", "gmpt_forest_clear", "gmpt_synchronize",
+                "gmpt_set_option", "gmpt_get_option", "gmpt_autotune", "gmpt_get_stats",
                 "gmpt_device_count", "gmpt_device_malloc", "gmpt_device_free", "gmpt_memcpy_htod",
-                "gmpt_memcpy_dtoh", "gmpt_timer_start", "gmpt_timer_stop", "gmpt_keccak256_batch"]
+                "gmpt_memcpy_dtoh", "gmpt_device_copy", "gmpt_timer_start", "gmpt_timer_stop", "gmpt_keccak256_batch",
+                "gmpt_keccak256_batch_mode", "gmpt_server_create", "gmpt_server_destroy", "gmpt_server_submit",
+                "gmpt_server_poll", "gmpt_server_wait", "gmpt_server_drain", "gmpt_server_get_stats",
+                "gmpt_server_latencies"]
 LIBGMPT_SYMBOLS = ["build_mpt_2phase", "build_mpt_olc", "preprocess", "get_all_nodes", "init_mpt",
                    "insert_mpt_2phase", "insert_mpt_olc", "free_mpt"]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("batched_requests", ctypes.c_uint64),
+                ("batched_keys", ctypes.c_uint64), ("deferred_mem", ctypes.c_uint64), ("busy_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class _NodeSet(ctypes.Structure):
@@ -71,6 +84,7 @@ def lib():
         "gmpt_puts_latching": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(i32)], i32),
         "gmpt_hash_onepass": ([vp, vp, i32], i32),
         "gmpt_get_root_hash": ([vp, u32, vp, ctypes.POINTER(i32)], i32),
+        "gmpt_get_root_hashes": ([vp, u32, u32, vp, vp], i32),
         "gmpt_gets": ([vp, ctypes.POINTER(_Batch), vp, vp], i32),
         "gmpt_get_values": ([vp, ctypes.POINTER(_Batch), vp, i64, vp, vp], i32),
         "gmpt_export_nodes": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(vp),
@@ -79,15 +93,31 @@ def lib():
         "gmpt_reset_trie": ([vp, u32], i32),
         "gmpt_forest_clear": ([vp], i32),
         "gmpt_synchronize": ([vp], i32),
+        "gmpt_set_option": ([vp, ctypes.c_char_p, i64], i32),
+        "gmpt_get_option": ([vp, ctypes.c_char_p, ctypes.POINTER(i64)], i32),
+        "gmpt_autotune": ([vp, ctypes.POINTER(_Batch), i32, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)],
+                          i32),
         "gmpt_get_stats": ([vp, ctypes.POINTER(Stats)], i32),
         "gmpt_device_count": ([ctypes.POINTER(i32)], i32),
         "gmpt_device_malloc": ([ctypes.POINTER(vp), ctypes.c_size_t], i32),
         "gmpt_device_free": ([vp], i32),
         "gmpt_memcpy_htod": ([vp, vp, ctypes.c_size_t], i32),
         "gmpt_memcpy_dtoh": ([vp, vp, ctypes.c_size_t], i32),
+        "gmpt_device_copy": ([vp, vp, ctypes.c_size_t], i32),
         "gmpt_timer_start": ([vp], i32),
         "gmpt_timer_stop": ([vp, ctypes.POINTER(ctypes.c_float)], i32),
         "gmpt_keccak256_batch": ([vp, vp, vp, u32, vp], i32),
+        "gmpt_keccak256_batch_mode": ([vp, vp, vp, u32, vp, i32], i32),
+        "gmpt_server_create": ([i32, u32, u32, ctypes.c_uint64, u32, i32, ctypes.POINTER(vp)], i32),
+        "gmpt_server_destroy": ([vp], None),
+        "gmpt_server_submit": ([vp, u32, ctypes.POINTER(_Batch), ctypes.POINTER(ctypes.c_uint64)], i32),
+        "gmpt_server_poll": ([vp, ctypes.c_uint64, ctypes.POINTER(i32), vp, ctypes.POINTER(i32),
+                              ctypes.POINTER(ctypes.c_double)], i32),
+        "gmpt_server_wait": ([vp, ctypes.c_uint64, ctypes.POINTER(i32), vp, ctypes.POINTER(i32),
+                              ctypes.POINTER(ctypes.c_double)], i32),
+        "gmpt_server_drain": ([vp], i32),
+        "gmpt_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], i32),
+        "gmpt_server_latencies": ([vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)], i32),
         "build_mpt_2phase": ([i32, vp, vp, vp, vp, vp, i32], vp),
         "build_mpt_olc": ([i32, vp, vp, vp, vp, vp, i32], vp),
         "preprocess": ([], i32),
@@ -180,6 +210,18 @@ class DeviceBatch:
         self.struct = s
         self.n = b.n
 
+    def view(self, start, count):
+        """Keys [start, start + count) as a single-trie device batch sharing
+        this batch's buffers (index pairs keep addressing the whole buffers)."""
+        v = DeviceBatch.__new__(DeviceBatch)
+        s = _Batch.from_buffer_copy(self.struct)
+        s.keys_indexs = self.struct.keys_indexs + 8 * start
+        s.values_indexs = self.struct.values_indexs + 16 * start
+        s.trie_ids = None
+        s.n = count
+        v.bufs, v.tid, v.struct, v.n = [], None, s, count
+        return v
+
     def free(self):
         for x in self.bufs:
             x.free()
@@ -232,6 +274,14 @@ class MPT:
         sz = ctypes.c_int()
         _check(lib().gmpt_get_root_hash(self._f, trie, _p(out), ctypes.byref(sz)))
         return out[:sz.value].tobytes()
+
+    def get_root_hashes(self, first=0, count=None):
+        """Root hashes of tries [first, first + count) read with one copy."""
+        count = self.num_tries - first if count is None else count
+        out = np.zeros(32 * max(count, 1), np.uint8)
+        sz = np.zeros(max(count, 1), np.int32)
+        _check(lib().gmpt_get_root_hashes(self._f, first, count, _p(out), _p(sz)))
+        return [out[32 * i:32 * i + sz[i]].tobytes() for i in range(count)]
 
     # --- gets (gpu_mpt.cuh:139-140) -------------------------------------------
     def gets_parallel(self, batch):
@@ -289,6 +339,22 @@ class MPT:
     def synchronize(self):
         _check(lib().gmpt_synchronize(self._f))
 
+    def set_option(self, name, value):
+        _check(lib().gmpt_set_option(self._f, name.encode(), int(value)))
+
+    def get_option(self, name):
+        v = ctypes.c_int64()
+        _check(lib().gmpt_get_option(self._f, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, sample, mode=0):
+        """Picks hc_spread_max by timing PhaseHC on `sample`; returns (value, ms)."""
+        s = sample.struct if isinstance(sample, DeviceBatch) else host_batch(sample)
+        v = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _check(lib().gmpt_autotune(self._f, ctypes.byref(s), mode, ctypes.byref(v), ctypes.byref(ms)))
+        return v.value, ms.value
+
     def stats(self):
         s = Stats()
         _check(lib().gmpt_get_stats(self._f, ctypes.byref(s)))
@@ -303,8 +369,76 @@ class MPT:
         return ms.value
 
 
-def keccak256_batch(messages):
-    """Keccak-256 of each message, computed on the GPU."""
+class Server:
+    """Continuous-batching insert + hash service over `num_tries` HBM-resident
+    tries (gmpt_server_* in gmpt_amd.h)."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, -1
+
+    def __init__(self, device=0, num_tries=1, lanes=2, mem_budget=0, max_batch_keys=0, mode=0):
+        self._s = ctypes.c_void_p()
+        _check(lib().gmpt_server_create(device, num_tries, lanes, mem_budget, max_batch_keys, mode,
+                                        ctypes.byref(self._s)))
+        self._keep = {}
+
+    def submit(self, trie, batch):
+        """Queues one host batch for `trie`; returns the request id."""
+        s = host_batch(batch)
+        rid = ctypes.c_uint64()
+        _check(lib().gmpt_server_submit(self._s, trie, ctypes.byref(s), ctypes.byref(rid)))
+        return rid.value
+
+    def _result(self, fn, rid):
+        st = ctypes.c_int()
+        root = np.zeros(32, np.uint8)
+        sz = ctypes.c_int()
+        lat = ctypes.c_double()
+        _check(fn(self._s, rid, ctypes.byref(st), _p(root), ctypes.byref(sz), ctypes.byref(lat)))
+        if st.value == self.FAILED:
+            raise GmptError(lib().gmpt_last_error().decode())
+        if st.value != self.DONE:
+            return st.value, None, None
+        return st.value, root[:sz.value].tobytes(), lat.value
+
+    def poll(self, rid):
+        """(state, root or None, latency ms or None)."""
+        return self._result(lib().gmpt_server_poll, rid)
+
+    def wait(self, rid):
+        """Blocks until request `rid` completes: (root, latency ms)."""
+        _, root, lat = self._result(lib().gmpt_server_wait, rid)
+        return root, lat
+
+    def drain(self):
+        _check(lib().gmpt_server_drain(self._s))
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().gmpt_server_get_stats(self._s, ctypes.byref(s)))
+        return s.as_dict()
+
+    def latencies(self):
+        cnt = ctypes.c_uint64()
+        _check(lib().gmpt_server_latencies(self._s, None, 0, ctypes.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), np.float64)
+        _check(lib().gmpt_server_latencies(self._s, _p(out), cnt.value, ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+    def close(self):
+        if self._s and self._s.value:
+            lib().gmpt_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def keccak256_batch(messages, mode=0):
+    """Keccak-256 of each message, computed on the GPU (mode 0: one state per
+    lane, mode 1: one state spread over 25 lanes)."""
     n = len(messages)
     lens = np.array([len(m) for m in messages], np.uint32)
     offs = np.zeros(n, np.uint64)
@@ -312,7 +446,7 @@ def keccak256_batch(messages):
         offs[1:] = np.cumsum(lens[:-1].astype(np.uint64))
     data = np.frombuffer(b"".join(messages) + b"\0" * 8, np.uint8).copy()
     out = np.zeros(32 * max(n, 1), np.uint8)
-    _check(lib().gmpt_keccak256_batch(_p(data), _p(offs), _p(lens), n, _p(out)))
+    _check(lib().gmpt_keccak256_batch_mode(_p(data), _p(offs), _p(lens), n, _p(out), mode))
     return [out[32 * i:32 * i + 32].tobytes() for i in range(n)]
 
 

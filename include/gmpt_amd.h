@@ -83,6 +83,10 @@ int gmpt_hash_onepass(gmpt_forest *f, void *targets, int n_targets);
  * is shorter than 32 bytes, or 0 for an empty trie.
  * Replaces: MPT::get_root_hash (gpu_mpt.cuh:151-152, 1217-1247). */
 int gmpt_get_root_hash(gmpt_forest *f, uint32_t trie, uint8_t out[32], int *size);
+/* Root hashes of tries [first, first + count) with one device copy: out holds
+ * 32 bytes per trie (zero padded), sizes[i] as for gmpt_get_root_hash.
+ * Replaces: MPT::get_root_hash called once per trie (gpu_mpt.cuh:1217-1247). */
+int gmpt_get_root_hashes(gmpt_forest *f, uint32_t first, uint32_t count, uint8_t *out, int *sizes);
 /* For each key: the host pointer given at insert (values_hps) and the value
  * size; absent keys give NULL and size 0.
  * Replaces: MPT::gets_parallel (gpu_mpt.cuh:139-140, 984-1036). */
@@ -104,6 +108,16 @@ int gmpt_reset_trie(gmpt_forest *f, uint32_t trie);
  * which in the reference allocates a new 16 GB pool each time. */
 int gmpt_forest_clear(gmpt_forest *f);
 int gmpt_synchronize(gmpt_forest *f);
+/* Tunables by name.  "hc_spread_max": PhaseHC levels with at most this many
+ * dirty nodes run Keccak with one state spread over 25 lanes (latency), larger
+ * levels one state per lane (throughput).  Default 8192, or the
+ * GMPT_HC_SPREAD_MAX environment variable.  No reference counterpart. */
+int gmpt_set_option(gmpt_forest *f, const char *name, int64_t value);
+int gmpt_get_option(gmpt_forest *f, const char *name, int64_t *value);
+/* Autotuner: inserts `sample` (mode 0 PhaseNU, 1 LockNU) into a scratch forest
+ * on the same device, times PhaseHC for each candidate hc_spread_max, and sets
+ * the fastest on f.  No reference counterpart. */
+int gmpt_autotune(gmpt_forest *f, const gmpt_batch *sample, int mode, int64_t *chosen, double *best_ms);
 int gmpt_get_stats(gmpt_forest *f, gmpt_stats *out);
 
 /* Device helpers so callers can keep batches resident in HBM. */
@@ -112,6 +126,9 @@ int gmpt_device_malloc(void **p, size_t bytes);
 int gmpt_device_free(void *p);
 int gmpt_memcpy_htod(void *dst, const void *src, size_t bytes);
 int gmpt_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned sizes);
+ * also the known-traffic kernel that calibrates memory counters. */
+int gmpt_device_copy(void *dst, const void *src, size_t bytes);
 /* Device-time stopwatch on the forest's stream. */
 int gmpt_timer_start(gmpt_forest *f);
 int gmpt_timer_stop(gmpt_forest *f, float *ms);
@@ -119,6 +136,43 @@ int gmpt_timer_stop(gmpt_forest *f, float *ms);
  * Replaces: GPUHashMultiThread batch hashing (batch_mode_hash.cuh:89-115). */
 int gmpt_keccak256_batch(const uint8_t *data, const uint64_t *offsets, const uint32_t *lengths, uint32_t n,
                          uint8_t *out);
+/* As gmpt_keccak256_batch with the permutation layout chosen: mode 0 keeps one
+ * state per GPU lane (PhaseHC's large levels), mode 1 spreads one state over 25
+ * lanes (PhaseHC's small levels).  No reference counterpart (test hook). */
+int gmpt_keccak256_batch_mode(const uint8_t *data, const uint64_t *offsets, const uint32_t *lengths, uint32_t n,
+                              uint8_t *out, int mode);
+
+/* ---- Serving: continuous batching over HBM-resident tries ----------------
+ * No reference counterpart (the reference builds one trie per call).  A
+ * server holds num_tries resident tries split over `lanes` forests (one HIP
+ * stream and one worker thread each; trie t lives in lane t % lanes).  A
+ * request is one host batch for one trie; the server copies it at submit.
+ * Each lane repeatedly inserts every queued request it can admit (at most one
+ * per trie, so a trie's requests complete in submit order; total keys up to
+ * max_batch_keys; node-pool bytes within mem_budget / lanes) as one
+ * multi-trie batch (mode 0: PhaseNU, 1: LockNU), hashes once, and completes
+ * each request with its trie's post-state root.  mem_budget = 0 takes 80% of
+ * the device's free memory.  Request states: 0 queued, 1 running, 2 done,
+ * -1 failed (gmpt_last_error() after the poll says why). */
+typedef struct gmpt_server gmpt_server;
+typedef struct gmpt_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t batches, batched_requests, batched_keys;
+  uint64_t deferred_mem;  /* admissions postponed by the memory budget */
+  double busy_s;          /* summed lane time inside insert + hash */
+} gmpt_server_stats;
+int gmpt_server_create(int device, uint32_t num_tries, uint32_t lanes, uint64_t mem_budget, uint32_t max_batch_keys,
+                       int mode, gmpt_server **out);
+void gmpt_server_destroy(gmpt_server *s);
+int gmpt_server_submit(gmpt_server *s, uint32_t trie, const gmpt_batch *b, uint64_t *req_id);
+int gmpt_server_poll(gmpt_server *s, uint64_t req_id, int *state, uint8_t root[32], int *root_size,
+                     double *latency_ms);
+int gmpt_server_wait(gmpt_server *s, uint64_t req_id, int *state, uint8_t root[32], int *root_size,
+                     double *latency_ms);
+int gmpt_server_drain(gmpt_server *s);
+int gmpt_server_get_stats(gmpt_server *s, gmpt_server_stats *out);
+/* Latencies (ms, submit to root available) of completed requests, in request order. */
+int gmpt_server_latencies(gmpt_server *s, double *out, uint64_t cap, uint64_t *count);
 
 #ifdef __cplusplus
 }

@@ -38,15 +38,16 @@ def _rand_batch(workloads, seed, n, vlo, vhi, key_len=32):
     return workloads.batch_from_fixed(keys, flat, lens)
 
 
-def test_keccak_matches_oracle(gmpt, oracle):
+@pytest.mark.parametrize("kmode", [0, 1], ids=["lane", "spread"])
+def test_keccak_matches_oracle(gmpt, oracle, kmode):
     with open(os.path.join(GOLDEN, "keccak256.json")) as f:
         vecs = json.load(f)
     msgs = [bytes.fromhex(v["msg"]) for v in vecs]
-    got = gmpt.keccak256_batch(msgs)
+    got = gmpt.keccak256_batch(msgs, kmode)
     assert [g.hex() for g in got] == [v["digest"] for v in vecs]
     rng = random.Random(3)
     msgs = [bytes(rng.getrandbits(8) for _ in range(L)) for L in list(range(0, 420)) + [543, 544, 1087, 1088, 2000]]
-    got = gmpt.keccak256_batch(msgs)
+    got = gmpt.keccak256_batch(msgs, kmode)
     for m, g in zip(msgs, got):
         assert g == oracle.keccak256(m), len(m)
 
@@ -247,3 +248,26 @@ def test_reference_c_abi(gmpt, workloads, oracle):
     L.free_mpt(m)
     for p in (h, h2, h3, h4, ns.hashs, ns.encs, ns.encs_indexs):
         L.gmpt_free_host(p)
+
+
+@pytest.mark.parametrize("spread_max", [0, 1 << 30], ids=["lane-only", "spread-only"])
+def test_keccak_layouts_give_same_root(gmpt, workloads, oracle, spread_max):
+    b = _rand_batch(workloads, 91, 20000, 20, 300)
+    t = gmpt.MPT(device=0)
+    t.set_option("hc_spread_max", spread_max)
+    assert t.get_option("hc_spread_max") == spread_max
+    targets, n = t.puts_2phase_with_valuehp(b)
+    t.hash_onepass_v2(targets, n)
+    assert t.get_root_hash() == oracle.root_of(b)
+
+
+def test_autotune_keeps_results(gmpt, workloads, oracle):
+    b = _rand_batch(workloads, 92, 6000, 70, 110)
+    t = gmpt.MPT(device=0)
+    v, ms = t.autotune(b)
+    assert t.get_option("hc_spread_max") == v and ms > 0
+    with pytest.raises(gmpt.GmptError):
+        t.set_option("no_such_option", 1)
+    targets, n = t.puts_2phase_with_valuehp(b)
+    t.hash_onepass_v2(targets, n)
+    assert t.get_root_hash() == oracle.root_of(b)
