@@ -105,6 +105,28 @@ int gmpt_puts_latching(gmpt_forest *f, const gmpt_batch *b, void **targets, int 
   return f->f->puts_latching(d, targets, n_targets);
 }
 
+int gmpt_puts_with_read(gmpt_forest *f, const gmpt_batch *b, const uint8_t *rw_flags, int mode, int *read_num,
+                        const uint8_t **read_values_hps, int *read_values_sizes, void **targets, int *n_targets) {
+  if (!f || !read_num) return fail("null argument");
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (PhaseNU) or 1 (LockNU)");
+  BatchDesc d;
+  int r;
+  if ((r = to_desc(b, &d)) || (r = check_put_batch(b))) return r;
+  std::unique_ptr<uint64_t[]> hp(new (std::nothrow) uint64_t[d.n + 1]);
+  std::unique_ptr<int32_t[]> sz(new (std::nothrow) int32_t[d.n + 1]);
+  if (!hp || !sz) return fail("out of host memory");
+  uint32_t nr = 0;
+  r = f->f->puts_with_read(d, rw_flags, mode, &nr, hp.get(), sz.get(), targets, n_targets);
+  *read_num = (int)nr;
+  if (r) return r;
+  for (uint32_t i = 0; i < nr; i++) {
+    bool found = sz[i] >= 0;
+    if (read_values_hps) read_values_hps[i] = found ? (const uint8_t *)(uintptr_t)hp[i] : nullptr;
+    if (read_values_sizes) read_values_sizes[i] = found ? sz[i] : 0;
+  }
+  return 0;
+}
+
 int gmpt_hash_onepass(gmpt_forest *f, void *targets, int n_targets) {
   if (!f) return fail("null forest");
   return f->f->hash(targets, n_targets);

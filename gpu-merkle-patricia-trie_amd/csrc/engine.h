@@ -57,6 +57,13 @@ class Forest {
   int puts_latching(const BatchDesc &b, void **targets, int *n_targets);
   // PhaseHC over the dirty list, then every trie's root hash is refreshed.
   int hash(void *targets, int n_targets);
+  // Mixed batch: rw_flags[i] = 0 reads key i, 1 writes it (host or device
+  // array, as the batch).  Reads see the tries as they were before the batch
+  // (all reads are ordered before all writes); their results come back in
+  // batch order of the reads (*read_num of them).  Writes then run as one
+  // PhaseNU (mode 0) or LockNU (mode 1) insert.
+  int puts_with_read(const BatchDesc &b, const uint8_t *rw_flags, int mode, uint32_t *read_num,
+                     uint64_t *read_hps, int32_t *read_sizes, void **targets, int *n_targets);
   int root_hash(uint32_t trie, uint8_t *out, int *size);
   // Root hashes of tries [first, first + count) in one copy: 32-byte slots
   // (zero padded) and sizes.
@@ -111,7 +118,7 @@ class Forest {
   std::vector<DevBuf> vchunks_;
   size_t vchunk_used_ = 0;
   DevBuf st_keys_, st_kidx_, st_vidx_, st_hps_, st_tid_;
-  DevBuf ws_, ws2_;
+  DevBuf ws_, ws2_, rw_;
   DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_;
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation

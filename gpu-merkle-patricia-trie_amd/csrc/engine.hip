@@ -89,7 +89,7 @@ Forest::~Forest() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
-                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_};
+                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_, &rw_};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (DevBuf &b : vchunks_)
@@ -204,7 +204,7 @@ int Forest::sync_counters() {
     HIP_TRY(hipMemsetAsync(&d_cnt_->err, 0, 2 * sizeof(uint32_t), stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     const char *what = code == ERR_BAD_KEY ? "malformed key (needs hex nibbles < 16 and one terminator 16 at the end, "
-                                             "at most 254 nibbles) or trie id out of range"
+                                             "at most 254 nibbles), trie id out of range, or rw flag other than 0 / 1"
                        : code == ERR_POOL  ? "node pool exhausted"
                                            : "internal error";
     set_error(std::string(what) + " (code " + std::to_string(code) + ", at " + std::to_string(arg) + ")");
@@ -442,7 +442,7 @@ int Forest::hash(void *targets, int n_targets) {
     uint64_t *slot = (uint64_t *)hc_slot_.p;
     uint32_t *encl = (uint32_t *)hc_enc_.p;
     HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
-                        ws2_.p, stream_));
+                        encl, ws2_.p, stream_));
     HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     const HcMeta &M = *(const HcMeta *)h_meta_;
@@ -462,6 +462,7 @@ int Forest::hash(void *targets, int n_targets) {
       start += cnt;
     }
     HIP_TRY(hipEventRecord(ev1_, stream_));
+    HIP_TRY(hc_count(P, encl, (const HcMeta *)hc_meta_.p, n, stream_));
   }
   HIP_TRY(hc_roots(pools(), ntries_, stream_));
   HIP_TRY(hipMemsetAsync(&d_cnt_->n_dirty, 0, sizeof(uint32_t), stream_));
@@ -616,6 +617,69 @@ int Forest::gets(const BatchDesc &keys, uint64_t *hps, int32_t *sizes) {
   if (hps) HIP_TRY(hipMemcpyAsync(hps, d_hp, 8ull * n, hipMemcpyDeviceToHost, stream_));
   if (sizes) HIP_TRY(hipMemcpyAsync(sizes, d_size, 4ull * n, hipMemcpyDeviceToHost, stream_));
   return sync_counters();
+}
+
+int Forest::puts_with_read(const BatchDesc &b, const uint8_t *rw_flags, int mode, uint32_t *read_num,
+                           uint64_t *read_hps, int32_t *read_sizes, void **targets, int *n_targets) {
+  int r;
+  if ((r = set_device())) return r;
+  *read_num = 0;
+  if (b.n == 0) return mode == 0 ? puts_2phase(b, targets, n_targets) : puts_latching(b, targets, n_targets);
+  if (!rw_flags) {
+    set_error("null rw_flags");
+    return -3;
+  }
+  const uint32_t n = b.n;
+  BatchDesc d;
+  uint64_t vbase;
+  if ((r = stage_batch(b, &d, &vbase, true))) return r;  // values land in the value arena (or stay adopted)
+  // split layout: flags copy, isr, pos, nread, read keys/tids, write keys/values/hps/tids, scan temp
+  const size_t need = n + 8ull * n + 16 + 12ull * n + 40ull * n + scan_temp_bytes(n) + 4096;
+  if ((r = ensure_ws(rw_, need))) return r;
+  Carver c(rw_.p);
+  uint8_t *dflags = c.take<uint8_t>(n);
+  uint32_t *isr = c.take<uint32_t>(n);
+  uint32_t *pos = c.take<uint32_t>(n);
+  uint32_t *nread = c.take<uint32_t>(4);
+  int32_t *rk = c.take<int32_t>(2 * (size_t)n);
+  int32_t *rtid = c.take<int32_t>(n);
+  int32_t *wk = c.take<int32_t>(2 * (size_t)n);
+  int64_t *wv = c.take<int64_t>(2 * (size_t)n);
+  uint64_t *whp = c.take<uint64_t>(n);
+  int32_t *wtid = c.take<int32_t>(n);
+  void *temp = c.take<uint8_t>(scan_temp_bytes(n));
+  if (c.off > rw_.cap) {
+    set_error("workspace carve overflow");
+    return -4;
+  }
+  HIP_TRY(hipMemcpyAsync(dflags, rw_flags, n, b.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                         stream_));
+  const int32_t *tid = ntries_ > 1 ? d.trie_ids : nullptr;
+  HIP_TRY(rw_split(dflags, n, d.keys_indexs, d.values_indexs, d.values_hps, tid, isr, pos, nread, rk, rtid, wk, wv,
+                   whp, wtid, &d_cnt_->err, temp, stream_));
+  HIP_TRY(hipMemcpyAsync(h_scal_, nread, 4, hipMemcpyDeviceToHost, stream_));
+  if ((r = sync_counters())) return r;
+  const uint32_t nr = h_scal_[0];
+  if (nr > n) {
+    set_error("internal error: read count");
+    return -4;
+  }
+  BatchDesc rb = d, wb = d;
+  rb.keys_indexs = rk;
+  rb.trie_ids = tid ? rtid : nullptr;
+  rb.n = nr;
+  rb.on_device = true;
+  wb.keys_indexs = wk;
+  wb.values_indexs = wv;
+  wb.values_hps = d.values_hps ? whp : nullptr;
+  wb.trie_ids = tid ? wtid : nullptr;
+  wb.n = n - nr;
+  wb.on_device = true;
+  wb.adopt_values = true;  // already in the value arena, or the caller's adopted buffer
+  wb.values_bytes = (const uint8_t *)(uintptr_t)vbase;
+  if (nr > 0 && (r = gets(rb, read_hps, read_sizes))) return r;
+  *read_num = nr;
+  return mode == 0 ? puts_2phase(wb, targets, n_targets) : puts_latching(wb, targets, n_targets);
 }
 
 int Forest::get_values(const BatchDesc &keys, uint8_t *out, int64_t out_cap, int64_t *offsets, int32_t *sizes) {

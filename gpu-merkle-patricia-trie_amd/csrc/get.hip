@@ -236,4 +236,51 @@ done:
   return rc;
 }
 
+namespace {
+
+__global__ __launch_bounds__(256) void k_rw_mark(const uint8_t *flags, uint32_t n, uint32_t *isr, uint32_t *err) {
+  uint32_t i = grid_tid();
+  if (i >= n) return;
+  uint8_t f = flags[i];
+  if (f > 1) atomicCAS(err, 0u, (uint32_t)ERR_BAD_KEY);
+  isr[i] = f == 0 ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_rw_scatter(const uint8_t *flags, uint32_t n, const int32_t *kidx,
+                                                    const int64_t *vidx, const uint64_t *hps, const int32_t *tid,
+                                                    const uint32_t *pos, int32_t *rk, int32_t *rtid, int32_t *wk,
+                                                    int64_t *wv, uint64_t *whp, int32_t *wtid) {
+  uint32_t i = grid_tid();
+  if (i >= n) return;
+  uint32_t r = pos[i];
+  if (flags[i] == 0) {
+    rk[2 * r] = kidx[2 * i];
+    rk[2 * r + 1] = kidx[2 * i + 1];
+    if (tid) rtid[r] = tid[i];
+  } else {
+    uint32_t w = i - r;
+    wk[2 * w] = kidx[2 * i];
+    wk[2 * w + 1] = kidx[2 * i + 1];
+    wv[2 * w] = vidx[2 * i];
+    wv[2 * w + 1] = vidx[2 * i + 1];
+    if (hps) whp[w] = hps[i];
+    if (tid) wtid[w] = tid[i];
+  }
+}
+
+}  // namespace
+
+hipError_t rw_split(const uint8_t *flags, uint32_t n, const int32_t *kidx, const int64_t *vidx, const uint64_t *hps,
+                    const int32_t *tid, uint32_t *isr, uint32_t *pos, uint32_t *nread, int32_t *rk, int32_t *rtid,
+                    int32_t *wk, int64_t *wv, uint64_t *whp, int32_t *wtid, uint32_t *err, void *temp,
+                    hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_rw_mark<<<nblk(n), kB, 0, s>>>(flags, n, isr, err);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_exclusive_u32(isr, pos, n, nread, temp, s);
+  if (e != hipSuccess) return e;
+  k_rw_scatter<<<nblk(n), kB, 0, s>>>(flags, n, kidx, vidx, hps, tid, pos, rk, rtid, wk, wv, whp, wtid);
+  return hipGetLastError();
+}
+
 }  // namespace gmpt

@@ -317,9 +317,15 @@ __global__ __launch_bounds__(256) void k_lock_insert(NuArgs a, uint64_t key_cap,
     if (__ballot(!done) == 0) break;
   }
   if (active && !done) raise_error(a.P.cnt, ERR_INTERNAL, 20);
-  if (restarts) {
+  // one atomic pair per wave (per-lane atomics on one counter serialise)
+  uint32_t fails = restarts ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) {
+    restarts += (uint32_t)__shfl_xor((int)restarts, off);
+    fails += (uint32_t)__shfl_xor((int)fails, off);
+  }
+  if (__lane_id() == 0 && restarts) {
     atomicAdd(&a.P.cnt->n_restarts, restarts);
-    atomicAdd(&a.P.cnt->n_lock_fail, 1u);
+    atomicAdd(&a.P.cnt->n_lock_fail, fails);
   }
 }
 

@@ -27,6 +27,9 @@
 // a value node contributes its raw bytes, and a node whose own encoding is
 // shorter than 32 bytes keeps the raw encoding instead of a digest.
 #include "phasehc.h"
+
+#include <algorithm>
+
 #include "keccak.h"
 #include "prims.h"
 
@@ -128,17 +131,28 @@ __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
   if (i < n) p[i] = 0;
 }
 
-// Counting sort of the dirty list by bin.  Counts are aggregated per block in
-// LDS so each block issues one global atomic per bin it touches (a global
-// atomic per node serialises on the few hot bins).
-__global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *list, uint32_t n, uint32_t *hist) {
+// Counting sort of the dirty list by bin.  The first pass computes each
+// entry's bin and block estimate once (pointer chasing) and keeps them packed
+// in `packed` (bin | blocks << 12, blocks saturated at 2^20 - 1); counts are
+// aggregated per block in LDS so each block issues one global atomic per bin
+// it touches (a global atomic per node serialises on the few hot bins).
+constexpr uint32_t kPackedNbMax = (1u << 20) - 1;
+
+__global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restrict__ list, uint32_t n,
+                                                 uint32_t *__restrict__ hist, uint32_t *__restrict__ packed) {
   __shared__ uint32_t h[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
   for (uint32_t i = blockIdx.x * kHcTile + threadIdx.x; i < min(n, (blockIdx.x + 1) * kHcTile); i += blockDim.x) {
     ref_t r = list[i];
-    uint32_t nb;
-    if (r != REF_NULL) atomicAdd(&h[bin_of(P, r, &nb)], 1u);  // builds leave null holes
+    uint32_t pk = 0xffffffffu;  // builds leave null holes
+    if (r != REF_NULL) {
+      uint32_t nb;
+      uint32_t bin = bin_of(P, r, &nb);
+      atomicAdd(&h[bin], 1u);
+      pk = bin | (min(nb, kPackedNbMax) << 12);
+    }
+    packed[i] = pk;
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x)
@@ -147,23 +161,21 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *list, uin
 
 // Scatters each node to its bin and records its scratch slot size
 // (136 bytes per estimated block) at the same sorted position.
-__global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *list, uint32_t n, uint32_t *cursor,
-                                                    ref_t *out, uint64_t *slot) {
+__global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *__restrict__ list,
+                                                    const uint32_t *__restrict__ packed, uint32_t n,
+                                                    uint32_t *__restrict__ cursor, ref_t *__restrict__ out,
+                                                    uint64_t *__restrict__ slot) {
   __shared__ uint32_t cnt[kHcBins];
   __shared__ uint32_t base[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
   constexpr int kPer = kHcTile / 256;
-  uint32_t bin[kPer], rank[kPer], nb[kPer];
-  ref_t r[kPer];
+  uint32_t pk[kPer], rank[kPer];
 #pragma unroll
   for (int q = 0; q < kPer; q++) {
     uint32_t i = blockIdx.x * kHcTile + q * 256 + threadIdx.x;
-    r[q] = i < n ? list[i] : REF_NULL;
-    if (r[q] != REF_NULL) {
-      bin[q] = bin_of(P, r[q], &nb[q]);
-      rank[q] = atomicAdd(&cnt[bin[q]], 1u);
-    }
+    pk[q] = i < n ? packed[i] : 0xffffffffu;
+    if (pk[q] != 0xffffffffu) rank[q] = atomicAdd(&cnt[pk[q] & 0xfff], 1u);
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x)
@@ -171,10 +183,14 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *list, 
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kPer; q++)
-    if (r[q] != REF_NULL) {
-      uint32_t pos = base[bin[q]] + rank[q];
-      out[pos] = r[q];
-      slot[pos] = 136ull * nb[q];
+    if (pk[q] != 0xffffffffu) {
+      uint32_t i = blockIdx.x * kHcTile + q * 256 + threadIdx.x;
+      uint32_t pos = base[pk[q] & 0xfff] + rank[q];
+      ref_t r = list[i];
+      uint32_t nb = pk[q] >> 12;
+      if (nb == kPackedNbMax) bin_of(P, r, &nb);  // saturated: recompute exactly
+      out[pos] = r;
+      slot[pos] = 136ull * nb;
     }
 }
 
@@ -507,14 +523,31 @@ __device__ __forceinline__ bool encode_leaf_fast(const Pools &P, ref_t r, uint64
   return true;
 }
 
-__device__ __forceinline__ void count_work(const Pools &P, uint32_t perms, uint32_t bytes) {
-  for (int off = 32; off > 0; off >>= 1) {
-    perms += (uint32_t)__shfl_xor((int)perms, off);
-    bytes += (uint32_t)__shfl_xor((int)bytes, off);
+// Work counters of a hash pass: one reduction over the encoding lengths
+// after the last level (per-wave atomics on one counter serialise when waves
+// are many).
+__global__ __launch_bounds__(256) void k_hc_count(Pools P, const uint32_t *__restrict__ enc_len,
+                                                  const uint32_t *__restrict__ nvalid) {
+  __shared__ unsigned long long sp[4], sb[4];
+  const uint32_t n = *nvalid;
+  unsigned long long perms = 0, bytes = 0;
+  for (uint32_t i = grid_tid(); i < n; i += gridDim.x * blockDim.x) {
+    uint32_t e = enc_len[i];
+    bytes += e;
+    perms += e >= 32 ? keccak_blocks(e) : 0u;
   }
-  if (__lane_id() == 0 && (perms | bytes)) {
-    atomicAdd(&P.cnt->n_perms, (unsigned long long)perms);
-    atomicAdd(&P.cnt->enc_bytes, (unsigned long long)bytes);
+  for (int off = 32; off > 0; off >>= 1) {
+    perms += (unsigned long long)__shfl_xor((long long)perms, off);
+    bytes += (unsigned long long)__shfl_xor((long long)bytes, off);
+  }
+  if (__lane_id() == 0) {
+    sp[threadIdx.x >> 6] = perms;
+    sb[threadIdx.x >> 6] = bytes;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&P.cnt->n_perms, sp[0] + sp[1] + sp[2] + sp[3]);
+    atomicAdd(&P.cnt->enc_bytes, sb[0] + sb[1] + sb[2] + sb[3]);
   }
 }
 
@@ -524,20 +557,12 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
                                                         uint64_t *__restrict__ scratch,
                                                         uint32_t *__restrict__ enc_len) {
   uint32_t i = grid_tid();
-  uint32_t perms = 0, bytes = 0;
-  if (i < n) {
-    ref_t r = list[i];
-    uint64_t *out = scratch + slot[i] / 8;
-    uint32_t enc, nb;
-    if (encode_leaf_fast(P, r, out, &enc, &nb)) {
-      perms = nb;
-    } else {
-      perms = encode_generic(P, r, out, &enc);
-    }
-    enc_len[i] = enc;
-    bytes = enc;
-  }
-  count_work(P, perms, bytes);
+  if (i >= n) return;
+  ref_t r = list[i];
+  uint64_t *out = scratch + slot[i] / 8;
+  uint32_t enc, nb;
+  if (!encode_leaf_fast(P, r, out, &enc, &nb)) encode_generic(P, r, out, &enc);
+  enc_len[i] = enc;
 }
 
 constexpr uint32_t kInnerGroup = 16;              // lanes per node (one per nibble slot)
@@ -616,21 +641,14 @@ __global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *_
     }
   }
   __syncthreads();
-  uint32_t perms = 0, bytes = 0;
   if (fast) {
     uint64_t *out = scratch + slot[i] / 8;
     for (uint32_t w = s; w < 17 * nb; w += kInnerGroup) out[w] = lds[g][w];
-    if (s == 0) {
-      enc_len[i] = enc;
-      perms = nb;
-      bytes = enc;
-    }
+    if (s == 0) enc_len[i] = enc;
   } else if (active && s == 0) {
-    perms = encode_generic(P, r, scratch + slot[i] / 8, &enc);
+    encode_generic(P, r, scratch + slot[i] / 8, &enc);
     enc_len[i] = enc;
-    bytes = enc;
   }
-  count_work(P, perms, bytes);
 }
 
 // ----------------------------------------------------------------- permute --
@@ -668,30 +686,41 @@ __global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list
 // lane t holds word A[t % 5, t / 5] as (l, h) halves.  Lanes 25..31 compute
 // values nobody reads.  Shuffles stay inside the 32-lane group.
 struct SpreadLane {
-  uint32_t t, x, y, rho, pi_src, n1, n2;
+  uint32_t t, rho, xm, xp, s0, s1, s2;
+  // pi moves A[x', y'] to B[y', 2x' + 3y']: lane (x, y) receives B[x, y] from
+  // lane x' + 5y' with x' = 3 (y - 3x) mod 5, y' = x
+  static __device__ __forceinline__ uint32_t pi_src(uint32_t x, uint32_t y) {
+    return (3 * (y + 5 - (3 * x) % 5)) % 5 + 5 * x;
+  }
   __device__ __forceinline__ explicit SpreadLane(uint32_t gl) {
     t = gl < 25 ? gl : 0;
-    x = t % 5;
-    y = t / 5;
+    uint32_t x = t % 5, y = t / 5;
     rho = (uint32_t)kKeccak.rho[t];
-    // pi moves A[x', y'] to B[y', 2x' + 3y']; this lane (x, y) receives from
-    // x' = 3 (y - 3x) mod 5, y' = x
-    pi_src = (3 * (y + 5 - (3 * x) % 5)) % 5 + 5 * x;
-    n1 = (x + 1) % 5 + 5 * y;
-    n2 = (x + 2) % 5 + 5 * y;
+    xm = (x + 4) % 5;
+    xp = (x + 1) % 5;
+    s0 = pi_src(x, y);
+    s1 = pi_src((x + 1) % 5, y);
+    s2 = pi_src((x + 2) % 5, y);
   }
 };
 
 __device__ __forceinline__ uint32_t sh32(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src, 32); }
 
+// Two exchange stages per round: theta gathers columns x - 1 and x + 1
+// directly (no separate parity exchange); pi and chi's two row neighbours are
+// gathered together from the rho-rotated words.
 __device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const SpreadLane &c) {
 #pragma unroll 1
   for (int ir = 0; ir < 24; ir++) {
-    // theta: column parity C[x], then A ^= C[x - 1] ^ rotl(C[x + 1], 1)
-    uint32_t cl = xor3(xor3(sh32(l, c.x), sh32(l, c.x + 5), sh32(l, c.x + 10)), sh32(l, c.x + 15), sh32(l, c.x + 20));
-    uint32_t ch = xor3(xor3(sh32(h, c.x), sh32(h, c.x + 5), sh32(h, c.x + 10)), sh32(h, c.x + 15), sh32(h, c.x + 20));
-    uint32_t xm = (c.x + 4) % 5, xp = (c.x + 1) % 5;
-    uint32_t cml = sh32(cl, xm), cmh = sh32(ch, xm), cpl = sh32(cl, xp), cph = sh32(ch, xp);
+    // theta: A ^= C[x - 1] ^ rotl(C[x + 1], 1)
+    uint32_t cml = xor3(xor3(sh32(l, c.xm), sh32(l, c.xm + 5), sh32(l, c.xm + 10)), sh32(l, c.xm + 15),
+                        sh32(l, c.xm + 20));
+    uint32_t cmh = xor3(xor3(sh32(h, c.xm), sh32(h, c.xm + 5), sh32(h, c.xm + 10)), sh32(h, c.xm + 15),
+                        sh32(h, c.xm + 20));
+    uint32_t cpl = xor3(xor3(sh32(l, c.xp), sh32(l, c.xp + 5), sh32(l, c.xp + 10)), sh32(l, c.xp + 15),
+                        sh32(l, c.xp + 20));
+    uint32_t cph = xor3(xor3(sh32(h, c.xp), sh32(h, c.xp + 5), sh32(h, c.xp + 10)), sh32(h, c.xp + 15),
+                        sh32(h, c.xp + 20));
     l = xor3(l, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
     h = xor3(h, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
     // rho: rotate this lane's word by its own offset
@@ -699,11 +728,9 @@ __device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const Sp
     uint32_t a = c.rho >= 32 ? h : l, b = c.rho >= 32 ? l : h;
     uint32_t nl = rr ? __builtin_amdgcn_alignbit(a, b, 32 - rr) : a;
     uint32_t nh = rr ? __builtin_amdgcn_alignbit(b, a, 32 - rr) : b;
-    // pi
-    uint32_t bl = sh32(nl, c.pi_src), bh = sh32(nh, c.pi_src);
-    // chi
-    l = chi3(bl, sh32(bl, c.n1), sh32(bl, c.n2));
-    h = chi3(bh, sh32(bh, c.n1), sh32(bh, c.n2));
+    // pi + chi: B[x, y] ^ (~B[x + 1, y] & B[x + 2, y])
+    l = chi3(sh32(nl, c.s0), sh32(nl, c.s1), sh32(nl, c.s2));
+    h = chi3(sh32(nh, c.s0), sh32(nh, c.s1), sh32(nh, c.s2));
     // iota
     if (c.t == 0) {
       uint64_t rc = kKeccak.rc[ir];
@@ -863,7 +890,7 @@ __global__ __launch_bounds__(256) void k_enc_write(Pools P, const ref_t *refs, u
 size_t hc_temp_bytes(uint32_t n) { return scan_temp_bytes(n > kHcBins ? n : kHcBins); }
 
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
-                       uint64_t *slot, void *temp, hipStream_t s) {
+                       uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s) {
   k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins);
   GMPT_LAUNCH_CHECK();
   if (n == 0) {
@@ -871,11 +898,11 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
     return hipGetLastError();
   }
   unsigned tiles = (unsigned)((n + kHcTile - 1) / kHcTile);
-  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist);
+  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist, packed);
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(meta->hist, cursor, kHcBins, &meta->nvalid, temp, s);
   if (e != hipSuccess) return e;
-  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, n, cursor, out, slot);
+  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, packed, n, cursor, out, slot);
   GMPT_LAUNCH_CHECK();
   k_hc_slot_tail<<<nblk(n), kB, 0, s>>>(&meta->nvalid, n, slot);
   GMPT_LAUNCH_CHECK();
@@ -894,6 +921,12 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     k_hc_perm_spread<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
   else
     k_hc_perm_lane<<<nblk(n), kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
+  return hipGetLastError();
+}
+
+hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n), 512), kB, 0, s>>>(P, enc_len, &meta->nvalid);
   return hipGetLastError();
 }
 

@@ -72,10 +72,61 @@ __device__ __forceinline__ uint32_t pack8(uint64_t h) {
   return (uint32_t)((t & 0xFFull) | ((t >> 8) & 0xFF00ull) | ((t >> 16) & 0xFF0000ull) | ((t >> 24) & 0xFF000000ull));
 }
 
+// Hex-key staging: the block's keys are read from LDS when their bytes span
+// at most kKeyStage bytes (keys stored in batch order), so the block's HBM
+// reads are coalesced 16-byte loads instead of one strided stream per lane.
+constexpr uint32_t kKeyStage = 24576;
+
+struct KeyStager {
+  uintptr_t base = 0;  // device address of lds[0] (16-byte aligned), 0 when not staged
+  const uint8_t *lds = nullptr;
+  // pointer to hex byte `off` of the batch's key buffer
+  __device__ __forceinline__ const uint8_t *at(const NuArgs &a, int64_t off) const {
+    return base ? lds + ((uintptr_t)(a.kh + off) - base) : a.kh + off;
+  }
+};
+
+// Every thread of the block must call this (it synchronises the block).
+__device__ __forceinline__ KeyStager stage_keys(const NuArgs &a, uint8_t *lds) {
+  __shared__ uint64_t red_lo[4], red_hi[4];
+  uint32_t i = grid_tid();
+  uint64_t lo = ~0ull, hi = 0;
+  if (i < a.n && key_range_ok(a, i)) {
+    lo = (uint64_t)a.kidx[2 * i];
+    hi = (uint64_t)a.kidx[2 * i + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, (uint64_t)__shfl_xor((unsigned long long)lo, off));
+    hi = max(hi, (uint64_t)__shfl_xor((unsigned long long)hi, off));
+  }
+  if (__lane_id() == 0) {
+    red_lo[threadIdx.x >> 6] = lo;
+    red_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  lo = min(min(red_lo[0], red_lo[1]), min(red_lo[2], red_lo[3]));
+  hi = max(max(red_hi[0], red_hi[1]), max(red_hi[2], red_hi[3]));
+  KeyStager st;
+  st.lds = lds;
+  if (lo == ~0ull) return st;
+  // aligned 16-byte words covering [kh + lo, kh + hi]: they stay inside the
+  // memory pages holding those bytes
+  uintptr_t b = ((uintptr_t)(a.kh + lo)) & ~(uintptr_t)15;
+  uintptr_t e = ((uintptr_t)(a.kh + hi) + 16) & ~(uintptr_t)15;
+  if (e - b > kKeyStage) return st;
+  uint32_t nw = (uint32_t)((e - b) / 16);
+  for (uint32_t k = threadIdx.x; k < nw; k += blockDim.x) ((uint4 *)lds)[k] = ((const uint4 *)b)[k];
+  __syncthreads();
+  st.base = b;
+  return st;
+}
+
 // Validates every key, and reduces per block the common prefix with key 0
 // and the longest key (no global atomics: one slot per block).
 __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, uint32_t *blk_len) {
   __shared__ uint32_t sw[4], sl[4];
+  __shared__ __align__(16) uint8_t lds[kKeyStage + 16];
+  const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
   uint32_t lcp0 = UINT_MAX, len = 0;
   if (i < a.n) {
@@ -84,7 +135,7 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
     if (ok) {
       int32_t s = a.kidx[2 * i], e = a.kidx[2 * i + 1];
       L = e - s + 1;
-      const uint8_t *k = a.kh + s;
+      const uint8_t *k = stg.at(a, s);
       uint32_t nn = (uint32_t)L - 1;  // nibbles before the terminator
       ok = k[nn] == kNibbleT;
       for (uint32_t o = 0; ok && o < nn; o += 8) {
@@ -165,10 +216,12 @@ __global__ __launch_bounds__(256) void k_keys_reduce(NuArgs a, const uint32_t *b
 // Packs nibbles two per byte into the key arena and builds the sort window:
 // 16 nibbles from the batch's common prefix W, terminator and padding as 0.
 __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
+  __shared__ __align__(16) uint8_t lds[kKeyStage + 16];
+  const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
   if (i >= a.n) return;
   uint32_t L = a.klen[i];
-  const uint8_t *k = a.kh + (key_range_ok(a, i) ? a.kidx[2 * i] : 0);
+  const uint8_t *k = key_range_ok(a, i) ? stg.at(a, a.kidx[2 * i]) : a.kh;
   uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + a.kpoff[i]);
   uint32_t nb = (L / 2 + 8) & ~7u;
   uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
@@ -636,15 +689,24 @@ __global__ __launch_bounds__(256) void k_emit_write(NuArgs a) {
 
 // ------------------------------------------------------------- 4. build --
 
-__global__ __launch_bounds__(256) void k_st_level(NuArgs a, int k) {
+// Sparse-table levels k0 + 1 .. k0 + cnt (cnt <= 4) from level k0: entry q
+// of level l is min(SP[q .. q + 2^l)), stored where q + 2^l <= nsp.
+__global__ __launch_bounds__(256) void k_st_levels(NuArgs a, int k0, int cnt) {
   uint32_t q = grid_tid();
   uint32_t ne = a.scal[SC_NE];
   uint32_t nsp = ne > 0 ? ne - 1 : 0;
-  uint32_t half = 1u << (k - 1);
-  if (q + 2 * half > nsp) return;
-  const uint8_t *lo = a.st + (size_t)(k - 1) * a.st_stride;
-  uint8_t *hi = a.st + (size_t)k * a.st_stride;
-  hi[q] = min(lo[q], lo[q + half]);
+  const size_t h = (size_t)1 << k0;
+  if (q + 2 * h > nsp) return;
+  const uint8_t *lo = a.st + (size_t)k0 * a.st_stride;
+  uint32_t mn = lo[q];  // min of lo[q + m h] for m < have
+  uint32_t have = 1;
+  for (int l = 1; l <= cnt; l++) {
+    size_t span = h << l;  // level k0 + l covers span entries
+    if (q + span > nsp) return;
+    for (uint32_t m = have; m < 2 * have; m++) mn = min(mn, (uint32_t)lo[q + m * h]);
+    a.st[(size_t)(k0 + l) * a.st_stride + q] = (uint8_t)mn;
+    have *= 2;
+  }
 }
 
 struct ST {
@@ -1090,8 +1152,9 @@ hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s) {
 
 hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
   uint32_t ne_max = 2 * a.n + 2;
-  for (int k = 1; k < a.st_levels; k++) {
-    k_st_level<<<nblk(ne_max), kB, 0, s>>>(a, k);
+  for (int k = 1; k < a.st_levels; k += 4) {
+    int cnt = a.st_levels - k < 4 ? a.st_levels - k : 4;
+    k_st_levels<<<nblk(ne_max), kB, 0, s>>>(a, k - 1, cnt);
     GMPT_LAUNCH_CHECK();
   }
   k_updates<<<nblk(a.n), kB, 0, s>>>(a, ne_max);

@@ -41,7 +41,7 @@ class Stats(ctypes.Structure):
 
 # every symbol the headers in include/ declare
 GMPT_SYMBOLS = ["gmpt_forest_create", "gmpt_forest_destroy", "gmpt_last_error", "gmpt_puts_2phase",
-                "gmpt_puts_latching", "gmpt_hash_onepass", "gmpt_get_root_hash", "gmpt_get_root_hashes", "gmpt_gets", "gmpt_get_values",
+                "gmpt_puts_latching", "gmpt_puts_with_read", "gmpt_hash_onepass", "gmpt_get_root_hash", "gmpt_get_root_hashes", "gmpt_gets", "gmpt_get_values",
                 "gmpt_export_nodes", "gmpt_free_host", "gmpt_reset_trie", "gmpt_forest_clear", "gmpt_synchronize",
                 "gmpt_set_option", "gmpt_get_option", "gmpt_autotune", "gmpt_get_stats",
                 "gmpt_device_count", "gmpt_device_malloc", "gmpt_device_free", "gmpt_memcpy_htod",
@@ -82,6 +82,8 @@ def lib():
         "gmpt_last_error": ([], ctypes.c_char_p),
         "gmpt_puts_2phase": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(i32)], i32),
         "gmpt_puts_latching": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(vp), ctypes.POINTER(i32)], i32),
+        "gmpt_puts_with_read": ([vp, ctypes.POINTER(_Batch), vp, i32, ctypes.POINTER(i32), vp, vp,
+                                 ctypes.POINTER(vp), ctypes.POINTER(i32)], i32),
         "gmpt_hash_onepass": ([vp, vp, i32], i32),
         "gmpt_get_root_hash": ([vp, u32, vp, ctypes.POINTER(i32)], i32),
         "gmpt_get_root_hashes": ([vp, u32, u32, vp, vp], i32),
@@ -263,6 +265,24 @@ class MPT:
     def puts_latching_with_valuehp_v2(self, batch, values_hps=None):
         """LockNU: returns (hash targets, count) for hash_onepass_v2."""
         return self._puts(lib().gmpt_puts_latching, batch, values_hps)
+
+    def puts_with_read(self, batch, rw_flags, mode=0, values_hps=None):
+        """Mixed batch (gpu_mpt.cuh:74-78, 105-110): rw_flags[i] 0 = read, 1 =
+        write.  Returns (targets, count, reads) with reads = [(host pointer,
+        size)] in batch order of the reads, seen before the batch's writes."""
+        s = batch.struct if isinstance(batch, DeviceBatch) else host_batch(batch, values_hps)
+        flags = rw_flags if isinstance(rw_flags, DeviceBuffer) else np.ascontiguousarray(rw_flags, np.uint8)
+        fptr = flags.ptr if isinstance(flags, DeviceBuffer) else _p(flags)
+        n = batch.n
+        rn = ctypes.c_int()
+        hps = np.zeros(max(n, 1), np.uint64)
+        sizes = np.zeros(max(n, 1), np.int32)
+        t = ctypes.c_void_p()
+        nt = ctypes.c_int()
+        _check(lib().gmpt_puts_with_read(self._f, ctypes.byref(s), fptr, mode, ctypes.byref(rn), _p(hps), _p(sizes),
+                                         ctypes.byref(t), ctypes.byref(nt)))
+        reads = [(int(hps[i]), int(sizes[i])) for i in range(rn.value)]
+        return t, nt.value, reads
 
     # --- hashing (gpu_mpt.cuh:131) --------------------------------------------
     def hash_onepass_v2(self, targets, n):

@@ -76,6 +76,19 @@ int gmpt_puts_2phase(gmpt_forest *f, const gmpt_batch *b, void **targets, int *n
 /* LockNU batch insert.
  * Replaces: MPT::puts_latching_with_valuehp_v2 (gpu_mpt.cuh:69-71, 554-615). */
 int gmpt_puts_latching(gmpt_forest *f, const gmpt_batch *b, void **targets, int *n_targets);
+/* Mixed read/write batch: rw_flags[i] = 0 reads key i, 1 writes it (a host
+ * array for host batches, a device array for device batches).  Reads see the
+ * tries as they were before the batch (reads are ordered before the batch's
+ * writes); *read_num results come back in batch order: the host pointer given
+ * at insert and the value size (NULL and 0 for an absent key), into arrays of
+ * capacity n.  The writes then run as one PhaseNU (mode 0) or LockNU (mode 1)
+ * insert; *targets / *n_targets as for the plain inserts.
+ * Replaces: MPT::puts_2phase_with_valuehp_with_read (gpu_mpt.cuh:105-110,
+ * 1609-1713) and MPT::puts_latching_with_valuehp_v2_with_read
+ * (gpu_mpt.cuh:74-78, 684-765); their experiment-recorder arguments have no
+ * counterpart.  Read results there come back in arbitrary order. */
+int gmpt_puts_with_read(gmpt_forest *f, const gmpt_batch *b, const uint8_t *rw_flags, int mode, int *read_num,
+                        const uint8_t **read_values_hps, int *read_values_sizes, void **targets, int *n_targets);
 /* PhaseHC: re-hash every dirty node bottom-up and refresh each trie's root.
  * Replaces: MPT::hash_onepass_v2 (gpu_mpt.cuh:131, 1162-1191). */
 int gmpt_hash_onepass(gmpt_forest *f, void *targets, int n_targets);

@@ -71,6 +71,29 @@ class MPT {
     return {static_cast<Node **>(t), nt};
   }
 
+  // gpu_mpt.cuh:105-110, 1609-1713 (mixed reads and writes, PhaseNU writes);
+  // reads see the trie before the batch's writes and come back in batch order
+  std::tuple<Node **, int> puts_2phase_with_valuehp_with_read(const uint8_t *keys_hexs, int *keys_indexs,
+                                                              int &read_num, const uint8_t *rw_flags,
+                                                              const uint8_t *values_bytes, int64_t *values_indexs,
+                                                              const uint8_t **values_hps, int n,
+                                                              const uint8_t **read_values_hps,
+                                                              int *read_values_sizes) {
+    return with_read(0, keys_hexs, keys_indexs, read_num, rw_flags, values_bytes, values_indexs, values_hps, n,
+                     read_values_hps, read_values_sizes);
+  }
+
+  // gpu_mpt.cuh:74-78, 684-765 (mixed reads and writes, LockNU writes)
+  std::tuple<Node **, int> puts_latching_with_valuehp_v2_with_read(const uint8_t *keys_hexs, int *keys_indexs,
+                                                                   int &read_num, const uint8_t *rw_flags,
+                                                                   const uint8_t *values_bytes,
+                                                                   int64_t *values_indexs, const uint8_t **values_hps,
+                                                                   int n, const uint8_t **read_values_hps,
+                                                                   int *read_values_sizes) {
+    return with_read(1, keys_hexs, keys_indexs, read_num, rw_flags, values_bytes, values_indexs, values_hps, n,
+                     read_values_hps, read_values_sizes);
+  }
+
   // gpu_mpt.cuh:131, 1162-1191 (PhaseHC)
   void hash_onepass_v2(Node **d_hash_nodes, int n) { check(gmpt_hash_onepass(f_, d_hash_nodes, n)); }
 
@@ -121,6 +144,15 @@ class MPT {
   gmpt_forest *handle() const { return f_; }
 
  private:
+  std::tuple<Node **, int> with_read(int mode, const uint8_t *kh, int *ki, int &read_num, const uint8_t *flags,
+                                     const uint8_t *vb, int64_t *vi, const uint8_t **hps, int n,
+                                     const uint8_t **rhps, int *rsizes) {
+    gmpt_batch b = batch(kh, ki, vb, vi, hps, n);
+    void *t = nullptr;
+    int nt = 0;
+    check(gmpt_puts_with_read(f_, &b, flags, mode, &read_num, rhps, rsizes, &t, &nt));
+    return {static_cast<Node **>(t), nt};
+  }
   static void check(int rc) {
     if (rc != 0) throw gmpt_error(gmpt_last_error());
   }

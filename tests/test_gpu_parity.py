@@ -271,3 +271,52 @@ def test_autotune_keeps_results(gmpt, workloads, oracle):
     targets, n = t.puts_2phase_with_valuehp(b)
     t.hash_onepass_v2(targets, n)
     assert t.get_root_hash() == oracle.root_of(b)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_read_write(gmpt, workloads, oracle, mode):
+    """gets + puts in one batch (the *_with_read entry points): reads see the
+    trie before the batch's writes, in batch order; writes land as usual."""
+    rng = np.random.default_rng(123)
+    pre = _rand_batch(workloads, 70, 3000, 30, 100)
+    pre_hps = np.array([pre.values_bytes.ctypes.data + int(pre.values_indexs[2 * i]) for i in range(pre.n)],
+                       np.uint64)
+    t = gmpt.MPT(device=0)
+    _put(t, pre, mode, pre_hps)
+    # ops: reads of existing keys, reads of absent keys, updates, inserts,
+    # and keys both read and written in the batch
+    fresh = _rand_batch(workloads, 71, 800, 30, 100)
+    picks = rng.choice(pre.n, 1200, replace=False)
+    ops_keys, ops_vals, flags, want_reads = [], [], [], []
+    for j, i in enumerate(picks):
+        ops_keys.append(bytes(pre.key(int(i))))
+        if j % 3 == 0:  # update
+            ops_vals.append(bytes(rng.integers(0, 256, 50, dtype=np.uint8)))
+            flags.append(1)
+        else:
+            ops_vals.append(b"")
+            flags.append(0)
+            want_reads.append((int(pre_hps[i]), pre.value(int(i)).__len__()))
+    for i in range(fresh.n):
+        if i % 2 == 0:  # insert
+            ops_keys.append(bytes(fresh.key(i)))
+            ops_vals.append(fresh.value(i))
+            flags.append(1)
+        else:  # read of an absent key
+            ops_keys.append(bytes(fresh.key(i)))
+            ops_vals.append(b"")
+            flags.append(0)
+            want_reads.append((0, 0))
+    ops = workloads.batch_from_hex(ops_keys, ops_vals)
+    targets, n, reads = t.puts_with_read(ops, np.array(flags, np.uint8), 0 if mode == "2phase" else 1)
+    t.hash_onepass_v2(targets, n)
+    assert reads == want_reads
+    o = oracle.OracleMPT()
+    o.puts(pre)
+    w = [i for i in range(ops.n) if flags[i] == 1]
+    o.puts(ops.subset(w))
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    assert t.get_values(ops.subset(w)) == [ops.value(i) for i in w]
+    with pytest.raises(gmpt.GmptError):
+        t.puts_with_read(ops, np.full(ops.n, 2, np.uint8))
