@@ -74,6 +74,7 @@ def bench_phase(pkg, args):
     gen_s = time.time() - t0
     db = pkg.DeviceBatch(b, adopt_values=True)
     t = pkg.MPT(device=0)
+    spread_max, tune_ms = t.autotune(db)  # picks the Keccak layout threshold on this batch
     roots = []
 
     def step():
@@ -125,6 +126,7 @@ def bench_phase(pkg, args):
                           "frac": t_roof * 1e3 / ms_per_step, "bytes_in": b_in, "bytes_nodes": b_nodes,
                           "device_ms_per_step": dev_ms / args.steps},
         "mputs_per_s": value / 1e6, "mhashes_per_s": st["hashed_nodes"] / (ms_per_step * 1e-3) / 1e6,
+        "autotune": {"hc_spread_max": spread_max, "hash_ms": tune_ms},
         "root": roots[-1].hex(),
     }
     if len(set(roots)) != 1:

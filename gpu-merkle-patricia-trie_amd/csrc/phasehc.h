@@ -9,7 +9,7 @@ constexpr uint32_t kHcMaxDepth = 255;  // levels: 0 = leaves, 1 + (255 - depth) 
 constexpr uint32_t kHcLevels = kHcMaxDepth + 2;
 constexpr uint32_t kHcBins = kHcLevels * kHcClasses;
 // Levels with at most this many nodes spread each Keccak state over 25 lanes.
-constexpr uint32_t kHcSpreadMaxDefault = 8192;
+constexpr uint32_t kHcSpreadMaxDefault = 1024;
 
 // Classification results, read back by the host in one copy.
 struct HcMeta {

@@ -425,55 +425,46 @@ __device__ __forceinline__ uint64_t low_bytes(uint64_t v, uint32_t n) { return n
 constexpr uint32_t kLeafFastValue = 120;  // value bytes held in registers
 constexpr uint32_t kLeafFastNibbles = 64; // key nibbles held in registers
 
-// Leaf short node [compact key, value] with a value of <= 120 bytes and a key
-// remainder of <= 64 nibbles.  Every load is issued before the first store:
-// on this GPU loads and stores share one completion counter, so a load that
-// follows a store waits for the store, and the generic encoder's interleaved
-// accesses serialise.  Returns false (nothing written) when the node does not
-// qualify or its encoding is shorter than 32 bytes.
-__device__ __forceinline__ bool encode_leaf_fast(const Pools &P, ref_t r, uint64_t *__restrict__ out,
-                                                 uint32_t *enc_out, uint32_t *nb_out) {
-  const Short &S = P.shrt[ref_idx(r)];
-  const uint64_t key_off = S.key_off;
-  const ref_t ch = S.child;
-  const uint32_t ks = S.key_start, cnt = (uint32_t)S.key_len - S.term, term = S.term;
-  if (ref_type(ch) != T_VALUE || cnt > kLeafFastNibbles) return false;
-  const Value &V = P.value[ref_idx(ch)];
-  const uint64_t va = V.addr;
-  const uint32_t vsz = V.size;
-  // packed key bytes b[0..] from byte ks / 2: up to 33 bytes, 6 aligned words
-  const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
-  const uint32_t koff = (uint32_t)(ka & 7);
-  const uint64_t *kw_p = (const uint64_t *)(ka - koff);
-  const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
-  const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;  // one byte more for the straddled pairs
-  uint64_t kw[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
-  if (vsz > kLeafFastValue) return false;
-  const uint32_t voff = (uint32_t)(va & 7);
-  const uint64_t *vw_p = (const uint64_t *)(va - voff);
-  const uint32_t nvw = (voff + vsz + 7) / 8;
-  uint64_t vw[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) vw[q] = (uint32_t)q < nvw ? vw_p[q] : 0ull;
+// Work counters of a hash pass: one reduction over the encoding lengths
+// after the last level (per-wave atomics on one counter serialise when waves
+// are many).
+__global__ __launch_bounds__(256) void k_hc_count(Pools P, const uint32_t *__restrict__ enc_len,
+                                                  const uint32_t *__restrict__ nvalid) {
+  __shared__ unsigned long long sp[4], sb[4];
+  const uint32_t n = *nvalid;
+  unsigned long long perms = 0, bytes = 0;
+  for (uint32_t i = grid_tid(); i < n; i += gridDim.x * blockDim.x) {
+    uint32_t e = enc_len[i];
+    bytes += e;
+    perms += e >= 32 ? keccak_blocks(e) : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    perms += (unsigned long long)__shfl_xor((long long)perms, off);
+    bytes += (unsigned long long)__shfl_xor((long long)bytes, off);
+  }
+  if (__lane_id() == 0) {
+    sp[threadIdx.x >> 6] = perms;
+    sb[threadIdx.x >> 6] = bytes;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&P.cnt->n_perms, sp[0] + sp[1] + sp[2] + sp[3]);
+    atomicAdd(&P.cnt->enc_bytes, sb[0] + sb[1] + sb[2] + sb[3]);
+  }
+}
 
-  // lengths
-  const uint32_t kc = cnt / 2 + 1;
-  const uint32_t key_item = kc == 1 ? 1u : 1u + kc;
-  const uint32_t v0 = (uint32_t)(funnel8(vw[0], vw[1], voff) & 0xff);
-  const bool v_raw = vsz == 1 && v0 <= 0x7f;
-  const uint32_t val_item = v_raw ? 1u : rlp_hdr_len(vsz) + vsz;
-  const uint32_t payload = key_item + val_item;
-  const uint32_t enc = rlp_hdr_len(payload) + payload;
-  if (enc < 32) return false;
-  const uint32_t nb = keccak_blocks(enc);
-
-  WordWriter w(out);
+// Writes the list header, the compact-key item and the value header of a leaf
+// [compact key, value] from registers into `out` (bytes [0, head) of the
+// message).  Full words are stored whole, the last partial word byte by byte,
+// so the value bytes other lanes write after `head` are never overwritten.
+__device__ __forceinline__ void leaf_head(uint8_t *out, uint32_t payload, const uint64_t kw[6], uint32_t koff,
+                                          uint32_t ks, uint32_t cnt, uint32_t term, uint32_t vsz, bool v_raw,
+                                          uint32_t vb0) {
+  WordWriter w((uint64_t *)out);
   uint32_t hn;
   uint64_t h = rlp_hdr(payload, 0xC0, 0xF7, &hn);
   w.inline_bytes(h, hn);
-  // compact key: flag byte, then the nibble pairs
+  const uint32_t kc = cnt / 2 + 1;
   uint64_t b0 = funnel8(kw[0], kw[1], koff);
   uint32_t first_nib = (ks & 1) ? (uint32_t)(b0 & 15) : (uint32_t)((b0 >> 4) & 15);
   uint32_t flag = (term << 5) | ((cnt & 1) << 4) | ((cnt & 1) ? first_nib : 0u);
@@ -503,66 +494,109 @@ __device__ __forceinline__ bool encode_leaf_fast(const Pools &P, ref_t r, uint64
       }
     }
   }
-  // value item
   if (v_raw) {
-    w.inline_bytes(v0, 1);
+    w.inline_bytes(vb0, 1);
   } else {
     h = rlp_hdr(vsz, 0x80, 0xB7, &hn);
     w.inline_bytes(h, hn);
-#pragma unroll
-    for (int k = 0; k < 15; k++) {
-      if ((uint32_t)(8 * k) < vsz) {
-        uint32_t t = min(8u, vsz - 8 * k);
-        w.push(low_bytes(funnel8(vw[k], vw[k + 1], voff), t), t);
-      }
-    }
   }
-  w.finish_keccak(nb * 17);
-  *enc_out = enc;
-  *nb_out = nb;
-  return true;
+  for (uint32_t q = 0; q < w.accn; q++) out[8 * w.nw + q] = (uint8_t)(w.acc >> (8 * q));
 }
 
-// Work counters of a hash pass: one reduction over the encoding lengths
-// after the last level (per-wave atomics on one counter serialise when waves
-// are many).
-__global__ __launch_bounds__(256) void k_hc_count(Pools P, const uint32_t *__restrict__ enc_len,
-                                                  const uint32_t *__restrict__ nvalid) {
-  __shared__ unsigned long long sp[4], sb[4];
-  const uint32_t n = *nvalid;
-  unsigned long long perms = 0, bytes = 0;
-  for (uint32_t i = grid_tid(); i < n; i += gridDim.x * blockDim.x) {
-    uint32_t e = enc_len[i];
-    bytes += e;
-    perms += e >= 32 ? keccak_blocks(e) : 0u;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    perms += (unsigned long long)__shfl_xor((long long)perms, off);
-    bytes += (unsigned long long)__shfl_xor((long long)bytes, off);
-  }
-  if (__lane_id() == 0) {
-    sp[threadIdx.x >> 6] = perms;
-    sb[threadIdx.x >> 6] = bytes;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(&P.cnt->n_perms, sp[0] + sp[1] + sp[2] + sp[3]);
-    atomicAdd(&P.cnt->enc_bytes, sb[0] + sb[1] + sb[2] + sb[3]);
-  }
-}
+constexpr uint32_t kLeafGroup = 8;                 // lanes per leaf
+constexpr uint32_t kLeafNodes = 256 / kLeafGroup;  // leaves per block
+constexpr uint32_t kLeafWords = 34;                // two Keccak blocks per leaf in LDS
 
-// Leaf level: one thread per leaf short node.
+// Leaf level: 8 lanes per leaf short node [compact key, value].  For a value of
+// <= 120 bytes and a key remainder of <= 64 nibbles, lane s loads value bytes
+// [16 s, 16 s + 16) (the group's loads cover the value contiguously), lane 0
+// builds the headers and the compact key from registers, all write into an LDS
+// copy of the padded message, and the group stores the message words to the
+// scratch slot (consecutive leaves have adjacent slots, so the block's stores
+// are contiguous).  Other leaves take the generic encoder on lane 0.
 __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__restrict__ list,
                                                         const uint64_t *__restrict__ slot, uint32_t n,
                                                         uint64_t *__restrict__ scratch,
                                                         uint32_t *__restrict__ enc_len) {
-  uint32_t i = grid_tid();
-  if (i >= n) return;
-  ref_t r = list[i];
-  uint64_t *out = scratch + slot[i] / 8;
-  uint32_t enc, nb;
-  if (!encode_leaf_fast(P, r, out, &enc, &nb)) encode_generic(P, r, out, &enc);
-  enc_len[i] = enc;
+  __shared__ uint64_t lds[kLeafNodes][kLeafWords];
+  const uint32_t g = threadIdx.x / kLeafGroup, s = threadIdx.x % kLeafGroup;
+  const uint32_t i = blockIdx.x * kLeafNodes + g;
+  const bool active = i < n;
+  for (uint32_t q = s; q < kLeafWords; q += kLeafGroup) lds[g][q] = 0;
+  const ref_t r = active ? list[i] : REF_NULL;
+  uint64_t key_off = 0, va = 0;
+  uint32_t ks = 0, cnt = 0, term = 0, vsz = 0;
+  bool fast = false;
+  if (active) {
+    const Short &S = P.shrt[ref_idx(r)];
+    const ref_t ch = S.child;
+    key_off = S.key_off;
+    ks = S.key_start;
+    cnt = (uint32_t)S.key_len - S.term;
+    term = S.term;
+    if (ref_type(ch) == T_VALUE) {
+      const Value &V = P.value[ref_idx(ch)];
+      va = V.addr;
+      vsz = V.size;
+      fast = vsz <= kLeafFastValue && cnt <= kLeafFastNibbles;
+    }
+  }
+  // this lane's 16 value bytes (zero beyond the value)
+  const uint32_t voff = (uint32_t)(va & 7);
+  uint64_t v0 = 0, v1 = 0;
+  uint32_t vn = 0;  // value bytes held by this lane
+  if (fast && 16 * s < vsz) {
+    const uint64_t *vw = (const uint64_t *)(va - voff);
+    const uint32_t nvw = (voff + vsz + 7) / 8;
+    const uint64_t a0 = vw[2 * s];
+    const uint64_t a1 = 2 * s + 1 < nvw ? vw[2 * s + 1] : 0ull;
+    const uint64_t a2 = 2 * s + 2 < nvw ? vw[2 * s + 2] : 0ull;
+    vn = min(16u, vsz - 16 * s);
+    v0 = low_bytes(funnel8(a0, a1, voff), vn);
+    v1 = vn > 8 ? low_bytes(funnel8(a1, a2, voff), vn - 8) : 0ull;
+  }
+  const uint32_t vb0 = (uint32_t)__shfl((int)(uint32_t)v0, 0, kLeafGroup) & 0xffu;
+  const uint32_t kc = cnt / 2 + 1;
+  const uint32_t key_item = kc == 1 ? 1u : 1u + kc;
+  const bool v_raw = vsz == 1 && vb0 <= 0x7f;
+  const uint32_t vh = v_raw ? 0u : rlp_hdr_len(vsz);
+  const uint32_t payload = key_item + (v_raw ? 1u : vh + vsz);
+  const uint32_t enc = rlp_hdr_len(payload) + payload;
+  fast = fast && enc >= 32;
+  __syncthreads();
+  uint8_t *b = (uint8_t *)lds[g];
+  if (fast) {
+    if (s == 0) {
+      // packed key bytes from byte ks / 2: up to 33 bytes, 6 aligned words
+      const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
+      const uint32_t koff = (uint32_t)(ka & 7);
+      const uint64_t *kw_p = (const uint64_t *)(ka - koff);
+      const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
+      const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;
+      uint64_t kw[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
+      leaf_head(b, payload, kw, koff, ks, cnt, term, vsz, v_raw, vb0);
+      const uint32_t nb = keccak_blocks(enc);
+      b[enc] = 0x01;  // Keccak padding
+      b[136 * nb - 1] |= 0x80;
+    }
+    if (!v_raw && vn) {
+      const uint32_t pos = enc - vsz + 16 * s;
+      for (uint32_t q = 0; q < vn; q++) b[pos + q] = (uint8_t)((q < 8 ? v0 : v1) >> (8 * (q & 7)));
+    }
+  }
+  __syncthreads();
+  if (fast) {
+    const uint32_t nb = keccak_blocks(enc);
+    uint64_t *out = scratch + slot[i] / 8;
+    for (uint32_t q = s; q < 17 * nb; q += kLeafGroup) out[q] = lds[g][q];
+    if (s == 0) enc_len[i] = enc;
+  } else if (active && s == 0) {
+    uint32_t e;
+    encode_generic(P, r, scratch + slot[i] / 8, &e);
+    enc_len[i] = e;
+  }
 }
 
 constexpr uint32_t kInnerGroup = 16;              // lanes per node (one per nibble slot)
@@ -913,7 +947,7 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (leaves)
-    k_hc_encode_leaf<<<nblk(n), kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+    k_hc_encode_leaf<<<(n + kLeafNodes - 1) / kLeafNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
   else
     k_hc_encode_inner<<<(n + kInnerNodes - 1) / kInnerNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
   GMPT_LAUNCH_CHECK();

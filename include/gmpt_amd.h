@@ -123,7 +123,7 @@ int gmpt_forest_clear(gmpt_forest *f);
 int gmpt_synchronize(gmpt_forest *f);
 /* Tunables by name.  "hc_spread_max": PhaseHC levels with at most this many
  * dirty nodes run Keccak with one state spread over 25 lanes (latency), larger
- * levels one state per lane (throughput).  Default 8192, or the
+ * levels one state per lane (throughput).  Default 1024, or the
  * GMPT_HC_SPREAD_MAX environment variable.  No reference counterpart. */
 int gmpt_set_option(gmpt_forest *f, const char *name, int64_t value);
 int gmpt_get_option(gmpt_forest *f, const char *name, int64_t *value);
