@@ -27,10 +27,24 @@ PKG_DIR = os.path.join(ROOT, "gpu-merkle-patricia-trie_amd")
 # MI355X published peaks used for the roofline
 HBM_PEAK_GBS = 8000.0
 CUS, LANES, PEAK_CLOCK_GHZ = 256, 64, 2.4
-# 64-bit logical ops per second: the VALU is 32-bit, a 64-bit xor/and/rotate
-# takes two lane-ops
-VALU64_PEAK_TOPS = CUS * LANES * PEAK_CLOCK_GHZ * 1e9 / 2 / 1e12
+VALU_PEAK_TOPS = CUS * LANES * PEAK_CLOCK_GHZ * 1e9 / 1e12  # 32-bit lane operations
+# 64-bit logical ops per second when every 64-bit xor/and/rotate costs two
+# 32-bit lane-ops
+VALU64_PEAK_TOPS = VALU_PEAK_TOPS / 2
 OPS64_PER_PERM = 3720  # Keccak-f[1600]: 155 64-bit lane-ops per round x 24 rounds
+# The fewest 32-bit VALU instructions a Keccak-f[1600] needs on gfx950, whose
+# 3-input v_bitop3_b32 does xor3 and chi's a ^ (~b & c) in one op: per round
+# theta parities 20, parity rotations 10, theta updates 50, rho 48, chi 50,
+# iota 2 = 180; x 24 rounds.
+INSTR_PER_PERM = 4320
+PERM_PEAK_G = VALU_PEAK_TOPS * 1e12 / INSTR_PER_PERM / 1e9
+
+
+# HBM bytes per launch of the leaf-level k_hc_perm_lane on the default 1M-key
+# workload: rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per dispatch,
+# the factors being what the same counters read for the 1 GiB k_copy16
+# calibration copy (--config hbm-copy).
+LEAF_PERM_TRAFFIC_BYTES = 307_465_891
 
 
 def _import_package():
@@ -87,12 +101,14 @@ def bench_phase(pkg, args):
         step()
     st = t.stats()
     t.synchronize()
-    hash_ms = []
+    hash_ms, leaf_ms_l = [], []
     t.timer_start()
     w0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-        hash_ms.append(t.stats()["last_hash_ms"])
+        s1 = t.stats()
+        hash_ms.append(s1["last_hash_ms"])
+        leaf_ms_l.append(s1["leaf_perm_ms"])
     t.synchronize()
     wall = time.perf_counter() - w0
     dev_ms = t.timer_stop()
@@ -100,14 +116,18 @@ def bench_phase(pkg, args):
     ms_per_step = wall * 1e3 / args.steps
     value = b.n * args.steps / wall
 
-    # roofline of the dominant kernel (PhaseHC's fused RLP encode + Keccak)
+    # roofline of the dominant kernel: the leaf level's Keccak permutations
+    # (k_hc_perm_lane), timed with HIP events on the forest's stream
     perms = st["perms"]
     hms = sum(hash_ms) / len(hash_ms)
-    achieved = OPS64_PER_PERM * perms / (hms * 1e-3) / 1e12
+    leaf_ms = sum(leaf_ms_l) / len(leaf_ms_l)
+    achieved = st["leaf_perms"] / (leaf_ms * 1e-3) / 1e9
     # whole-path roofline: input bytes + node bytes over HBM, Keccak over VALU
     b_in = b.key_bytes + 8 * b.n + b.value_bytes + 16 * b.n + 8 * b.n
     b_nodes = st["enc_bytes"] + 32 * st["hashed_nodes"]
-    t_roof = (b_in + b_nodes) / (HBM_PEAK_GBS * 1e9) + OPS64_PER_PERM * perms / (VALU64_PEAK_TOPS * 1e12)
+    t_hbm = (b_in + b_nodes) / (HBM_PEAK_GBS * 1e9)
+    t_roof = t_hbm + OPS64_PER_PERM * perms / (VALU64_PEAK_TOPS * 1e12)
+    t_roof_isa = t_hbm + perms / (PERM_PEAK_G * 1e9)
 
     out = {
         "metric": "MPT puts+root-hash ops/s at 1M 32B keys", "value": value, "unit": "puts+root-hash/s",
@@ -117,13 +137,18 @@ def bench_phase(pkg, args):
         "config": {"workload": "PhaseNU+PhaseHC, 1,000,000 unique 32-byte keys, 70-110-byte values, "
                                "one batch into an empty trie, inputs resident in HBM",
                    "keys": b.n, "value_bytes": b.value_bytes},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU64_PEAK_TOPS, "unit": "T64op/s",
-                     "frac": achieved / VALU64_PEAK_TOPS, "traffic": None,
-                     "kernel": "k_hc_hash (RLP encode + Keccak-f[1600], one state per lane)",
-                     "per_step": {"permutations": perms, "nodes_hashed": st["hashed_nodes"],
-                                  "hash_ms": hms, "launches": st["hash_launches"]}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PERM_PEAK_G, "unit": "Gperm/s",
+                     "frac": achieved / PERM_PEAK_G,
+                     "traffic": LEAF_PERM_TRAFFIC_BYTES if b.n == 1_000_000 else None,
+                     "kernel": "k_hc_perm_lane, leaf level (Keccak-f[1600], one state per lane)",
+                     "per_launch": {"permutations": st["leaf_perms"], "nodes": st["leaf_nodes"],
+                                    "ms": leaf_ms, "message_bytes": st["leaf_bytes"]},
+                     "peak_basis": f"{VALU_PEAK_TOPS:.2f} T 32-bit VALU ops/s / {INSTR_PER_PERM} ops per permutation"},
+        "hash": {"permutations": perms, "nodes_hashed": st["hashed_nodes"], "hash_ms": hms,
+                 "launches": st["hash_launches"], "gperm_per_s": perms / (hms * 1e-3) / 1e9},
         "path_roofline": {"t_roof_ms": t_roof * 1e3, "t_measured_ms": ms_per_step,
-                          "frac": t_roof * 1e3 / ms_per_step, "bytes_in": b_in, "bytes_nodes": b_nodes,
+                          "frac": t_roof * 1e3 / ms_per_step, "t_roof_isa_ms": t_roof_isa * 1e3,
+                          "frac_isa": t_roof_isa * 1e3 / ms_per_step, "bytes_in": b_in, "bytes_nodes": b_nodes,
                           "device_ms_per_step": dev_ms / args.steps},
         "mputs_per_s": value / 1e6, "mhashes_per_s": st["hashed_nodes"] / (ms_per_step * 1e-3) / 1e6,
         "autotune": {"hc_spread_max": spread_max, "hash_ms": tune_ms},

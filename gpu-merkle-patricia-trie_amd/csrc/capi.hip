@@ -232,6 +232,10 @@ int gmpt_get_stats(gmpt_forest *f, gmpt_stats *out) {
   out->regroups = s.regroups;
   out->perms = s.perms;
   out->enc_bytes = s.enc_bytes;
+  out->leaf_perm_ms = s.leaf_perm_ms;
+  out->leaf_perms = s.leaf_perms;
+  out->leaf_bytes = s.leaf_bytes;
+  out->leaf_nodes = s.leaf_nodes;
   out->pool_bytes_used = f->f->pool_bytes_used();
   out->pool_bytes_reserved = f->f->pool_bytes_reserved();
   return 0;

@@ -89,6 +89,8 @@ struct Counters {
   uint32_t pad;
   unsigned long long n_perms;    // PhaseHC: Keccak-f permutations run
   unsigned long long enc_bytes;  // PhaseHC: bytes of node encodings produced
+  unsigned long long leaf_perms; // PhaseHC: permutations of the leaf level
+  unsigned long long leaf_bytes; // PhaseHC: encoding bytes of the leaf level
 };
 
 enum : uint32_t {

@@ -43,6 +43,10 @@ struct OpStats {
   uint32_t regroups = 0;       // PhaseNU group regroupings (split groups around a subtree)
   uint64_t perms = 0;          // Keccak-f permutations of the last hash
   uint64_t enc_bytes = 0;      // node encoding bytes of the last hash
+  double leaf_perm_ms = 0;     // device time of the leaf level's permutation kernel
+  uint64_t leaf_perms = 0;     // permutations of the leaf level
+  uint64_t leaf_bytes = 0;     // encoding bytes of the leaf level
+  uint32_t leaf_nodes = 0;
 };
 
 class Forest {
@@ -122,7 +126,8 @@ class Forest {
   DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_;
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
+  hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   OpStats stats_;
 };
 

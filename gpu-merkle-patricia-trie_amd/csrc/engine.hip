@@ -77,6 +77,7 @@ Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
   if (hipHostMalloc(&h_scal_, 64) != hipSuccess) return;
   if (hipMemset(d_cnt_, 0, sizeof(Counters)) != hipSuccess) return;
   if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return;
+  if (hipEventCreate(&ev2_) != hipSuccess || hipEventCreate(&ev3_) != hipSuccess) return;
   if (grow(tries_, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1), 0) != 0) return;
   if (hipMemset(tries_.p, 0, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1)) != hipSuccess) return;
   if (hipHostMalloc(&h_meta_, sizeof(HcMeta)) != hipSuccess) return;
@@ -100,6 +101,8 @@ Forest::~Forest() {
   if (h_meta_) (void)hipHostFree(h_meta_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  if (ev2_) (void)hipEventDestroy(ev2_);
+  if (ev3_) (void)hipEventDestroy(ev3_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -429,6 +432,7 @@ int Forest::hash(void *targets, int n_targets) {
   uint32_t n = cnt_.n_dirty;
   stats_.hash_launches = 0;
   stats_.hashed_nodes = 0;
+  stats_.leaf_nodes = 0;
   stats_.last_hash_ms = 0;
   if (n > 0) {
     Pools P = pools();
@@ -449,31 +453,41 @@ int Forest::hash(void *targets, int n_targets) {
     if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
     uint64_t *scratch = (uint64_t *)hc_scratch_.p;
     HIP_TRY(hipEventRecord(ev0_, stream_));
-    uint32_t start = 0;
+    uint32_t start = 0, n_leaf = 0;
     for (uint32_t lv = 0; lv < kHcLevels; lv++) {
       uint32_t cnt = 0;
       for (uint32_t c = 0; c < kHcClasses; c++) cnt += M.hist[lv * kHcClasses + c];
       if (cnt) {
-        HIP_TRY(hc_hash_range(P, sorted + start, slot + start, cnt, scratch, encl + start, lv == 0, hc_spread_max_,
-                              stream_));
+        const bool leaves = lv == 0;
+        HIP_TRY(hc_hash_range(P, sorted + start, slot + start, cnt, scratch, encl + start, leaves, hc_spread_max_,
+                              stream_, leaves ? ev2_ : nullptr, leaves ? ev3_ : nullptr));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
+        if (leaves) n_leaf = cnt;
       }
       start += cnt;
     }
     HIP_TRY(hipEventRecord(ev1_, stream_));
-    HIP_TRY(hc_count(P, encl, (const HcMeta *)hc_meta_.p, n, stream_));
+    HIP_TRY(hc_count(P, encl, (const HcMeta *)hc_meta_.p, n, n_leaf, stream_));
+    stats_.leaf_nodes = n_leaf;
   }
   HIP_TRY(hc_roots(pools(), ntries_, stream_));
   HIP_TRY(hipMemsetAsync(&d_cnt_->n_dirty, 0, sizeof(uint32_t), stream_));
   if ((r = sync_counters())) return r;
   stats_.perms = cnt_.n_perms;
   stats_.enc_bytes = cnt_.enc_bytes;
-  HIP_TRY(hipMemsetAsync(&d_cnt_->n_perms, 0, 16, stream_));
+  stats_.leaf_perms = cnt_.leaf_perms;
+  stats_.leaf_bytes = cnt_.leaf_bytes;
+  HIP_TRY(hipMemsetAsync(&d_cnt_->n_perms, 0, 4 * sizeof(unsigned long long), stream_));
+  stats_.leaf_perm_ms = 0;
   if (n > 0) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.last_hash_ms = ms;
+    if (stats_.leaf_nodes) {
+      HIP_TRY(hipEventElapsedTime(&ms, ev2_, ev3_));
+      stats_.leaf_perm_ms = ms;
+    }
   }
   return 0;
 }

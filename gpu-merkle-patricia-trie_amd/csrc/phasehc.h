@@ -27,12 +27,16 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s);
 // Encodes and hashes list[0..n) (one level: no node depends on another);
 // slot and enc_len are the level's slices of the classification arrays;
-// leaves = the leaf level (level 0: only leaf short nodes).
+// leaves = the leaf level (level 0: only leaf short nodes).  Events, when
+// given, are recorded around the permutation kernel.
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
-                         uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s);
+                         uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s,
+                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr);
 // Adds the pass's permutations and encoding bytes (from the enc_len array of
-// the meta->nvalid hashed nodes) to the forest counters.
-hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, hipStream_t s);
+// the meta->nvalid hashed nodes) to the forest counters, and those of the
+// first n_leaf entries (the leaf level) to the leaf counters.
+hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
+                    hipStream_t s);
 // Copies every trie's root hash into its header.
 hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s);
 // Node export: encoding lengths, then encodings at offs and 32-byte hash slots.

@@ -428,10 +428,11 @@ constexpr uint32_t kLeafFastNibbles = 64; // key nibbles held in registers
 // Work counters of a hash pass: one reduction over the encoding lengths
 // after the last level (per-wave atomics on one counter serialise when waves
 // are many).
-__global__ __launch_bounds__(256) void k_hc_count(Pools P, const uint32_t *__restrict__ enc_len,
-                                                  const uint32_t *__restrict__ nvalid) {
+__global__ __launch_bounds__(256) void k_hc_count(const uint32_t *__restrict__ enc_len,
+                                                  const uint32_t *__restrict__ nvalid, uint32_t limit,
+                                                  unsigned long long *perms_out, unsigned long long *bytes_out) {
   __shared__ unsigned long long sp[4], sb[4];
-  const uint32_t n = *nvalid;
+  const uint32_t n = min(*nvalid, limit);
   unsigned long long perms = 0, bytes = 0;
   for (uint32_t i = grid_tid(); i < n; i += gridDim.x * blockDim.x) {
     uint32_t e = enc_len[i];
@@ -448,8 +449,8 @@ __global__ __launch_bounds__(256) void k_hc_count(Pools P, const uint32_t *__res
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(&P.cnt->n_perms, sp[0] + sp[1] + sp[2] + sp[3]);
-    atomicAdd(&P.cnt->enc_bytes, sb[0] + sb[1] + sb[2] + sb[3]);
+    atomicAdd(perms_out, sp[0] + sp[1] + sp[2] + sp[3]);
+    atomicAdd(bytes_out, sb[0] + sb[1] + sb[2] + sb[3]);
   }
 }
 
@@ -944,23 +945,36 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 }
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
-                         uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s) {
+                         uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
+                         hipEvent_t perm_end) {
   if (n == 0) return hipSuccess;
   if (leaves)
     k_hc_encode_leaf<<<(n + kLeafNodes - 1) / kLeafNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
   else
     k_hc_encode_inner<<<(n + kInnerNodes - 1) / kInnerNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
   GMPT_LAUNCH_CHECK();
+  if (perm_begin) {
+    hipError_t e = hipEventRecord(perm_begin, s);
+    if (e != hipSuccess) return e;
+  }
   if (n <= spread_max)
     k_hc_perm_spread<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
   else
     k_hc_perm_lane<<<nblk(n), kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
-  return hipGetLastError();
+  GMPT_LAUNCH_CHECK();
+  if (perm_end) return hipEventRecord(perm_end, s);
+  return hipSuccess;
 }
 
-hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, hipStream_t s) {
+hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
+                    hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n), 512), kB, 0, s>>>(P, enc_len, &meta->nvalid);
+  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n), 512), kB, 0, s>>>(enc_len, &meta->nvalid, n, &P.cnt->n_perms,
+                                                                     &P.cnt->enc_bytes);
+  GMPT_LAUNCH_CHECK();
+  if (n_leaf == 0) return hipSuccess;
+  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n_leaf), 512), kB, 0, s>>>(enc_len, &meta->nvalid, n_leaf,
+                                                                          &P.cnt->leaf_perms, &P.cnt->leaf_bytes);
   return hipGetLastError();
 }
 

@@ -33,7 +33,9 @@ class Stats(ctypes.Structure):
                 ("hashed_nodes", ctypes.c_uint32), ("restarts", ctypes.c_uint32), ("lock_fails", ctypes.c_uint32),
                 ("fallback_sorts", ctypes.c_uint32), ("regroups", ctypes.c_uint32), ("perms", ctypes.c_uint64),
                 ("enc_bytes", ctypes.c_uint64), ("pool_bytes_used", ctypes.c_uint64),
-                ("pool_bytes_reserved", ctypes.c_uint64)]
+                ("pool_bytes_reserved", ctypes.c_uint64), ("leaf_perm_ms", ctypes.c_double),
+                ("leaf_perms", ctypes.c_uint64), ("leaf_bytes", ctypes.c_uint64), ("leaf_nodes", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -60,6 +60,11 @@ typedef struct gmpt_stats {
   uint64_t enc_bytes;             /* node encoding bytes produced by the last hash */
   uint64_t pool_bytes_used;       /* HBM held by nodes, keys and values */
   uint64_t pool_bytes_reserved;   /* HBM reserved by the pools */
+  double leaf_perm_ms;            /* device time of the leaf level's permutation kernel (last hash) */
+  uint64_t leaf_perms;            /* its permutations */
+  uint64_t leaf_bytes;            /* encoding bytes of the leaf level */
+  uint32_t leaf_nodes;            /* leaf nodes hashed */
+  uint32_t pad;
 } gmpt_stats;
 
 /* Creates a forest of num_tries empty tries on device `device`.
