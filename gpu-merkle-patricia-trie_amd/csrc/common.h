@@ -122,45 +122,54 @@ __device__ __forceinline__ uint32_t packed_nibble(const uint8_t *k, uint32_t j) 
   return (j & 1) ? (b & 15u) : (b >> 4);
 }
 
-// Wave-aggregated bump allocation: every active lane that passes want=true
-// gets a distinct slot; one atomic per wave.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool want) {
-  uint64_t mask = __ballot(want);
-  if (!want) return 0;
-  uint32_t lane = __lane_id();
-  uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(below);
-}
-
-__device__ __forceinline__ void mark_new_dirty(const Pools &P, ref_t r, bool want) {
-  uint32_t slot = wave_alloc(&P.cnt->n_dirty, want);
-  if (want) {
-    if (slot < P.cap_dirty)
-      P.dirty[slot] = r;
-    else
-      raise_error(P.cnt, ERR_POOL, 4);
-  }
-}
-
 __device__ __forceinline__ uint32_t *node_dirty_flag(const Pools &P, ref_t r) {
   uint32_t t = ref_type(r), i = ref_idx(r);
   return t == T_BRANCH ? &P.branch[i].dirty : &P.shrt[i].dirty;
 }
 
-// Set a node's dirty flag; the lane that flips it 0 -> 1 appends the node to
-// the dirty list, so every node is listed at most once per hashing round.
-__device__ __forceinline__ void mark_dirty(const Pools &P, ref_t r, bool want) {
-  bool flip = false;
-  if (want) {
-    uint32_t *f = node_dirty_flag(P, r);
-    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-      flip = atomicExch(f, 1u) == 0u;
+// Block-staged dirty-list appends: lanes stage the nodes they flip in LDS
+// (an LDS atomic each) and the block appends them to the global list with one
+// global atomic in dirty_flush().  Appending straight to the list costs a
+// global atomic on one counter per wave and mark, which serialises when many
+// waves mark paths.  A full stage falls back to direct appends.
+struct DirtyStage {
+  ref_t *buf;     // LDS
+  uint32_t *cnt;  // LDS
+  uint32_t cap;
+  __device__ __forceinline__ void push(const Pools &P, ref_t r) {
+    uint32_t pos = atomicAdd(cnt, 1u);
+    if (pos < cap) {
+      buf[pos] = r;
+      return;
+    }
+    uint32_t slot = atomicAdd(&P.cnt->n_dirty, 1u);
+    if (slot < P.cap_dirty)
+      P.dirty[slot] = r;
+    else
+      raise_error(P.cnt, ERR_POOL, 4);
   }
-  mark_new_dirty(P, r, flip);
+  // marks an existing node: the lane that flips its flag stages it
+  __device__ __forceinline__ void mark(const Pools &P, ref_t r) {
+    uint32_t *f = node_dirty_flag(P, r);
+    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 && atomicExch(f, 1u) == 0u)
+      push(P, r);
+  }
+};
+
+// Every thread of the block calls this once, after its last push.
+__device__ __forceinline__ void dirty_flush(const Pools &P, const DirtyStage &ds) {
+  __shared__ uint32_t base;
+  __syncthreads();
+  const uint32_t n = min(*ds.cnt, ds.cap);
+  if (threadIdx.x == 0) base = n ? atomicAdd(&P.cnt->n_dirty, n) : 0u;
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    uint32_t slot = base + k;
+    if (slot < P.cap_dirty)
+      P.dirty[slot] = ds.buf[k];
+    else
+      raise_error(P.cnt, ERR_POOL, 4);
+  }
 }
 
 }  // namespace gmpt

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.h"
+#include "phasenu.h"
 
 namespace gmpt {
 
@@ -106,6 +107,8 @@ class Forest {
   int ensure_ws(DevBuf &b, size_t bytes);
   int sync_counters();
   int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
+  int carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid, void **temp);
+  int sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, const uint32_t **sv_out);
   uint8_t *value_space(uint64_t bytes);
 
   int device_ = 0;

@@ -292,6 +292,113 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
   return 0;
 }
 
+// Workspace of one PhaseNU-style call: every per-key array of NuArgs carved
+// from ws_ (sized by nu_ws_bytes), batch pointers and slot bases filled in.
+int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid, void **temp) {
+  const uint32_t n = d.n;
+  int r;
+  if ((r = ensure_ws(ws_, nu_ws_bytes(n, has_tid)))) return r;
+  Carver c(ws_.p);
+  memset(&a, 0, sizeof(a));
+  a.P = pools();
+  a.kh = d.keys_hexs;
+  a.kidx = d.keys_indexs;
+  a.vidx = d.values_indexs;
+  a.hps = d.values_hps;
+  a.tid = has_tid ? d.trie_ids : nullptr;
+  a.n = n;
+  a.ntries = ntries_;
+  a.kh_bytes = d.keys_bytes;
+  a.v_bytes = d.values_total;
+  a.kbase = key_used_;
+  a.vbase = vbase;
+  a.base_branch = cnt_.n_branch;
+  a.base_short = cnt_.n_short;
+  a.base_value = cnt_.n_value;
+  size_t ne = 2 * (size_t)n + 2;
+  a.klen = c.take<uint32_t>(n);
+  a.kpoff = c.take<uint64_t>(n);
+  a.sk0 = c.take<uint64_t>(n);
+  a.sk1 = c.take<uint64_t>(n);
+  a.sv0 = c.take<uint32_t>(n);
+  a.sv1 = c.take<uint32_t>(n);
+  if (has_tid) {
+    a.tk0 = c.take<uint32_t>(n);
+    a.tk1 = c.take<uint32_t>(n);
+  }
+  a.scal = c.take<uint32_t>(16);
+  a.lcp = c.take<uint8_t>(n);
+  a.flags = c.take<uint8_t>(n);
+  a.run = c.take<uint32_t>(n);
+  a.u_idx = c.take<uint32_t>(n);
+  a.u_trie = c.take<uint32_t>(n);
+  a.u_lcp = c.take<uint8_t>(n);
+  a.u_first = c.take<uint8_t>(n);
+  a.stop = c.take<Stop>(n);
+  a.grp = c.take<uint32_t>(n);
+  a.ginfo = c.take<GroupInfo>(n);
+  a.ecount = c.take<uint32_t>(n);
+  a.merge = c.take<uint32_t>(n);
+  a.u_idx2 = c.take<uint32_t>(n);
+  a.u_trie2 = c.take<uint32_t>(n);
+  a.perm0 = c.take<uint32_t>(n);
+  a.perm1 = c.take<uint32_t>(n);
+  a.gkey0 = c.take<uint32_t>(n);
+  a.gkey1 = c.take<uint32_t>(n);
+  a.u_lcp2 = c.take<uint8_t>(n);
+  a.stop2 = c.take<Stop>(n);
+  a.e_src = c.take<uint32_t>(ne);
+  a.e_grp = c.take<uint32_t>(ne);
+  a.st_levels = st_levels_for(ne);
+  a.st_stride = align_up(ne, 256);
+  a.st = c.take<uint8_t>(a.st_stride * (size_t)a.st_levels);
+  a.repof = c.take<uint32_t>(ne);
+  a.bid = c.take<uint32_t>(ne);
+  *temp = c.take<uint8_t>(std::max(radix_temp_bytes(n), scan_temp_bytes(ne)));
+  if (c.off > ws_.cap) {
+    set_error("workspace carve overflow");
+    return -4;
+  }
+  HIP_TRY(hipMemsetAsync(a.scal, 0, 64, stream_));
+  HIP_TRY(hipMemsetAsync(a.scal + SC_W, 0xff, 4, stream_));
+  return 0;
+}
+
+// Packs the keys, sorts them (by trie, then nibble path) and keeps the last
+// occurrence of each key: u_idx[0 .. scal[SC_M_UNIQ]) in sorted order.
+int Forest::sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, const uint32_t **sv_out) {
+  int r;
+  HIP_TRY(nu_prepare_keys(a, temp, stream_));
+  uint64_t *sk = nullptr;
+  uint32_t *sv = nullptr;
+  int cur = 0;
+  HIP_TRY(nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv, false, &cur));
+  HIP_TRY(nu_adjacent(a, sk, sv, 0, stream_));
+  if ((r = sync_counters())) return r;
+  if (cnt_.flag_fallback & 2u) {
+    // a run of equal leading window bits was too long to fix up: sort by
+    // all 64 window bits
+    HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+    HIP_TRY(nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv, true, &cur));
+    HIP_TRY(nu_adjacent(a, sk, sv, 0, stream_));
+    if ((r = sync_counters())) return r;
+  }
+  if (cnt_.flag_fallback) {
+    // Two distinct keys share the 16-nibble window: sort by the full keys.
+    stats_.fallback_sorts++;
+    HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    uint32_t W = h_scal_[SC_W], maxlen = h_scal_[SC_MAXLEN];
+    int nwords = (int)((maxlen > W ? maxlen - W : 1) + 15) / 16;
+    HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, false, &cur));
+    HIP_TRY(nu_adjacent(a, sk, sv, 1, stream_));
+  }
+  HIP_TRY(nu_unique(a, sv, temp, stream_));
+  *sv_out = sv;
+  return 0;
+}
+
 int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
   int r;
   if ((r = set_device())) return r;
@@ -313,93 +420,15 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     // slots: branches <= 2n+2, shorts <= 2(2n+2), values: 2n+2 for elements + n for updates
     if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, 3ull * n + 2, d.keys_bytes / 2 + 8ull * n))) return r;
     const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
-    if ((r = ensure_ws(ws_, nu_ws_bytes(n, has_tid)))) return r;
-    Carver c(ws_.p);
     NuArgs a;
-    memset(&a, 0, sizeof(a));
-    a.P = pools();
-    a.kh = d.keys_hexs;
-    a.kidx = d.keys_indexs;
-    a.vidx = d.values_indexs;
-    a.hps = d.values_hps;
-    a.tid = has_tid ? d.trie_ids : nullptr;
-    a.n = n;
-    a.ntries = ntries_;
-    a.kh_bytes = d.keys_bytes;
-    a.v_bytes = d.values_total;
-    a.kbase = key_used_;
-    a.vbase = vbase;
-    a.base_branch = cnt_.n_branch;
-    a.base_short = cnt_.n_short;
-    a.base_value = cnt_.n_value;
-    size_t ne = 2 * (size_t)n + 2;
-    a.klen = c.take<uint32_t>(n);
-    a.kpoff = c.take<uint64_t>(n);
-    a.sk0 = c.take<uint64_t>(n);
-    a.sk1 = c.take<uint64_t>(n);
-    a.sv0 = c.take<uint32_t>(n);
-    a.sv1 = c.take<uint32_t>(n);
-    if (has_tid) {
-      a.tk0 = c.take<uint32_t>(n);
-      a.tk1 = c.take<uint32_t>(n);
-    }
-    a.scal = c.take<uint32_t>(16);
-    a.lcp = c.take<uint8_t>(n);
-    a.flags = c.take<uint8_t>(n);
-    a.run = c.take<uint32_t>(n);
-    a.u_idx = c.take<uint32_t>(n);
-    a.u_trie = c.take<uint32_t>(n);
-    a.u_lcp = c.take<uint8_t>(n);
-    a.u_first = c.take<uint8_t>(n);
-    a.stop = c.take<Stop>(n);
-    a.grp = c.take<uint32_t>(n);
-    a.ginfo = c.take<GroupInfo>(n);
-    a.ecount = c.take<uint32_t>(n);
-    a.merge = c.take<uint32_t>(n);
-    a.u_idx2 = c.take<uint32_t>(n);
-    a.u_trie2 = c.take<uint32_t>(n);
-    a.perm0 = c.take<uint32_t>(n);
-    a.perm1 = c.take<uint32_t>(n);
-    a.gkey0 = c.take<uint32_t>(n);
-    a.gkey1 = c.take<uint32_t>(n);
-    a.u_lcp2 = c.take<uint8_t>(n);
-    a.stop2 = c.take<Stop>(n);
-    a.e_src = c.take<uint32_t>(ne);
-    a.e_grp = c.take<uint32_t>(ne);
-    a.st_levels = st_levels_for(ne);
-    a.st_stride = align_up(ne, 256);
-    a.st = c.take<uint8_t>(a.st_stride * (size_t)a.st_levels);
-    a.repof = c.take<uint32_t>(ne);
-    a.bid = c.take<uint32_t>(ne);
-    void *temp = c.take<uint8_t>(std::max(radix_temp_bytes(n), scan_temp_bytes(ne)));
-    if (c.off > ws_.cap) {
-      set_error("workspace carve overflow");
-      return -4;
-    }
-    HIP_TRY(hipMemsetAsync(a.scal, 0, 64, stream_));
-    HIP_TRY(hipMemsetAsync(a.scal + SC_W, 0xff, 4, stream_));
-    HIP_TRY(nu_prepare_keys(a, temp, stream_));
+    void *temp = nullptr;
+    if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
     uint32_t trie_bits = 0;
     if (has_tid) {
       while ((1ull << trie_bits) < ntries_) trie_bits++;
     }
-    uint64_t *sk = nullptr;
-    uint32_t *sv = nullptr;
-    HIP_TRY(nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv));
-    HIP_TRY(nu_adjacent(a, sk, sv, 0, stream_));
-    if ((r = sync_counters())) return r;
-    if (cnt_.flag_fallback) {
-      // Two distinct keys share the 16-nibble window: sort by the full keys.
-      stats_.fallback_sorts++;
-      HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
-      HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
-      HIP_TRY(hipStreamSynchronize(stream_));
-      uint32_t W = h_scal_[SC_W], maxlen = h_scal_[SC_MAXLEN];
-      int nwords = (int)((maxlen > W ? maxlen - W : 1) + 15) / 16;
-      HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv));
-      HIP_TRY(nu_adjacent(a, sk, sv, 1, stream_));
-    }
-    HIP_TRY(nu_unique(a, sv, temp, stream_));
+    const uint32_t *sv = nullptr;
+    if ((r = sort_unique(a, trie_bits, temp, &sv))) return r;
     HIP_TRY(nu_descend_and_group(a, temp, stream_));
     HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
     if ((r = sync_counters())) return r;
@@ -758,6 +787,10 @@ int Forest::export_nodes(const BatchDesc &keys, uint8_t **hashes, uint8_t **encs
   return export_paths(this, d, hashes, encs, enc_indexs, num);
 }
 
+// LockNU.  Writes of the same key are combined first (the batch is sorted
+// and only each key's last write is kept, which is the final state the
+// sequential order gives), then the surviving writes insert concurrently with
+// optimistic lock coupling.
 int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
   int r;
   if ((r = set_device())) return r;
@@ -769,46 +802,26 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
     set_error("device batch needs keys_bytes and values_total");
     return -3;
   }
+  stats_.fallback_sorts = 0;
+  stats_.regroups = 0;
   if (b.n > 0) {
     BatchDesc d;
     uint64_t vbase = 0;
     if ((r = stage_batch(b, &d, &vbase, true))) return r;
     const uint32_t n = b.n;
     if ((r = ensure_pools(n, 2ull * n, n, d.keys_bytes / 2 + 8ull * n))) return r;
-    if ((r = ensure_ws(ws_, 20ull * n + 8192 + scan_temp_bytes(n)))) return r;
-    HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
-    Carver c(ws_.p);
+    const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
     NuArgs a;
-    memset(&a, 0, sizeof(a));
-    a.P = pools();
-    a.kh = d.keys_hexs;
-    a.kidx = d.keys_indexs;
-    a.vidx = d.values_indexs;
-    a.hps = d.values_hps;
-    a.tid = ntries_ > 1 ? d.trie_ids : nullptr;
-    a.n = n;
-    a.ntries = ntries_;
-    a.kh_bytes = d.keys_bytes;
-    a.v_bytes = d.values_total;
-    a.kbase = key_used_;
-    a.vbase = vbase;
-    a.klen = c.take<uint32_t>(n);
-    a.kpoff = c.take<uint64_t>(n);
-    a.scal = c.take<uint32_t>(16);
-    a.ecount = c.take<uint32_t>(n);  // per-block partials of the key scan
-    a.grp = c.take<uint32_t>(n);
+    void *temp = nullptr;
+    if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
     a.seq_base = lock_seq_;
-    a.base_branch = cnt_.n_branch;
-    a.base_short = cnt_.n_short;
-    a.base_value = cnt_.n_value;
-    void *temp = c.take<uint8_t>(scan_temp_bytes(n));
-    if (c.off > ws_.cap) {
-      set_error("workspace carve overflow");
-      return -4;
+    uint32_t trie_bits = 0;
+    if (has_tid) {
+      while ((1ull << trie_bits) < ntries_) trie_bits++;
     }
-    HIP_TRY(hipMemsetAsync(a.scal, 0, 64, stream_));
-    HIP_TRY(hipMemsetAsync(a.scal + SC_W, 0xff, 4, stream_));
-    HIP_TRY(nu_prepare_keys(a, temp, stream_));
+    HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
+    const uint32_t *sv = nullptr;
+    if ((r = sort_unique(a, trie_bits, temp, &sv))) return r;
     HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
     if ((r = sync_counters())) return r;
     uint64_t ktot = *(uint64_t *)(h_scal_ + SC_KTOT);

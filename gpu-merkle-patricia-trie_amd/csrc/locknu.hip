@@ -107,15 +107,16 @@ __device__ __forceinline__ ref_t make_leaf(const NuArgs &a, const Ctx &c, uint32
 enum { ATT_DONE = 0, ATT_RESTART = 1, ATT_FAIL = 2 };
 
 // One optimistic attempt.  Returns ATT_DONE, or ATT_RESTART when a version
-// check or latch upgrade failed (nothing was modified).  Dirty marks use wave
-// ballots and so are reported through *mark* for the caller to apply.
+// check or latch upgrade failed (nothing was modified).  Path nodes are marked
+// dirty on the way down; the nodes a finished insert modified or created are
+// reported through mk for the caller to mark.
 struct Marks {
   ref_t existing[2];   // existing nodes whose encoding changes
   ref_t fresh[3];      // new short/branch nodes
   int ne, nf;
 };
 
-__device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk) {
+__device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk, DirtyStage &ds) {
   const Pools &P = a.P;
   mk.ne = mk.nf = 0;
   TrieHdr *H = &P.tries[c.trie];
@@ -151,7 +152,7 @@ __device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk) {
       if (!ver_check(&B->lock, v)) return ATT_RESTART;
       // on the key's path: re-hash (a mark left by an abandoned attempt only
       // costs a redundant re-hash)
-      mark_dirty(P, node, true);
+      ds.mark(P, node);
       parent = node;
       plock = &B->lock;
       pver = v;
@@ -205,7 +206,7 @@ __device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk) {
           mk.existing[mk.ne++] = node;
           return ATT_DONE;
         }
-        mark_dirty(P, node, true);
+        ds.mark(P, node);
         parent = node;
         plock = &S->lock;
         pver = v;
@@ -277,9 +278,19 @@ __device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk) {
   return ATT_RESTART;
 }
 
+// Lane j inserts the j-th surviving write: batch position u_idx[j] (the last
+// write of each distinct key, scal[SC_M_UNIQ] of them).
+constexpr uint32_t kDirtyStage = 4096;  // staged dirty-list entries per block
+
 __global__ __launch_bounds__(256) void k_lock_insert(NuArgs a, uint64_t key_cap, uint32_t max_trips) {
-  uint32_t i = grid_tid();
-  bool active = i < a.n;
+  __shared__ ref_t dbuf[kDirtyStage];
+  __shared__ uint32_t dcnt;
+  if (threadIdx.x == 0) dcnt = 0;
+  __syncthreads();
+  DirtyStage ds{dbuf, &dcnt, kDirtyStage};
+  const uint32_t j = grid_tid();
+  const bool active = j < a.scal[SC_M_UNIQ];
+  const uint32_t i = active ? a.u_idx[j] : 0u;
   Ctx c;
   c.i = i;
   c.key_cap = key_cap;
@@ -297,25 +308,20 @@ __global__ __launch_bounds__(256) void k_lock_insert(NuArgs a, uint64_t key_cap,
   for (uint32_t trip = 0; trip < max_trips; trip++) {
     int res = ATT_RESTART;
     if (!done) {
-      res = attempt(a, c, mk);
+      res = attempt(a, c, mk, ds);
       if (res == ATT_DONE)
         done = true;
       else
         restarts++;
     }
-    bool fin = res == ATT_DONE;
-    // dirty marks of this trip's finished inserts: the modified existing
-    // nodes, then the new nodes
-    for (int q = 0; q < 2; q++) {
-      bool want = fin && q < mk.ne;
-      mark_dirty(a.P, want ? mk.existing[q] : REF_NULL, want);
-    }
-    for (int q = 0; q < 3; q++) {
-      bool want = fin && q < mk.nf;
-      mark_new_dirty(a.P, want ? mk.fresh[q] : REF_NULL, want);
+    if (res == ATT_DONE) {
+      // the modified existing nodes, then the new nodes
+      for (int q = 0; q < mk.ne; q++) ds.mark(a.P, mk.existing[q]);
+      for (int q = 0; q < mk.nf; q++) ds.push(a.P, mk.fresh[q]);
     }
     if (__ballot(!done) == 0) break;
   }
+  dirty_flush(a.P, ds);
   if (active && !done) raise_error(a.P.cnt, ERR_INTERNAL, 20);
   // one atomic pair per wave (per-lane atomics on one counter serialise)
   uint32_t fails = restarts ? 1u : 0u;

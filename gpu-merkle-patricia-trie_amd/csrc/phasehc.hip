@@ -422,8 +422,7 @@ __device__ __forceinline__ uint64_t funnel8(uint64_t a, uint64_t b, uint32_t off
 }
 __device__ __forceinline__ uint64_t low_bytes(uint64_t v, uint32_t n) { return n >= 8 ? v : v & ((1ull << (8 * n)) - 1); }
 
-constexpr uint32_t kLeafFastValue = 120;  // value bytes held in registers
-constexpr uint32_t kLeafFastNibbles = 64; // key nibbles held in registers
+constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
 
 // Work counters of a hash pass: one reduction over the encoding lengths
 // after the last level (per-wave atomics on one counter serialise when waves
@@ -454,14 +453,13 @@ __global__ __launch_bounds__(256) void k_hc_count(const uint32_t *__restrict__ e
   }
 }
 
-// Writes the list header, the compact-key item and the value header of a leaf
-// [compact key, value] from registers into `out` (bytes [0, head) of the
-// message).  Full words are stored whole, the last partial word byte by byte,
-// so the value bytes other lanes write after `head` are never overwritten.
-__device__ __forceinline__ void leaf_head(uint8_t *out, uint32_t payload, const uint64_t kw[6], uint32_t koff,
+// Writes the list header, the compact-key item and the value header (or a
+// single raw value byte) of a leaf [compact key, value] from registers into
+// `out` as whole little-endian words; the bytes past the head are zero.
+__device__ __forceinline__ void leaf_head(uint64_t *out, uint32_t payload, const uint64_t kw[6], uint32_t koff,
                                           uint32_t ks, uint32_t cnt, uint32_t term, uint32_t vsz, bool v_raw,
                                           uint32_t vb0) {
-  WordWriter w((uint64_t *)out);
+  WordWriter w(out);
   uint32_t hn;
   uint64_t h = rlp_hdr(payload, 0xC0, 0xF7, &hn);
   w.inline_bytes(h, hn);
@@ -501,29 +499,44 @@ __device__ __forceinline__ void leaf_head(uint8_t *out, uint32_t payload, const 
     h = rlp_hdr(vsz, 0x80, 0xB7, &hn);
     w.inline_bytes(h, hn);
   }
-  for (uint32_t q = 0; q < w.accn; q++) out[8 * w.nw + q] = (uint8_t)(w.acc >> (8 * q));
+  if (w.accn) out[w.nw] = w.acc;
+}
+
+// Bytes [o, o + 8) of the value at va (vsz bytes), zero outside [0, vsz),
+// little-endian; reads only aligned words that hold value bytes.
+__device__ __forceinline__ uint64_t value_word(uint64_t va, uint32_t vsz, int64_t o) {
+  const int64_t lo = o > 0 ? o : 0, hi = o + 8 < (int64_t)vsz ? o + 8 : (int64_t)vsz;
+  if (lo >= hi) return 0ull;
+  const uint64_t a_lo = va + (uint64_t)lo;
+  const uint64_t *q = (const uint64_t *)(a_lo & ~7ull);
+  const uint32_t sh = (uint32_t)(a_lo & 7);
+  const uint32_t nbytes = (uint32_t)(hi - lo);
+  uint64_t v = q[0] >> (8 * sh);
+  if (sh + nbytes > 8) v |= q[1] << (64 - 8 * sh);
+  v = low_bytes(v, nbytes);
+  return v << (8 * (uint32_t)(lo - o));
 }
 
 constexpr uint32_t kLeafGroup = 8;                 // lanes per leaf
 constexpr uint32_t kLeafNodes = 256 / kLeafGroup;  // leaves per block
-constexpr uint32_t kLeafWords = 34;                // two Keccak blocks per leaf in LDS
+constexpr uint32_t kLeafHeadWords = 6;             // list header + compact key + value header <= 44 bytes
 
-// Leaf level: 8 lanes per leaf short node [compact key, value].  For a value of
-// <= 120 bytes and a key remainder of <= 64 nibbles, lane s loads value bytes
-// [16 s, 16 s + 16) (the group's loads cover the value contiguously), lane 0
-// builds the headers and the compact key from registers, all write into an LDS
-// copy of the padded message, and the group stores the message words to the
-// scratch slot (consecutive leaves have adjacent slots, so the block's stores
-// are contiguous).  Other leaves take the generic encoder on lane 0.
+// Leaf level: 8 lanes per leaf short node [compact key, value] whose key
+// remainder has <= 64 nibbles.  Lane 0 builds the message head (list header,
+// compact key, value header) in LDS; then lane s computes message words s,
+// s + 8, ... of the padded message -- head bytes, value bytes (two aligned
+// loads per word, consecutive across the group) and padding -- and stores
+// them, so the group's loads and stores are contiguous and no store waits on
+// a later load.  Other leaves take the generic encoder on lane 0.
 __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__restrict__ list,
                                                         const uint64_t *__restrict__ slot, uint32_t n,
                                                         uint64_t *__restrict__ scratch,
                                                         uint32_t *__restrict__ enc_len) {
-  __shared__ uint64_t lds[kLeafNodes][kLeafWords];
+  __shared__ uint64_t head[kLeafNodes][kLeafHeadWords];
   const uint32_t g = threadIdx.x / kLeafGroup, s = threadIdx.x % kLeafGroup;
   const uint32_t i = blockIdx.x * kLeafNodes + g;
   const bool active = i < n;
-  for (uint32_t q = s; q < kLeafWords; q += kLeafGroup) lds[g][q] = 0;
+  if (s < kLeafHeadWords) head[g][s] = 0;
   const ref_t r = active ? list[i] : REF_NULL;
   uint64_t key_off = 0, va = 0;
   uint32_t ks = 0, cnt = 0, term = 0, vsz = 0;
@@ -539,24 +552,10 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
       const Value &V = P.value[ref_idx(ch)];
       va = V.addr;
       vsz = V.size;
-      fast = vsz <= kLeafFastValue && cnt <= kLeafFastNibbles;
+      fast = cnt <= kLeafFastNibbles;
     }
   }
-  // this lane's 16 value bytes (zero beyond the value)
-  const uint32_t voff = (uint32_t)(va & 7);
-  uint64_t v0 = 0, v1 = 0;
-  uint32_t vn = 0;  // value bytes held by this lane
-  if (fast && 16 * s < vsz) {
-    const uint64_t *vw = (const uint64_t *)(va - voff);
-    const uint32_t nvw = (voff + vsz + 7) / 8;
-    const uint64_t a0 = vw[2 * s];
-    const uint64_t a1 = 2 * s + 1 < nvw ? vw[2 * s + 1] : 0ull;
-    const uint64_t a2 = 2 * s + 2 < nvw ? vw[2 * s + 2] : 0ull;
-    vn = min(16u, vsz - 16 * s);
-    v0 = low_bytes(funnel8(a0, a1, voff), vn);
-    v1 = vn > 8 ? low_bytes(funnel8(a1, a2, voff), vn - 8) : 0ull;
-  }
-  const uint32_t vb0 = (uint32_t)__shfl((int)(uint32_t)v0, 0, kLeafGroup) & 0xffu;
+  const uint32_t vb0 = fast ? (uint32_t)(value_word(va, vsz, 0) & 0xff) : 0u;
   const uint32_t kc = cnt / 2 + 1;
   const uint32_t key_item = kc == 1 ? 1u : 1u + kc;
   const bool v_raw = vsz == 1 && vb0 <= 0x7f;
@@ -565,33 +564,31 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
   const uint32_t enc = rlp_hdr_len(payload) + payload;
   fast = fast && enc >= 32;
   __syncthreads();
-  uint8_t *b = (uint8_t *)lds[g];
-  if (fast) {
-    if (s == 0) {
-      // packed key bytes from byte ks / 2: up to 33 bytes, 6 aligned words
-      const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
-      const uint32_t koff = (uint32_t)(ka & 7);
-      const uint64_t *kw_p = (const uint64_t *)(ka - koff);
-      const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
-      const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;
-      uint64_t kw[6];
+  if (fast && s == 0) {
+    // packed key bytes from byte ks / 2: up to 33 bytes, 6 aligned words
+    const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
+    const uint32_t koff = (uint32_t)(ka & 7);
+    const uint64_t *kw_p = (const uint64_t *)(ka - koff);
+    const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
+    const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;
+    uint64_t kw[6];
 #pragma unroll
-      for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
-      leaf_head(b, payload, kw, koff, ks, cnt, term, vsz, v_raw, vb0);
-      const uint32_t nb = keccak_blocks(enc);
-      b[enc] = 0x01;  // Keccak padding
-      b[136 * nb - 1] |= 0x80;
-    }
-    if (!v_raw && vn) {
-      const uint32_t pos = enc - vsz + 16 * s;
-      for (uint32_t q = 0; q < vn; q++) b[pos + q] = (uint8_t)((q < 8 ? v0 : v1) >> (8 * (q & 7)));
-    }
+    for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
+    leaf_head(head[g], payload, kw, koff, ks, cnt, term, vsz, v_raw, vb0);
   }
   __syncthreads();
   if (fast) {
     const uint32_t nb = keccak_blocks(enc);
+    const uint32_t vstart = enc - vsz;  // message offset of the value bytes (unless raw, in the head)
     uint64_t *out = scratch + slot[i] / 8;
-    for (uint32_t q = s; q < 17 * nb; q += kLeafGroup) out[q] = lds[g][q];
+    for (uint32_t w = s; w < 17 * nb; w += kLeafGroup) {
+      const uint32_t x0 = 8 * w;
+      uint64_t word = w < kLeafHeadWords ? head[g][w] : 0ull;
+      if (!v_raw && x0 + 8 > vstart && x0 < enc) word |= value_word(va, vsz, (int64_t)x0 - (int64_t)vstart);
+      if (enc >= x0 && enc < x0 + 8) word |= 0x01ull << (8 * (enc - x0));  // Keccak padding
+      if (w + 1 == 17 * nb) word |= 0x80ull << 56;
+      out[w] = word;
+    }
     if (s == 0) enc_len[i] = enc;
   } else if (active && s == 0) {
     uint32_t e;

@@ -92,8 +92,12 @@ enum {
 };
 
 hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s);
+// Sorts the batch: by the 16-nibble windows (full = 0; first by their top bits
+// with a fix-up of equal runs, or by all 64 bits when win_all_bits) or by the
+// whole keys (full = number of 16-nibble words), then by trie.  *cur_io (if
+// given) is the ping-pong buffer holding the current order, in and out.
 hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
-                   uint32_t **sv);
+                   uint32_t **sv, bool win_all_bits = false, int *cur_io = nullptr);
 hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s);
 hipError_t nu_unique(const NuArgs &a, const uint32_t *sv, void *temp, hipStream_t s);
 // Descends every unique key and groups them by stop point; sets SC_MERGE when

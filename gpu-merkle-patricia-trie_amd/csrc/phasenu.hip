@@ -22,6 +22,7 @@ namespace gmpt {
 namespace {
 
 constexpr int kB = 256;
+constexpr uint32_t kDirtyStage = 4096;  // staged dirty-list entries per block
 
 __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -338,6 +339,11 @@ __global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint32_t *sv, co
 // --------------------------------------------------- 3. descend + groups --
 
 __global__ __launch_bounds__(256) void k_descend(NuArgs a) {
+  __shared__ ref_t dbuf[kDirtyStage];
+  __shared__ uint32_t dcnt;
+  if (threadIdx.x == 0) dcnt = 0;
+  __syncthreads();
+  DirtyStage ds{dbuf, &dcnt, kDirtyStage};
   uint32_t i = grid_tid();
   uint32_t m = a.scal[SC_M_UNIQ];
   bool active = i < m;
@@ -366,8 +372,7 @@ __global__ __launch_bounds__(256) void k_descend(NuArgs a) {
       raise_error(P.cnt, ERR_BAD_KEY, idx);
     }
   }
-  // the loop is written so every lane reaches the wave-wide dirty marks
-  for (int guard = 0; guard < 2 * kMaxKeyNibbles + 4; guard++) {
+  for (int guard = 0; guard < 2 * kMaxKeyNibbles + 4 && !done; guard++) {
     bool want_mark = false;
     ref_t mark_ref = REF_NULL;
     if (!done) {
@@ -443,13 +448,13 @@ __global__ __launch_bounds__(256) void k_descend(NuArgs a) {
         done = true;
       }
     }
-    mark_dirty(P, mark_ref, want_mark);
-    if (__ballot(!done) == 0) break;
+    if (want_mark) ds.mark(P, mark_ref);
   }
   if (active) {
     if (!done) raise_error(P.cnt, ERR_INTERNAL, 10);
     a.stop[i] = st;
   }
+  dirty_flush(P, ds);
 }
 
 __device__ __forceinline__ bool same_group(const Stop &x, uint32_t tx, const Stop &y, uint32_t ty) {
@@ -1047,16 +1052,61 @@ hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s) {
   return hipGetLastError();
 }
 
+namespace {
+
+// After a stable sort by the windows' top kTopBits bits: every run of equal
+// top bits is sorted by the whole window with a stable insertion sort by the
+// thread at the run's start (runs of uniform keys hold one or two entries).
+// A run longer than kFixMaxRun sets bit 1 of flag_fallback: the host then
+// sorts by all 64 bits.
+constexpr int kTopBits = 24;
+constexpr uint32_t kFixMaxRun = 64;
+
+__global__ __launch_bounds__(256) void k_fix_runs(NuArgs a, uint64_t *sk, uint32_t *sv) {
+  uint32_t j = grid_tid();
+  if (j >= a.n) return;
+  const uint64_t top = sk[j] >> (64 - kTopBits);
+  if (j > 0 && (sk[j - 1] >> (64 - kTopBits)) == top) return;  // not a run start
+  uint32_t L = 1;
+  while (j + L < a.n && L <= kFixMaxRun && (sk[j + L] >> (64 - kTopBits)) == top) L++;
+  if (L == 1) return;
+  if (L > kFixMaxRun) {
+    atomicOr(&a.P.cnt->flag_fallback, 2u);
+    return;
+  }
+  for (uint32_t q = 1; q < L; q++) {
+    const uint64_t k = sk[j + q];
+    const uint32_t v = sv[j + q];
+    uint32_t p = q;
+    while (p > 0 && sk[j + p - 1] > k) {
+      sk[j + p] = sk[j + p - 1];
+      sv[j + p] = sv[j + p - 1];
+      p--;
+    }
+    sk[j + p] = k;
+    sv[j + p] = v;
+  }
+}
+
+}  // namespace
+
 hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
-                   uint32_t **sv) {
+                   uint32_t **sv, bool win_all_bits, int *cur_io) {
   uint64_t *k[2] = {a.sk0, a.sk1};
   uint32_t *v[2] = {a.sv0, a.sv1};
-  int cur = 0, which = 0;
+  int cur = cur_io ? *cur_io : 0, which = 0;
   hipError_t e;
-  if (!full) {
-    e = radix_sort_u64(k[0], v[0], k[1], v[1], a.n, 0, 64, &which, temp, s);
+  if (!full && !win_all_bits) {
+    // top bits, then each run of equal top bits by the whole window
+    e = radix_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, 64 - kTopBits, 64, &which, temp, s);
     if (e != hipSuccess) return e;
-    cur = which;
+    cur ^= which;
+    k_fix_runs<<<nblk(a.n), kB, 0, s>>>(a, k[cur], v[cur]);
+    GMPT_LAUNCH_CHECK();
+  } else if (!full) {
+    e = radix_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, 0, 64, &which, temp, s);
+    if (e != hipSuccess) return e;
+    cur ^= which;
   } else {
     // LSD over (words..., length): length first, then the words from last to first
     int nwords = full;
@@ -1085,6 +1135,7 @@ hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStrea
   }
   *sk = k[cur];
   *sv = v[cur];
+  if (cur_io) *cur_io = cur;
   return hipSuccess;
 }
 

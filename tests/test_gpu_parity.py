@@ -320,3 +320,38 @@ def test_mixed_read_write(gmpt, workloads, oracle, mode):
     assert t.get_values(ops.subset(w)) == [ops.value(i) for i in w]
     with pytest.raises(gmpt.GmptError):
         t.puts_with_read(ops, np.full(ops.n, 2, np.uint8))
+
+
+def test_long_runs_of_equal_leading_window_bits(gmpt, workloads, oracle):
+    """Keys whose sort windows share their leading 24 bits in runs far longer
+    than the sort's fix-up handles (the 64-bit re-sort path), plus keys equal
+    in the whole 16-nibble window (the full-key path)."""
+    rng = np.random.default_rng(33)
+    n = 3000
+    keys = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+    keys[:, 0] = rng.integers(0, 2, n) * 0x10  # first nibble 0 or 1, so the common prefix is empty
+    keys[:, 1:3] = 0x5a                           # the next 5 nibbles are fixed
+    keys[: n // 3, 3:8] = 0x77                    # a third also share nibbles 6..15
+    keys = np.unique(keys, axis=0)
+    flat, lens = workloads.random_values(rng, len(keys), 40, 90)
+    b = workloads.batch_from_fixed(keys, flat, lens)
+    b = b.subset(rng.permutation(b.n))
+    t = gmpt.MPT(device=0)
+    _put(t, b, "2phase")
+    assert t.get_root_hash() == oracle.root_of(b)
+    assert t.stats()["fallback_sorts"] == 1
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_long_values(gmpt, workloads, oracle, mode):
+    """Leaves of many Keccak blocks (values up to 3,000 bytes) and a few
+    single-byte and empty values."""
+    rng = np.random.default_rng(44)
+    b = _rand_batch(workloads, 45, 1500, 900, 3000)
+    small = _rand_batch(workloads, 46, 40, 0, 1)
+    allb = workloads.concat_batches([b, small], trie_ids=[0, 0])
+    allb.trie_ids = None
+    allb = allb.subset(rng.permutation(allb.n))
+    t = gmpt.MPT(device=0)
+    _put(t, allb, mode)
+    assert t.get_root_hash() == oracle.root_of(allb)
