@@ -32,14 +32,33 @@ constexpr int kB = 256;
 __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 inline unsigned nblk(size_t n) { return (unsigned)((n + kB - 1) / kB); }
 
+// Every word another lane may change during the insert (versions, child
+// references, short-node fields, value order stamps) is read with a coherent
+// agent-scope load.  Such loads read memory in the order they return, so the
+// optimistic read protocol only needs each load of a step to have returned
+// before the next step's loads issue: load_order() waits for outstanding
+// loads.  An acquire load would also invalidate the CU's caches at every
+// version check.
 template <typename T>
 __device__ __forceinline__ T ld(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint64_t ver_read(uint64_t *l) {
-  return __hip_atomic_load(l, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void load_order() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
-__device__ __forceinline__ bool ver_check(uint64_t *l, uint64_t v) { return ver_read(l) == v; }
+// version read before a node's contents
+__device__ __forceinline__ uint64_t ver_read(uint64_t *l) {
+  uint64_t v = ld(l);
+  load_order();
+  return v;
+}
+// version re-check after the contents were read
+__device__ __forceinline__ bool ver_check(uint64_t *l, uint64_t v) {
+  load_order();
+  return ld(l) == v;
+}
 __device__ __forceinline__ bool ver_upgrade(uint64_t *l, uint64_t v) {
   uint64_t exp = v;
   return __hip_atomic_compare_exchange_strong(l, &exp, v + 2, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
