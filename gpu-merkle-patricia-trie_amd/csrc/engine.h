@@ -108,7 +108,8 @@ class Forest {
   int sync_counters();
   int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
   int carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid, void **temp);
-  int sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, const uint32_t **sv_out);
+  int sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int *cur);
+  int next_pass(int pass);
   uint8_t *value_space(uint64_t bytes);
 
   int device_ = 0;

@@ -326,7 +326,7 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
     a.tk0 = c.take<uint32_t>(n);
     a.tk1 = c.take<uint32_t>(n);
   }
-  a.scal = c.take<uint32_t>(16);
+  a.scal = d_cnt_->scal;  // copied back with the counters
   a.lcp = c.take<uint8_t>(n);
   a.flags = c.take<uint8_t>(n);
   a.run = c.take<uint32_t>(n);
@@ -359,44 +359,44 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
     set_error("workspace carve overflow");
     return -4;
   }
-  HIP_TRY(hipMemsetAsync(a.scal, 0, 64, stream_));
-  HIP_TRY(hipMemsetAsync(a.scal + SC_W, 0xff, 4, stream_));
+  HIP_TRY(nu_init_scalars(a, stream_));
   return 0;
 }
 
-// Packs the keys, sorts them (by trie, then nibble path) and keeps the last
-// occurrence of each key: u_idx[0 .. scal[SC_M_UNIQ]) in sorted order.
-int Forest::sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, const uint32_t **sv_out) {
-  int r;
-  HIP_TRY(nu_prepare_keys(a, temp, stream_));
+// Sorts the packed keys (by trie, then nibble path) and keeps the last
+// occurrence of each key: u_idx[0 .. scal[SC_M_UNIQ]) in sorted order.  pass 0
+// sorts by the windows' leading bits with a fix-up of equal runs, pass 1 by
+// all 64 window bits, pass 2 by the whole keys.  No host synchronisation:
+// the caller reads flag_fallback at its next one and, if a flag is set, runs
+// next_pass() and repeats the work that followed.
+int Forest::sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int *cur) {
   uint64_t *sk = nullptr;
   uint32_t *sv = nullptr;
-  int cur = 0;
-  HIP_TRY(nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv, false, &cur));
-  HIP_TRY(nu_adjacent(a, sk, sv, 0, stream_));
-  if ((r = sync_counters())) return r;
-  if (cnt_.flag_fallback & 2u) {
-    // a run of equal leading window bits was too long to fix up: sort by
-    // all 64 window bits
-    HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
-    HIP_TRY(nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv, true, &cur));
-    HIP_TRY(nu_adjacent(a, sk, sv, 0, stream_));
+  int nwords = 0;
+  if (pass == 2) {
+    int r;
     if ((r = sync_counters())) return r;
+    uint32_t W = cnt_.scal[SC_W], maxlen = cnt_.scal[SC_MAXLEN];
+    nwords = (int)((maxlen > W ? maxlen - W : 1) + 15) / 16;
   }
-  if (cnt_.flag_fallback) {
-    // Two distinct keys share the 16-nibble window: sort by the full keys.
-    stats_.fallback_sorts++;
-    HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
-    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipStreamSynchronize(stream_));
-    uint32_t W = h_scal_[SC_W], maxlen = h_scal_[SC_MAXLEN];
-    int nwords = (int)((maxlen > W ? maxlen - W : 1) + 15) / 16;
-    HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, false, &cur));
-    HIP_TRY(nu_adjacent(a, sk, sv, 1, stream_));
-  }
+  HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, pass == 1, cur));
+  HIP_TRY(nu_adjacent(a, sk, sv, pass == 2 ? 1 : 0, stream_));
   HIP_TRY(nu_unique(a, sv, temp, stream_));
-  *sv_out = sv;
   return 0;
+}
+
+// After a synchronisation: the sort pass to redo with, or -1 when the last
+// sort stands.  A run of equal leading window bits too long to fix up asks
+// for all 64 window bits; two distinct keys with equal windows ask for a
+// sort by whole keys.
+int Forest::next_pass(int pass) {
+  uint32_t f = cnt_.flag_fallback;
+  if (f == 0 || pass == 2) return -1;
+  if (f & 1u) {
+    stats_.fallback_sorts++;
+    return 2;
+  }
+  return 1;
 }
 
 int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
@@ -427,22 +427,30 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     if (has_tid) {
       while ((1ull << trie_bits) < ntries_) trie_bits++;
     }
-    const uint32_t *sv = nullptr;
-    if ((r = sort_unique(a, trie_bits, temp, &sv))) return r;
-    HIP_TRY(nu_descend_and_group(a, temp, stream_));
-    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
-    if ((r = sync_counters())) return r;
-    if (h_scal_[SC_MERGE]) {
+    HIP_TRY(nu_prepare_keys(a, temp, stream_));
+    int cur = 0;
+    for (int pass = 0; pass >= 0;) {
+      if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) return r;
+      HIP_TRY(nu_descend_and_group(a, temp, stream_));
+      if ((r = sync_counters())) return r;
+      pass = next_pass(pass);
+      if (pass >= 0) {
+        // redo from the sort; marks the discarded descent made are on the
+        // batch's own paths and stay valid
+        HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+        HIP_TRY(hipMemsetAsync(a.scal + SC_MERGE, 0, 4, stream_));
+      }
+    }
+    if (cnt_.scal[SC_MERGE]) {
       // a split group straddles a subtree: make every group contiguous
-      uint32_t m = h_scal_[SC_M_UNIQ];
+      uint32_t m = cnt_.scal[SC_M_UNIQ];
       HIP_TRY(nu_regroup(a, m, m, temp, stream_));
       stats_.regroups++;
     }
     HIP_TRY(nu_emit(a, temp, stream_));
     HIP_TRY(nu_build(a, temp, stream_));
-    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
     if ((r = sync_counters())) return r;
-    key_used_ += *(uint64_t *)(h_scal_ + SC_KTOT);
+    key_used_ += *(const uint64_t *)(cnt_.scal + SC_KTOT);
     key_used_ = align_up(key_used_, 8);
   }
   if (targets) *targets = dirty_.p;
@@ -500,14 +508,13 @@ int Forest::hash(void *targets, int n_targets) {
     HIP_TRY(hc_count(P, encl, (const HcMeta *)hc_meta_.p, n, n_leaf, stream_));
     stats_.leaf_nodes = n_leaf;
   }
-  HIP_TRY(hc_roots(pools(), ntries_, stream_));
-  HIP_TRY(hipMemsetAsync(&d_cnt_->n_dirty, 0, sizeof(uint32_t), stream_));
+  HIP_TRY(hc_roots(pools(), ntries_, stream_));  // also empties the dirty list
   if ((r = sync_counters())) return r;
-  stats_.perms = cnt_.n_perms;
-  stats_.enc_bytes = cnt_.enc_bytes;
-  stats_.leaf_perms = cnt_.leaf_perms;
-  stats_.leaf_bytes = cnt_.leaf_bytes;
-  HIP_TRY(hipMemsetAsync(&d_cnt_->n_perms, 0, 4 * sizeof(unsigned long long), stream_));
+  // the counters are zeroed by the classification, so only a pass that ran sets them
+  stats_.perms = n ? cnt_.n_perms : 0;
+  stats_.enc_bytes = n ? cnt_.enc_bytes : 0;
+  stats_.leaf_perms = n ? cnt_.leaf_perms : 0;
+  stats_.leaf_bytes = n ? cnt_.leaf_bytes : 0;
   stats_.leaf_perm_ms = 0;
   if (n > 0) {
     float ms = 0;
@@ -570,9 +577,7 @@ int Forest::reset_trie(uint32_t trie) {
 int Forest::clear() {
   int r;
   if ((r = set_device())) return r;
-  HIP_TRY(hipMemsetAsync(d_cnt_, 0, sizeof(Counters), stream_));
-  HIP_TRY(hipMemsetAsync(tries_.p, 0, sizeof(TrieHdr) * ntries_, stream_));
-  HIP_TRY(hipStreamSynchronize(stream_));
+  HIP_TRY(hc_clear(d_cnt_, (TrieHdr *)tries_.p, ntries_, stream_));
   memset(&cnt_, 0, sizeof(cnt_));
   key_used_ = 0;
   // keep the first value chunk, release the others
@@ -820,11 +825,15 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
       while ((1ull << trie_bits) < ntries_) trie_bits++;
     }
     HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
-    const uint32_t *sv = nullptr;
-    if ((r = sort_unique(a, trie_bits, temp, &sv))) return r;
-    HIP_TRY(hipMemcpyAsync(h_scal_, a.scal, 64, hipMemcpyDeviceToHost, stream_));
-    if ((r = sync_counters())) return r;
-    uint64_t ktot = *(uint64_t *)(h_scal_ + SC_KTOT);
+    HIP_TRY(nu_prepare_keys(a, temp, stream_));
+    int cur = 0;
+    for (int pass = 0; pass >= 0;) {
+      if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) return r;
+      if ((r = sync_counters())) return r;
+      pass = next_pass(pass);
+      if (pass >= 0) HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+    }
+    uint64_t ktot = *(const uint64_t *)(cnt_.scal + SC_KTOT);
     HIP_TRY(lock_insert(a, key_used_ + ktot, stream_));
     if ((r = sync_counters())) return r;
     key_used_ = align_up(key_used_ + ktot, 8);

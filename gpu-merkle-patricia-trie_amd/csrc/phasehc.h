@@ -37,7 +37,9 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
 // first n_leaf entries (the leaf level) to the leaf counters.
 hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
                     hipStream_t s);
-// Copies every trie's root hash into its header.
+// Zeroes the forest's counters and every trie header (empty tries).
+hipError_t hc_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries, hipStream_t s);
+// Copies every trie's root hash into its header and empties the dirty list.
 hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s);
 // Node export: encoding lengths, then encodings at offs and 32-byte hash slots.
 hipError_t hc_enc_len(const Pools &P, const ref_t *refs, uint32_t m, uint64_t *len, hipStream_t s);

@@ -806,6 +806,7 @@ __global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *li
 
 __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries) {
   uint32_t t = grid_tid();
+  if (t == 0) P.cnt->n_dirty = 0;  // every listed node has been hashed
   if (t >= ntries) return;
   TrieHdr &T = P.tries[t];
   ref_t r = T.root;
@@ -925,6 +926,9 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s) {
   k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins);
   GMPT_LAUNCH_CHECK();
+  // the pass's work counters (permutations, bytes, leaf permutations, leaf bytes)
+  k_hc_zero<<<1, kB, 0, s>>>((uint32_t *)&P.cnt->n_perms, 8);
+  GMPT_LAUNCH_CHECK();
   if (n == 0) {
     k_hc_zero<<<1, kB, 0, s>>>(&meta->nvalid, 4);  // nvalid, pad, scratch_bytes
     return hipGetLastError();
@@ -972,6 +976,21 @@ hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta,
   if (n_leaf == 0) return hipSuccess;
   k_hc_count<<<(unsigned)std::min<size_t>(nblk(n_leaf), 512), kB, 0, s>>>(enc_len, &meta->nvalid, n_leaf,
                                                                           &P.cnt->leaf_perms, &P.cnt->leaf_bytes);
+  return hipGetLastError();
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries) {
+  uint32_t t = grid_tid();
+  if (t < sizeof(Counters) / 4) ((uint32_t *)cnt)[t] = 0;
+  for (uint32_t q = t; q < ntries * (uint32_t)(sizeof(TrieHdr) / 4); q += gridDim.x * blockDim.x)
+    ((uint32_t *)tries)[q] = 0;
+}
+}  // namespace
+
+hipError_t hc_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries, hipStream_t s) {
+  size_t words = std::max<size_t>(sizeof(Counters) / 4, (size_t)ntries * sizeof(TrieHdr) / 4);
+  k_clear<<<(unsigned)std::min<size_t>(nblk(words), 1024), kB, 0, s>>>(cnt, tries, ntries);
   return hipGetLastError();
 }
 

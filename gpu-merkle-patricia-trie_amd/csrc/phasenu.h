@@ -91,6 +91,8 @@ enum {
   SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11, SC_COUNT = 16
 };
 
+// Zeroes the call's device scalars (SC_W starts at all ones for its minimum).
+hipError_t nu_init_scalars(const NuArgs &a, hipStream_t s);
 hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s);
 // Sorts the batch: by the 16-nibble windows (full = 0; first by their top bits
 // with a fix-up of equal runs, or by all 64 bits when win_all_bits) or by the

@@ -1040,6 +1040,18 @@ __global__ void k_build_end(NuArgs a, uint32_t upd_off) {
 
 // ------------------------------------------------------------- host side --
 
+namespace {
+__global__ void k_init_scalars(uint32_t *scal) {
+  uint32_t t = threadIdx.x;
+  if (t < SC_COUNT) scal[t] = t == SC_W ? 0xffffffffu : 0u;
+}
+}  // namespace
+
+hipError_t nu_init_scalars(const NuArgs &a, hipStream_t s) {
+  k_init_scalars<<<1, 64, 0, s>>>(a.scal);
+  return hipGetLastError();
+}
+
 hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s) {
   // per-block partial results live in ecount / grp, unused until grouping
   k_keys_scan<<<nblk(a.n), kB, 0, s>>>(a, a.ecount, a.grp);
