@@ -427,29 +427,36 @@ constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in r
 // Work counters of a hash pass: one reduction over the encoding lengths
 // after the last level (per-wave atomics on one counter serialise when waves
 // are many).
-__global__ __launch_bounds__(256) void k_hc_count(const uint32_t *__restrict__ enc_len,
-                                                  const uint32_t *__restrict__ nvalid, uint32_t limit,
-                                                  unsigned long long *perms_out, unsigned long long *bytes_out) {
-  __shared__ unsigned long long sp[4], sb[4];
-  const uint32_t n = min(*nvalid, limit);
-  unsigned long long perms = 0, bytes = 0;
+__global__ __launch_bounds__(256) void k_hc_count(Counters *cnt, const uint32_t *__restrict__ enc_len,
+                                                  const uint32_t *__restrict__ nvalid, uint32_t n_leaf) {
+  __shared__ unsigned long long sh[4][4];
+  const uint32_t n = *nvalid;
+  // perms, bytes over all entries; perms, bytes over the leaf entries
+  unsigned long long v[4] = {0, 0, 0, 0};
   for (uint32_t i = grid_tid(); i < n; i += gridDim.x * blockDim.x) {
-    uint32_t e = enc_len[i];
-    bytes += e;
-    perms += e >= 32 ? keccak_blocks(e) : 0u;
+    const uint32_t e = enc_len[i];
+    const uint32_t p = e >= 32 ? keccak_blocks(e) : 0u;
+    v[0] += p;
+    v[1] += e;
+    if (i < n_leaf) {
+      v[2] += p;
+      v[3] += e;
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    perms += (unsigned long long)__shfl_xor((long long)perms, off);
-    bytes += (unsigned long long)__shfl_xor((long long)bytes, off);
-  }
-  if (__lane_id() == 0) {
-    sp[threadIdx.x >> 6] = perms;
-    sb[threadIdx.x >> 6] = bytes;
-  }
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    for (int off = 32; off > 0; off >>= 1) v[q] += (unsigned long long)__shfl_xor((long long)v[q], off);
+  if (__lane_id() == 0)
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[threadIdx.x >> 6][q] = v[q];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(perms_out, sp[0] + sp[1] + sp[2] + sp[3]);
-    atomicAdd(bytes_out, sb[0] + sb[1] + sb[2] + sb[3]);
+  if (threadIdx.x < 4) {
+    unsigned long long t = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    unsigned long long *dst = threadIdx.x == 0 ? &cnt->n_perms
+                              : threadIdx.x == 1 ? &cnt->enc_bytes
+                              : threadIdx.x == 2 ? &cnt->leaf_perms
+                                                 : &cnt->leaf_bytes;
+    if (t) atomicAdd(dst, t);
   }
 }
 
@@ -520,6 +527,7 @@ __device__ __forceinline__ uint64_t value_word(uint64_t va, uint32_t vsz, int64_
 constexpr uint32_t kLeafGroup = 8;                 // lanes per leaf
 constexpr uint32_t kLeafNodes = 256 / kLeafGroup;  // leaves per block
 constexpr uint32_t kLeafHeadWords = 6;             // list header + compact key + value header <= 44 bytes
+constexpr uint32_t kLeafChunk = 5;                 // message words per lane per load/store round
 
 // Leaf level: 8 lanes per leaf short node [compact key, value] whose key
 // remainder has <= 64 nibbles.  Lane 0 builds the message head (list header,
@@ -581,13 +589,24 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
     const uint32_t nb = keccak_blocks(enc);
     const uint32_t vstart = enc - vsz;  // message offset of the value bytes (unless raw, in the head)
     uint64_t *out = scratch + slot[i] / 8;
-    for (uint32_t w = s; w < 17 * nb; w += kLeafGroup) {
-      const uint32_t x0 = 8 * w;
-      uint64_t word = w < kLeafHeadWords ? head[g][w] : 0ull;
-      if (!v_raw && x0 + 8 > vstart && x0 < enc) word |= value_word(va, vsz, (int64_t)x0 - (int64_t)vstart);
-      if (enc >= x0 && enc < x0 + 8) word |= 0x01ull << (8 * (enc - x0));  // Keccak padding
-      if (w + 1 == 17 * nb) word |= 0x80ull << 56;
-      out[w] = word;
+    // this lane's words in chunks of kLeafChunk: every load of a chunk is
+    // issued before its stores (a load after a store waits for the store)
+    const uint32_t nw = 17 * nb;
+    for (uint32_t w0 = s; w0 < nw; w0 += kLeafGroup * kLeafChunk) {
+      uint64_t word[kLeafChunk];
+#pragma unroll
+      for (uint32_t q = 0; q < kLeafChunk; q++) {
+        const uint32_t w = w0 + q * kLeafGroup, x0 = 8 * w;
+        uint64_t v = w < kLeafHeadWords ? head[g][w] : 0ull;
+        if (w < nw && !v_raw && x0 + 8 > vstart && x0 < enc)
+          v |= value_word(va, vsz, (int64_t)x0 - (int64_t)vstart);
+        if (enc >= x0 && enc < x0 + 8) v |= 0x01ull << (8 * (enc - x0));  // Keccak padding
+        if (w + 1 == nw) v |= 0x80ull << 56;
+        word[q] = v;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kLeafChunk; q++)
+        if (w0 + q * kLeafGroup < nw) out[w0 + q * kLeafGroup] = word[q];
     }
     if (s == 0) enc_len[i] = enc;
   } else if (active && s == 0) {
@@ -601,23 +620,19 @@ constexpr uint32_t kInnerGroup = 16;              // lanes per node (one per nib
 constexpr uint32_t kInnerNodes = 256 / kInnerGroup;
 constexpr uint32_t kInnerWords = 80;              // 640 LDS bytes per node >= 4 Keccak blocks
 
-// Inner levels: 16 lanes per node.  For a branch whose 16 nibble slots hold
+// Encodes inner node r with 16 lanes (s = 0..15 of a 16-lane-aligned segment,
+// part = this thread is one of them).  For a branch whose 16 nibble slots hold
 // only empty slots and hashed (32-byte) children, and whose slot 16 is empty
 // (every branch of fixed-length keys), lane s loads child s's digest, the
-// group prefix-sums the item lengths, each lane writes its item into an LDS
-// copy of the padded message, and the group streams the message words to the
-// scratch slot.  Other nodes (extension short nodes, branches with embedded
-// children or a value) take the generic encoder on the group's first lane.
-__global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *__restrict__ list,
-                                                         const uint64_t *__restrict__ slot, uint32_t n,
-                                                         uint64_t *__restrict__ scratch,
-                                                         uint32_t *__restrict__ enc_len) {
-  __shared__ uint64_t lds[kInnerNodes][kInnerWords];
-  const uint32_t g = threadIdx.x / kInnerGroup, s = threadIdx.x % kInnerGroup;
-  const uint32_t i = blockIdx.x * kInnerNodes + g;
-  const bool active = i < n;
-  const ref_t r = active ? list[i] : REF_NULL;
-  const bool is_branch = ref_type(r) == T_BRANCH;
+// segment prefix-sums the item lengths, each lane writes its item into the
+// node's LDS copy of the padded message (lds, kInnerWords words), and the
+// segment streams the message words to out.  Other nodes (extension short
+// nodes, branches with embedded children or a value) take the generic encoder
+// on lane 0.  Every thread of the block calls this (it synchronises the
+// block); lane 0 returns the encoding length.
+__device__ __forceinline__ uint32_t encode_inner_node(const Pools &P, ref_t r, bool active, bool part, uint32_t s,
+                                                      uint64_t *lds, uint64_t *out) {
+  const bool is_branch = part && active && ref_type(r) == T_BRANCH;
   ref_t c = REF_NULL, c16 = REF_NULL;
   if (is_branch) {
     const Branch &B = P.branch[ref_idx(r)];
@@ -637,9 +652,11 @@ __global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *_
   const bool ok = c == REF_NULL || (node_child && hs == 32);
   const uint64_t bad = __ballot(!ok);
   const uint32_t gbase = __lane_id() & ~(kInnerGroup - 1);
-  const bool fast = active && is_branch && c16 == REF_NULL && ((bad >> gbase) & 0xffffull) == 0;
+  const bool fast = is_branch && c16 == REF_NULL && ((bad >> gbase) & 0xffffull) == 0;
+  if (part) {
 #pragma unroll
-  for (uint32_t q = 0; q < kInnerWords / kInnerGroup; q++) lds[g][s + kInnerGroup * q] = 0;
+    for (uint32_t q = 0; q < kInnerWords / kInnerGroup; q++) lds[s + kInnerGroup * q] = 0;
+  }
   __syncthreads();
   uint32_t enc = 0, nb = 0;
   if (fast) {
@@ -656,7 +673,7 @@ __global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *_
     const uint64_t hdr = rlp_hdr(payload, 0xC0, 0xF7, &hn);
     enc = hn + payload;
     nb = keccak_blocks(enc);
-    uint8_t *b = (uint8_t *)lds[g];
+    uint8_t *b = (uint8_t *)lds;
     const uint32_t pos = hn + incl - len;
     if (c == REF_NULL) {
       b[pos] = 0x80;
@@ -674,13 +691,25 @@ __global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *_
   }
   __syncthreads();
   if (fast) {
-    uint64_t *out = scratch + slot[i] / 8;
-    for (uint32_t w = s; w < 17 * nb; w += kInnerGroup) out[w] = lds[g][w];
-    if (s == 0) enc_len[i] = enc;
-  } else if (active && s == 0) {
-    encode_generic(P, r, scratch + slot[i] / 8, &enc);
-    enc_len[i] = enc;
+    for (uint32_t w = s; w < 17 * nb; w += kInnerGroup) out[w] = lds[w];
+  } else if (part && active && s == 0) {
+    encode_generic(P, r, out, &enc);
   }
+  return enc;
+}
+
+// Inner levels: 16 lanes per node.
+__global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *__restrict__ list,
+                                                         const uint64_t *__restrict__ slot, uint32_t n,
+                                                         uint64_t *__restrict__ scratch,
+                                                         uint32_t *__restrict__ enc_len) {
+  __shared__ uint64_t lds[kInnerNodes][kInnerWords];
+  const uint32_t g = threadIdx.x / kInnerGroup, s = threadIdx.x % kInnerGroup;
+  const uint32_t i = blockIdx.x * kInnerNodes + g;
+  const bool active = i < n;
+  const ref_t r = active ? list[i] : REF_NULL;
+  const uint32_t enc = encode_inner_node(P, r, active, true, s, lds[g], active ? scratch + slot[i] / 8 : scratch);
+  if (active && s == 0) enc_len[i] = enc;
 }
 
 // ----------------------------------------------------------------- permute --
@@ -800,6 +829,32 @@ __global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *li
   sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, gl, c, l, h);
   if (nb == 0) return;
   ref_t r = list[i];
+  if (gl < 4) ((uint64_t *)node_hash(P, r))[gl] = ((uint64_t)h << 32) | l;
+  if (gl == 0) node_done(P, r, 32);
+}
+
+// Small inner levels in one launch: one node per 32-lane group; lanes 0..15
+// encode it (encode_inner_node), then the whole group runs the sponge with the
+// state spread over 25 lanes.  The message passes through the node's scratch
+// slot, which the block barrier makes visible to the group (it stays in cache).
+__global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restrict__ list,
+                                                  const uint64_t *__restrict__ slot, uint32_t n,
+                                                  uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
+  __shared__ uint64_t lds[256 / 32][kInnerWords];
+  const uint32_t gl = __lane_id() & 31, g = threadIdx.x / 32;
+  const uint32_t i = blockIdx.x * (256 / 32) + g;
+  const bool active = i < n;
+  const ref_t r = active ? list[i] : REF_NULL;
+  uint64_t *out = active ? scratch + slot[i] / 8 : scratch;
+  uint32_t enc = encode_inner_node(P, r, active, gl < kInnerGroup, gl & (kInnerGroup - 1), lds[g], out);
+  enc = (uint32_t)__shfl((int)enc, 0, 32);
+  if (active && gl == 0) enc_len[i] = enc;
+  __syncthreads();  // the group's message words are visible to all its lanes
+  const uint32_t nb = active && enc >= 32 ? keccak_blocks(enc) : 0u;  // uniform over the group
+  SpreadLane c(gl);
+  uint32_t l, h;
+  sponge_spread(out, nb, gl, c, l, h);
+  if (nb == 0) return;
   if (gl < 4) ((uint64_t *)node_hash(P, r))[gl] = ((uint64_t)h << 32) | l;
   if (gl == 0) node_done(P, r, 32);
 }
@@ -949,6 +1004,16 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
                          hipEvent_t perm_end) {
   if (n == 0) return hipSuccess;
+  if (!leaves && n <= spread_max) {  // small inner level: encode and hash in one launch
+    if (perm_begin) {
+      hipError_t e = hipEventRecord(perm_begin, s);
+      if (e != hipSuccess) return e;
+    }
+    k_hc_small<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+    GMPT_LAUNCH_CHECK();
+    if (perm_end) return hipEventRecord(perm_end, s);
+    return hipSuccess;
+  }
   if (leaves)
     k_hc_encode_leaf<<<(n + kLeafNodes - 1) / kLeafNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
   else
@@ -970,12 +1035,7 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
 hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
                     hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n), 512), kB, 0, s>>>(enc_len, &meta->nvalid, n, &P.cnt->n_perms,
-                                                                     &P.cnt->enc_bytes);
-  GMPT_LAUNCH_CHECK();
-  if (n_leaf == 0) return hipSuccess;
-  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n_leaf), 512), kB, 0, s>>>(enc_len, &meta->nvalid, n_leaf,
-                                                                          &P.cnt->leaf_perms, &P.cnt->leaf_bytes);
+  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n), 1024), kB, 0, s>>>(P.cnt, enc_len, &meta->nvalid, n_leaf);
   return hipGetLastError();
 }
 
