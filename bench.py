@@ -40,11 +40,11 @@ INSTR_PER_PERM = 4320
 PERM_PEAK_G = VALU_PEAK_TOPS * 1e12 / INSTR_PER_PERM / 1e9
 
 
-# HBM bytes per launch of the leaf-level k_hc_perm_lane on the default 1M-key
+# HBM bytes per launch of the leaf-level kernel on the default 1M-key
 # workload: rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per dispatch,
 # the factors being what the same counters read for the 1 GiB k_copy16
-# calibration copy (--config hbm-copy).
-LEAF_PERM_TRAFFIC_BYTES = 307_465_891
+# calibration copy (--config hbm-copy).  Keyed by "leaves fused".
+LEAF_TRAFFIC_BYTES = {False: 307_465_891, True: None}
 
 
 def _import_package():
@@ -88,6 +88,8 @@ def bench_phase(pkg, args):
     gen_s = time.time() - t0
     db = pkg.DeviceBatch(b, adopt_values=True)
     t = pkg.MPT(device=0)
+    t.set_option("hc_fuse_leaves", 0 if args.no_fuse_leaves else 1)
+    fused = not args.no_fuse_leaves
     spread_max, tune_ms = t.autotune(db)  # picks the Keccak layout threshold on this batch
     roots = []
 
@@ -117,7 +119,8 @@ def bench_phase(pkg, args):
     value = b.n * args.steps / wall
 
     # roofline of the dominant kernel: the leaf level's Keccak permutations
-    # (k_hc_perm_lane), timed with HIP events on the forest's stream
+    # (k_hc_leaf_fused, which also encodes the leaves, or k_hc_perm_lane),
+    # timed with HIP events on the forest's stream
     perms = st["perms"]
     hms = sum(hash_ms) / len(hash_ms)
     leaf_ms = sum(leaf_ms_l) / len(leaf_ms_l)
@@ -139,8 +142,10 @@ def bench_phase(pkg, args):
                    "keys": b.n, "value_bytes": b.value_bytes},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PERM_PEAK_G, "unit": "Gperm/s",
                      "frac": achieved / PERM_PEAK_G,
-                     "traffic": LEAF_PERM_TRAFFIC_BYTES if b.n == 1_000_000 else None,
-                     "kernel": "k_hc_perm_lane, leaf level (Keccak-f[1600], one state per lane)",
+                     "traffic": LEAF_TRAFFIC_BYTES[fused] if b.n == 1_000_000 else None,
+                     "kernel": ("k_hc_leaf_fused, leaf level (RLP encoding through LDS + Keccak-f[1600], "
+                                "one state per lane)") if fused else
+                               "k_hc_perm_lane, leaf level (Keccak-f[1600], one state per lane)",
                      "per_launch": {"permutations": st["leaf_perms"], "nodes": st["leaf_nodes"],
                                     "ms": leaf_ms, "message_bytes": st["leaf_bytes"]},
                      "peak_basis": f"{VALU_PEAK_TOPS:.2f} T 32-bit VALU ops/s / {INSTR_PER_PERM} ops per permutation"},
@@ -533,6 +538,8 @@ def main():
     ap.add_argument("--backlog", type=int, default=400, help="served: requests of the saturation run")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fuse-leaves", action="store_true",
+                    help="phase: leaf level as encode kernel + HBM scratch + permutation kernel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",

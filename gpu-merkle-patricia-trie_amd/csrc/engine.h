@@ -127,9 +127,10 @@ class Forest {
   size_t vchunk_used_ = 0;
   DevBuf st_keys_, st_kidx_, st_vidx_, st_hps_, st_tid_;
   DevBuf ws_, ws2_, rw_;
-  DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_;
+  DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_, hc_lmeta_, hc_lmeta_tmp_;
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
+  bool hc_fuse_leaves_ = true;   // leaves of <= 2 Keccak blocks: one encode+hash kernel
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   OpStats stats_;

@@ -90,7 +90,8 @@ Forest::~Forest() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
-                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_, &rw_};
+                    &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_,
+                    &hc_lmeta_, &hc_lmeta_tmp_, &rw_};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (DevBuf &b : vchunks_)
@@ -482,8 +483,15 @@ int Forest::hash(void *targets, int n_targets) {
     ref_t *sorted = (ref_t *)hc_sorted_.p;
     uint64_t *slot = (uint64_t *)hc_slot_.p;
     uint32_t *encl = (uint32_t *)hc_enc_.p;
+    uint64_t *lmeta = nullptr, *lmeta_tmp = nullptr;
+    if (hc_fuse_leaves_) {
+      if ((r = ensure_ws(hc_lmeta_, 16 * (size_t)n))) return r;
+      if ((r = ensure_ws(hc_lmeta_tmp_, 16 * (size_t)n))) return r;
+      lmeta = (uint64_t *)hc_lmeta_.p;
+      lmeta_tmp = (uint64_t *)hc_lmeta_tmp_.p;
+    }
     HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
-                        encl, ws2_.p, stream_));
+                        encl, ws2_.p, stream_, lmeta_tmp, lmeta));
     HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     const HcMeta &M = *(const HcMeta *)h_meta_;
@@ -496,8 +504,10 @@ int Forest::hash(void *targets, int n_targets) {
       for (uint32_t c = 0; c < kHcClasses; c++) cnt += M.hist[lv * kHcClasses + c];
       if (cnt) {
         const bool leaves = lv == 0;
+        // leaves of one or two Keccak blocks (classes 0 and 1, binned first)
+        const uint32_t n_fused = leaves && lmeta ? M.hist[0] + M.hist[1] : 0u;
         HIP_TRY(hc_hash_range(P, sorted + start, slot + start, cnt, scratch, encl + start, leaves, hc_spread_max_,
-                              stream_, leaves ? ev2_ : nullptr, leaves ? ev3_ : nullptr));
+                              stream_, leaves ? ev2_ : nullptr, leaves ? ev3_ : nullptr, n_fused, lmeta));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
         if (leaves) n_leaf = cnt;
@@ -596,6 +606,14 @@ int Forest::set_option(const std::string &name, int64_t value) {
     hc_spread_max_ = (uint32_t)value;
     return 0;
   }
+  if (name == "hc_fuse_leaves") {
+    if (value != 0 && value != 1) {
+      set_error("hc_fuse_leaves must be 0 or 1");
+      return -3;
+    }
+    hc_fuse_leaves_ = value != 0;
+    return 0;
+  }
   set_error("unknown option: " + name);
   return -3;
 }
@@ -603,6 +621,10 @@ int Forest::set_option(const std::string &name, int64_t value) {
 int Forest::get_option(const std::string &name, int64_t *value) const {
   if (name == "hc_spread_max") {
     *value = hc_spread_max_;
+    return 0;
+  }
+  if (name == "hc_fuse_leaves") {
+    *value = hc_fuse_leaves_ ? 1 : 0;
     return 0;
   }
   set_error("unknown option: " + name);
@@ -617,6 +639,7 @@ int Forest::autotune(const BatchDesc &sample, int mode, int64_t *chosen, double 
   uint32_t arg = hc_spread_max_;
   for (uint32_t c : kCand) {
     t.hc_spread_max_ = c;
+    t.hc_fuse_leaves_ = hc_fuse_leaves_;
     double ms = 0;
     for (int rep = 0; rep < 2; rep++) {  // the second run is timed (pools warm)
       int r;

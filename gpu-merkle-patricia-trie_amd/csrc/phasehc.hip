@@ -43,6 +43,7 @@ constexpr uint32_t kHcTile = 2048;  // dirty nodes per binning block
 __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 inline unsigned nblk(size_t n) { return (unsigned)((n + kB - 1) / kB); }
 
+
 __device__ __forceinline__ uint32_t be_bytes(uint64_t v) {
   uint32_t n = 1;
   while (v >>= 8) n++;
@@ -126,6 +127,24 @@ __device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r, uint32_t *bl
   return level * kHcClasses + cls;
 }
 
+// Leaf metadata for the LDS leaf kernel, recorded by the classification in
+// sorted order so that kernel's loads do not chase list -> short -> value:
+//   m0 = byte offset of the key's first packed byte in the key arena (40 bits)
+//        | key_start & 1 << 40 | key nibbles without terminator << 41 (8 bits)
+//        | term << 49 | value size << 50 (14 bits)
+//   m1 = device address of the value bytes
+// Only leaves of at most two Keccak blocks get one (value size < 272).
+__device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) {
+  const Short &S = P.shrt[ref_idx(r)];
+  const Value &V = P.value[ref_idx(S.child)];
+  const uint64_t cnt = (uint32_t)S.key_len - S.term;
+  m[0] = (S.key_off + S.key_start / 2) | ((uint64_t)(S.key_start & 1) << 40) | (cnt << 41) |
+         ((uint64_t)S.term << 49) | ((uint64_t)min(V.size, 16383u) << 50);
+  m[1] = V.addr;
+}
+constexpr uint32_t kLdsLeafBins = 2;  // level 0, classes 0 and 1: leaves of one or two blocks
+constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
+
 __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
   uint32_t i = grid_tid();
   if (i < n) p[i] = 0;
@@ -139,7 +158,8 @@ __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
 constexpr uint32_t kPackedNbMax = (1u << 20) - 1;
 
 __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restrict__ list, uint32_t n,
-                                                 uint32_t *__restrict__ hist, uint32_t *__restrict__ packed) {
+                                                 uint32_t *__restrict__ hist, uint32_t *__restrict__ packed,
+                                                 uint64_t *__restrict__ lmeta_tmp) {
   __shared__ uint32_t h[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
@@ -151,6 +171,7 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
       uint32_t bin = bin_of(P, r, &nb);
       atomicAdd(&h[bin], 1u);
       pk = bin | (min(nb, kPackedNbMax) << 12);
+      if (lmeta_tmp && bin < kLdsLeafBins) leaf_meta(P, r, lmeta_tmp + 2 * (size_t)i);
     }
     packed[i] = pk;
   }
@@ -164,7 +185,9 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
 __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *__restrict__ list,
                                                     const uint32_t *__restrict__ packed, uint32_t n,
                                                     uint32_t *__restrict__ cursor, ref_t *__restrict__ out,
-                                                    uint64_t *__restrict__ slot) {
+                                                    uint64_t *__restrict__ slot,
+                                                    const uint64_t *__restrict__ lmeta_tmp,
+                                                    uint64_t *__restrict__ lmeta) {
   __shared__ uint32_t cnt[kHcBins];
   __shared__ uint32_t base[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) cnt[b] = 0;
@@ -190,6 +213,13 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *__rest
       uint32_t nb = pk[q] >> 12;
       if (nb == kPackedNbMax) bin_of(P, r, &nb);  // saturated: recompute exactly
       out[pos] = r;
+      if (lmeta && (pk[q] & 0xfff) < kLdsLeafBins) {
+        const uint64_t m0 = lmeta_tmp[2 * (size_t)i];
+        lmeta[2 * (size_t)pos] = m0;
+        lmeta[2 * (size_t)pos + 1] = lmeta_tmp[2 * (size_t)i + 1];
+        // the word builder needs no scratch slot; the generic encoder does
+        if (((m0 >> 41) & 255u) <= kLeafFastNibbles) nb = 0;
+      }
       slot[pos] = 136ull * nb;
     }
 }
@@ -422,7 +452,6 @@ __device__ __forceinline__ uint64_t funnel8(uint64_t a, uint64_t b, uint32_t off
 }
 __device__ __forceinline__ uint64_t low_bytes(uint64_t v, uint32_t n) { return n >= 8 ? v : v & ((1ull << (8 * n)) - 1); }
 
-constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
 
 // Work counters of a hash pass: one reduction over the encoding lengths
 // after the last level (per-wave atomics on one counter serialise when waves
@@ -530,12 +559,15 @@ constexpr uint32_t kLeafHeadWords = 6;             // list header + compact key 
 constexpr uint32_t kLeafChunk = 5;                 // message words per lane per load/store round
 
 // Leaf level: 8 lanes per leaf short node [compact key, value] whose key
-// remainder has <= 64 nibbles.  Lane 0 builds the message head (list header,
-// compact key, value header) in LDS; then lane s computes message words s,
-// s + 8, ... of the padded message -- head bytes, value bytes (two aligned
-// loads per word, consecutive across the group) and padding -- and stores
-// them, so the group's loads and stores are contiguous and no store waits on
-// a later load.  Other leaves take the generic encoder on lane 0.
+// remainder has <= 64 nibbles.  Lane s computes message words s, s + 8, ...
+// of the padded message -- head bytes, value bytes (two aligned loads per
+// word, consecutive across the group) and padding -- and stores them, so the
+// group's loads and stores are contiguous and no store waits on a later load.
+// Lane 0 builds the head (list header, compact key, value header) in LDS.
+// The dependent loads are kept short: the key words are loaded with the value
+// node, the value's first byte only for a 1-byte value (which may be raw), and
+// the first chunk of value words before the block waits for the heads.
+// Other leaves take the generic encoder on lane 0.
 __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__restrict__ list,
                                                         const uint64_t *__restrict__ slot, uint32_t n,
                                                         uint64_t *__restrict__ scratch,
@@ -544,26 +576,37 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
   const uint32_t g = threadIdx.x / kLeafGroup, s = threadIdx.x % kLeafGroup;
   const uint32_t i = blockIdx.x * kLeafNodes + g;
   const bool active = i < n;
-  if (s < kLeafHeadWords) head[g][s] = 0;
   const ref_t r = active ? list[i] : REF_NULL;
   uint64_t key_off = 0, va = 0;
   uint32_t ks = 0, cnt = 0, term = 0, vsz = 0;
-  bool fast = false;
+  ref_t ch = REF_NULL;
   if (active) {
     const Short &S = P.shrt[ref_idx(r)];
-    const ref_t ch = S.child;
+    ch = S.child;
     key_off = S.key_off;
     ks = S.key_start;
     cnt = (uint32_t)S.key_len - S.term;
     term = S.term;
-    if (ref_type(ch) == T_VALUE) {
-      const Value &V = P.value[ref_idx(ch)];
-      va = V.addr;
-      vsz = V.size;
-      fast = cnt <= kLeafFastNibbles;
-    }
   }
-  const uint32_t vb0 = fast ? (uint32_t)(value_word(va, vsz, 0) & 0xff) : 0u;
+  bool fast = ref_type(ch) == T_VALUE && cnt <= kLeafFastNibbles;
+  // packed key bytes from byte ks / 2: up to 33 bytes, 6 aligned words
+  uint64_t kw[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t koff = 0;
+  if (fast && s == 0) {
+    const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
+    koff = (uint32_t)(ka & 7);
+    const uint64_t *kw_p = (const uint64_t *)(ka - koff);
+    const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
+    const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;
+#pragma unroll
+    for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
+  }
+  if (fast) {
+    const Value &V = P.value[ref_idx(ch)];
+    va = V.addr;
+    vsz = V.size;
+  }
+  const uint32_t vb0 = fast && vsz == 1 ? (uint32_t)(value_word(va, 1, 0) & 0xff) : 0u;
   const uint32_t kc = cnt / 2 + 1;
   const uint32_t key_item = kc == 1 ? 1u : 1u + kc;
   const bool v_raw = vsz == 1 && vb0 <= 0x7f;
@@ -571,42 +614,40 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
   const uint32_t payload = key_item + (v_raw ? 1u : vh + vsz);
   const uint32_t enc = rlp_hdr_len(payload) + payload;
   fast = fast && enc >= 32;
-  __syncthreads();
-  if (fast && s == 0) {
-    // packed key bytes from byte ks / 2: up to 33 bytes, 6 aligned words
-    const uintptr_t ka = (uintptr_t)(P.keys + key_off + ks / 2);
-    const uint32_t koff = (uint32_t)(ka & 7);
-    const uint64_t *kw_p = (const uint64_t *)(ka - koff);
-    const uint32_t kbytes = cnt ? (ks + cnt - 1) / 2 - ks / 2 + 1 : 1;
-    const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;
-    uint64_t kw[6];
+  const uint32_t nw = fast ? 17 * keccak_blocks(enc) : 0u;
+  const uint32_t vstart = enc - vsz;  // message offset of the value bytes (unless raw, in the head)
+  // this lane's words in chunks of kLeafChunk: every load of a chunk is
+  // issued before its stores (a load after a store waits for the store)
+  uint64_t word[kLeafChunk];
+  auto chunk = [&](uint32_t w0) {
 #pragma unroll
-    for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? kw_p[q] : 0ull;
+    for (uint32_t q = 0; q < kLeafChunk; q++) {
+      const uint32_t w = w0 + q * kLeafGroup, x0 = 8 * w;
+      uint64_t v = 0;
+      if (w < nw && !v_raw && x0 + 8 > vstart && x0 < enc) v = value_word(va, vsz, (int64_t)x0 - (int64_t)vstart);
+      if (enc >= x0 && enc < x0 + 8) v |= 0x01ull << (8 * (enc - x0));  // Keccak padding
+      if (w + 1 == nw) v |= 0x80ull << 56;
+      word[q] = v;
+    }
+  };
+  chunk(s);
+  if (fast && s == 0) {
+#pragma unroll
+    for (uint32_t q = 0; q < kLeafHeadWords; q++) head[g][q] = 0;
     leaf_head(head[g], payload, kw, koff, ks, cnt, term, vsz, v_raw, vb0);
   }
   __syncthreads();
   if (fast) {
-    const uint32_t nb = keccak_blocks(enc);
-    const uint32_t vstart = enc - vsz;  // message offset of the value bytes (unless raw, in the head)
+    static_assert(kLeafHeadWords <= kLeafGroup, "head words are in every lane's first word");
     uint64_t *out = scratch + slot[i] / 8;
-    // this lane's words in chunks of kLeafChunk: every load of a chunk is
-    // issued before its stores (a load after a store waits for the store)
-    const uint32_t nw = 17 * nb;
-    for (uint32_t w0 = s; w0 < nw; w0 += kLeafGroup * kLeafChunk) {
-      uint64_t word[kLeafChunk];
-#pragma unroll
-      for (uint32_t q = 0; q < kLeafChunk; q++) {
-        const uint32_t w = w0 + q * kLeafGroup, x0 = 8 * w;
-        uint64_t v = w < kLeafHeadWords ? head[g][w] : 0ull;
-        if (w < nw && !v_raw && x0 + 8 > vstart && x0 < enc)
-          v |= value_word(va, vsz, (int64_t)x0 - (int64_t)vstart);
-        if (enc >= x0 && enc < x0 + 8) v |= 0x01ull << (8 * (enc - x0));  // Keccak padding
-        if (w + 1 == nw) v |= 0x80ull << 56;
-        word[q] = v;
-      }
+    if (s < kLeafHeadWords) word[0] |= head[g][s];
+    for (uint32_t w0 = s;;) {
 #pragma unroll
       for (uint32_t q = 0; q < kLeafChunk; q++)
         if (w0 + q * kLeafGroup < nw) out[w0 + q * kLeafGroup] = word[q];
+      w0 += kLeafGroup * kLeafChunk;
+      if (w0 >= nw) break;
+      chunk(w0);
     }
     if (s == 0) enc_len[i] = enc;
   } else if (active && s == 0) {
@@ -743,6 +784,153 @@ __global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list
   node_done(P, r, 32);
 }
 
+// Leaf level, leaves of at most two Keccak blocks: encode and hash in one
+// launch, one thread per leaf, without a scratch message.  The leaf's
+// metadata comes from the classification (leaf_meta, sorted order), so the
+// loads are at most two deep; the thread builds the message head (list
+// header, compact key, value header) once, then computes each 8-byte message
+// word as it absorbs it: head word, value bytes (aligned value words shifted
+// by the value's offset in the message and masked to the value's bytes),
+// Keccak padding.  The state stays in registers as in k_hc_perm_lane.
+// Leaves the word builder does not cover (key remainder over 64 nibbles)
+// take the generic encoder into their scratch slot.
+
+// Aligned 8-byte load from global memory.
+typedef const __attribute__((address_space(1))) uint64_t gword_t;
+__device__ __forceinline__ uint64_t gload(uint64_t a) { return *(gword_t *)a; }
+
+// Message layout of a leaf from its metadata word m0 (leaf_meta) and the
+// value's first byte (used only for a 1-byte value, which may be raw).
+struct LeafLayout {
+  uint32_t cnt, vsz, payload, enc, nb;
+  bool raw, fast;
+};
+__device__ __forceinline__ LeafLayout leaf_layout(uint64_t m0, uint32_t vb0) {
+  LeafLayout l;
+  l.cnt = (uint32_t)(m0 >> 41) & 255u;
+  l.vsz = (uint32_t)(m0 >> 50);
+  const uint32_t kc = l.cnt / 2 + 1;
+  const uint32_t key_item = kc == 1 ? 1u : 1u + kc;
+  l.raw = l.vsz == 1 && vb0 <= 0x7f;
+  l.payload = key_item + (l.raw ? 1u : rlp_hdr_len(l.vsz) + l.vsz);
+  l.enc = rlp_hdr_len(l.payload) + l.payload;
+  l.nb = keccak_blocks(l.enc);
+  l.fast = l.cnt <= kLeafFastNibbles && l.enc >= 32 && l.nb <= 2;
+  return l;
+}
+
+__device__ __forceinline__ int32_t clamp_i(int32_t x, int32_t lo, int32_t hi) { return max(lo, min(x, hi)); }
+
+// Bytes [sh, sh + 8) of the little-endian 16-byte pair (lo, hi), sh in 0..7.
+__device__ __forceinline__ uint64_t funnel_bytes(uint64_t lo, uint64_t hi, uint32_t sh) {
+  const uint32_t d0 = (uint32_t)lo, d1 = (uint32_t)(lo >> 32), d2 = (uint32_t)hi, d3 = (uint32_t)(hi >> 32);
+  const bool s4 = sh >= 4;
+  const uint32_t x = s4 ? d1 : d0, y = s4 ? d2 : d1, z = s4 ? d3 : d2;
+  const uint32_t r0 = __builtin_amdgcn_alignbyte(y, x, sh & 3);
+  const uint32_t r1 = __builtin_amdgcn_alignbyte(z, y, sh & 3);
+  return ((uint64_t)r1 << 32) | r0;
+}
+
+// A leaf off the word builder's path: the generic encoder writes its padded
+// message to out (or, under 32 bytes, its raw encoding to the hash); returns
+// the blocks.  Kept out of line: it is rare and large.
+__device__ __noinline__ uint32_t leaf_generic(const Pools &P, ref_t r, uint64_t *out, uint32_t *enc) {
+  return encode_generic(P, r, out, enc);
+}
+
+constexpr uint32_t kLeafDirectHead = 7;  // LDS words per thread for the head (6 used; odd spreads banks)
+
+__global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__restrict__ list,
+                                                        const uint64_t *__restrict__ lmeta,
+                                                        const uint64_t *__restrict__ slot, uint32_t n,
+                                                        uint64_t *__restrict__ scratch,
+                                                        uint32_t *__restrict__ enc_len) {
+  __shared__ uint64_t head[256][kLeafDirectHead];
+  const uint32_t i = grid_tid();
+  if (i >= n) return;  // no block barrier below
+  const ref_t r = list[i];
+  const uint64_t m0 = lmeta[2 * (size_t)i], va = lmeta[2 * (size_t)i + 1];
+  const uint32_t vsz = (uint32_t)(m0 >> 50);
+  const uint32_t vb0 = vsz == 1 ? (uint32_t)(gload(va & ~7ull) >> (8 * (va & 7))) & 0xffu : 0u;
+  const LeafLayout l = leaf_layout(m0, vb0);
+  uint32_t enc = l.enc, nb = l.nb;
+  const uint64_t *gmsg = nullptr;  // generic leaves: the message in scratch
+  uint64_t hw[kLeafHeadWords] = {0, 0, 0, 0, 0, 0};
+  if (l.fast) {
+    const uint64_t ka = (uint64_t)(uintptr_t)P.keys + (m0 & ((1ull << 40) - 1));
+    const uint32_t koff = (uint32_t)(ka & 7);
+    const uint32_t ks1 = (uint32_t)(m0 >> 40) & 1u, term = (uint32_t)(m0 >> 49) & 1u;
+    const uint32_t kbytes = l.cnt ? (ks1 + l.cnt - 1) / 2 + 1 : 1;
+    const uint32_t nkw = (koff + kbytes + 1 + 7) / 8;
+    uint64_t kw[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? gload(ka - koff + 8 * q) : 0ull;
+    uint64_t *hp = head[threadIdx.x];
+#pragma unroll
+    for (uint32_t q = 0; q < kLeafHeadWords; q++) hp[q] = 0;
+    leaf_head(hp, l.payload, kw, koff, ks1, l.cnt, term, vsz, l.raw, vb0);
+#pragma unroll
+    for (uint32_t q = 0; q < kLeafHeadWords; q++) hw[q] = hp[q];
+  } else {
+    uint64_t *out = scratch + slot[i] / 8;
+    nb = leaf_generic(P, r, out, &enc);
+    gmsg = out;
+  }
+  enc_len[i] = enc;
+  if (nb == 0) return;  // raw encoding (< 32 bytes): the generic encoder set the hash
+  // value geometry: message byte x is value byte x - vstart, i.e. byte
+  // x + t0 from the aligned base V0; word w takes aligned words
+  // jbase + w and jbase + w + 1 shifted by sh bytes
+  const bool hasv = l.fast && !l.raw && vsz > 0;
+  const uint32_t vstart = enc - vsz;
+  const uint64_t V0 = va & ~7ull;
+  const uint32_t u = (uint32_t)(va & 7);
+  const int32_t J = hasv ? (int32_t)((u + vsz - 1) >> 3) : 0;  // last aligned word holding value bytes
+  const int32_t t0 = (int32_t)u - (int32_t)vstart;
+  const uint32_t sh = (uint32_t)t0 & 7u;
+  const int32_t jbase = (t0 - (int32_t)sh) >> 3;
+  uint64_t st[25];
+#pragma unroll
+  for (int q = 0; q < 25; q++) st[q] = 0;
+#pragma unroll
+  for (uint32_t b = 0; b < 2; b++) {
+    if (b < nb) {
+      if (l.fast) {
+        uint64_t W[18];
+#pragma unroll
+        for (int q = 0; q < 18; q++) {
+          const int32_t j = clamp_i(jbase + (int32_t)(17 * b) + q, 0, J);  // clamped into the value
+          W[q] = hasv ? gload(V0 + 8 * (uint64_t)(uint32_t)j) : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < 17; q++) {
+          const int32_t w = 17 * (int32_t)b + q;
+          // value bytes: message bytes [max(vstart, 8w), min(enc, 8w + 8)) of this word
+          const int32_t lo_b = clamp_i((int32_t)vstart - 8 * w, 0, 8);
+          const int32_t hi_b = clamp_i((int32_t)enc - 8 * w, 0, 8);
+          const uint64_t f = funnel_bytes(W[q], W[q + 1], sh);
+          const uint64_t mk = hasv && hi_b > lo_b ? (~0ull << (8 * lo_b)) & (~0ull >> (64 - 8 * hi_b)) : 0ull;
+          uint64_t v = f & mk;
+          if (w < (int32_t)kLeafHeadWords) v |= hw[w];
+          if ((int32_t)(enc >> 3) == w) v |= 0x01ull << (8 * (enc & 7));  // Keccak padding
+          if (q == 16 && b + 1 == nb) v |= 0x80ull << 56;
+          st[q] ^= v;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 17; q++) st[q] ^= gmsg[17 * b + q];
+      }
+      keccak_f1600(st);
+    }
+  }
+  uint64_t *hp = (uint64_t *)node_hash(P, r);
+  hp[0] = st[0];
+  hp[1] = st[1];
+  hp[2] = st[2];
+  hp[3] = st[3];
+  node_done(P, r, 32);
+}
+
 // Keccak-f[1600] with one state spread over lanes 0..24 of a 32-lane group:
 // lane t holds word A[t % 5, t / 5] as (l, h) halves.  Lanes 25..31 compute
 // values nobody reads.  Shuffles stay inside the 32-lane group.
@@ -771,7 +959,8 @@ __device__ __forceinline__ uint32_t sh32(uint32_t v, uint32_t src) { return (uin
 // directly (no separate parity exchange); pi and chi's two row neighbours are
 // gathered together from the rho-rotated words.
 __device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const SpreadLane &c) {
-#pragma unroll 1
+  const uint32_t iota = c.t == 0 ? ~0u : 0u;  // lane A[0, 0] takes the round constant
+#pragma unroll
   for (int ir = 0; ir < 24; ir++) {
     // theta: A ^= C[x - 1] ^ rotl(C[x + 1], 1)
     uint32_t cml = xor3(xor3(sh32(l, c.xm), sh32(l, c.xm + 5), sh32(l, c.xm + 10)), sh32(l, c.xm + 15),
@@ -792,12 +981,9 @@ __device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const Sp
     // pi + chi: B[x, y] ^ (~B[x + 1, y] & B[x + 2, y])
     l = chi3(sh32(nl, c.s0), sh32(nl, c.s1), sh32(nl, c.s2));
     h = chi3(sh32(nh, c.s0), sh32(nh, c.s1), sh32(nh, c.s2));
-    // iota
-    if (c.t == 0) {
-      uint64_t rc = kKeccak.rc[ir];
-      l ^= (uint32_t)rc;
-      h ^= (uint32_t)(rc >> 32);
-    }
+    // iota (unrolled: the constants are immediates, no load per round)
+    l ^= (uint32_t)kKeccak.rc[ir] & iota;
+    h ^= (uint32_t)(kKeccak.rc[ir] >> 32) & iota;
   }
 }
 
@@ -978,7 +1164,8 @@ __global__ __launch_bounds__(256) void k_enc_write(Pools P, const ref_t *refs, u
 size_t hc_temp_bytes(uint32_t n) { return scan_temp_bytes(n > kHcBins ? n : kHcBins); }
 
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
-                       uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s) {
+                       uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp,
+                       uint64_t *lmeta) {
   k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins);
   GMPT_LAUNCH_CHECK();
   // the pass's work counters (permutations, bytes, leaf permutations, leaf bytes)
@@ -989,11 +1176,11 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
     return hipGetLastError();
   }
   unsigned tiles = (unsigned)((n + kHcTile - 1) / kHcTile);
-  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist, packed);
+  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist, packed, lmeta ? lmeta_tmp : nullptr);
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(meta->hist, cursor, kHcBins, &meta->nvalid, temp, s);
   if (e != hipSuccess) return e;
-  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, packed, n, cursor, out, slot);
+  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, packed, n, cursor, out, slot, lmeta_tmp, lmeta);
   GMPT_LAUNCH_CHECK();
   k_hc_slot_tail<<<nblk(n), kB, 0, s>>>(&meta->nvalid, n, slot);
   GMPT_LAUNCH_CHECK();
@@ -1002,8 +1189,22 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
-                         hipEvent_t perm_end) {
+                         hipEvent_t perm_end, uint32_t n_fused, const uint64_t *lmeta) {
   if (n == 0) return hipSuccess;
+  if (leaves && n_fused && lmeta) {  // leaves of <= 2 Keccak blocks: encode and hash in one launch
+    if (perm_begin) {
+      hipError_t e = hipEventRecord(perm_begin, s);
+      if (e != hipSuccess) return e;
+    }
+    k_hc_leaf_direct<<<nblk(n_fused), kB, 0, s>>>(P, list, lmeta, slot, n_fused, scratch, enc_len);
+    GMPT_LAUNCH_CHECK();
+    if (perm_end) {
+      hipError_t e = hipEventRecord(perm_end, s);
+      if (e != hipSuccess) return e;
+    }
+    return hc_hash_range(P, list + n_fused, slot + n_fused, n - n_fused, scratch, enc_len + n_fused, true,
+                         spread_max, s, nullptr, nullptr, 0, nullptr);
+  }
   if (!leaves && n <= spread_max) {  // small inner level: encode and hash in one launch
     if (perm_begin) {
       hipError_t e = hipEventRecord(perm_begin, s);

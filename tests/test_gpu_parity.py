@@ -261,6 +261,47 @@ def test_keccak_layouts_give_same_root(gmpt, workloads, oracle, spread_max):
     assert t.get_root_hash() == oracle.root_of(b)
 
 
+def _leaf_case(workloads, case):
+    if case == "c2":
+        return _rand_batch(workloads, 93, 5000, 70, 110)
+    if case == "block-edge":  # 129 leaves: a partial second block of the fused kernel
+        return _rand_batch(workloads, 94, 129, 70, 110)
+    if case == "tiny-values":  # raw single-byte and empty values
+        return _rand_batch(workloads, 95, 3000, 0, 2)
+    if case == "long-keys":  # key remainders over 64 nibbles take the generic encoder
+        return _rand_batch(workloads, 96, 2000, 10, 60, key_len=44)
+    if case == "short-keys":  # leaf encodings under 32 bytes keep the raw encoding
+        rng = random.Random(97)
+        keys = list({bytes(rng.getrandbits(8) for _ in range(rng.randint(1, 3))) for _ in range(3000)})
+        vals = [bytes(rng.getrandbits(8) for _ in range(rng.randint(0, 40))) for _ in keys]
+        return workloads.batch_from_bytes(keys, vals)
+    return _rand_batch(workloads, 98, 8000, 20, 600)  # mixed: one to five blocks per leaf
+
+
+@pytest.mark.parametrize("fuse", [0, 1], ids=["leaf-kernels", "leaf-fused"])
+@pytest.mark.parametrize("case", ["c2", "block-edge", "tiny-values", "long-keys", "short-keys", "mixed"])
+def test_leaf_paths_match_oracle(gmpt, workloads, oracle, fuse, case):
+    """The fused leaf kernel (encoding through LDS) and the encode + scratch +
+    permutation kernels give the oracle's root and the same per-pass counts."""
+    b = _leaf_case(workloads, case)
+    t = gmpt.MPT(device=0)
+    t.set_option("hc_fuse_leaves", fuse)
+    assert t.get_option("hc_fuse_leaves") == fuse
+    targets, n = t.puts_2phase_with_valuehp(b)
+    t.hash_onepass_v2(targets, n)
+    assert t.get_root_hash() == oracle.root_of(b)
+    st = t.stats()
+    t2 = gmpt.MPT(device=0)
+    t2.set_option("hc_fuse_leaves", 1 - fuse)
+    targets, n = t2.puts_2phase_with_valuehp(b)
+    t2.hash_onepass_v2(targets, n)
+    st2 = t2.stats()
+    for k in ("perms", "enc_bytes", "leaf_perms", "leaf_bytes", "leaf_nodes", "hashed_nodes"):
+        assert st[k] == st2[k], k
+    with pytest.raises(gmpt.GmptError):
+        t.set_option("hc_fuse_leaves", 2)
+
+
 def test_autotune_keeps_results(gmpt, workloads, oracle):
     b = _rand_batch(workloads, 92, 6000, 70, 110)
     t = gmpt.MPT(device=0)
