@@ -44,7 +44,7 @@ PERM_PEAK_G = VALU_PEAK_TOPS * 1e12 / INSTR_PER_PERM / 1e9
 # workload: rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per dispatch,
 # the factors being what the same counters read for the 1 GiB k_copy16
 # calibration copy (--config hbm-copy).  Keyed by "leaves fused".
-LEAF_TRAFFIC_BYTES = {False: 307_465_891, True: None}
+LEAF_TRAFFIC_BYTES = {False: None, True: 582_339_167}
 
 
 def _import_package():
@@ -119,7 +119,7 @@ def bench_phase(pkg, args):
     value = b.n * args.steps / wall
 
     # roofline of the dominant kernel: the leaf level's Keccak permutations
-    # (k_hc_leaf_fused, which also encodes the leaves, or k_hc_perm_lane),
+    # (k_hc_leaf_direct, which also encodes the leaves, or k_hc_perm_lane),
     # timed with HIP events on the forest's stream
     perms = st["perms"]
     hms = sum(hash_ms) / len(hash_ms)
@@ -143,8 +143,8 @@ def bench_phase(pkg, args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PERM_PEAK_G, "unit": "Gperm/s",
                      "frac": achieved / PERM_PEAK_G,
                      "traffic": LEAF_TRAFFIC_BYTES[fused] if b.n == 1_000_000 else None,
-                     "kernel": ("k_hc_leaf_fused, leaf level (RLP encoding through LDS + Keccak-f[1600], "
-                                "one state per lane)") if fused else
+                     "kernel": ("k_hc_leaf_direct, leaf level (leaf encoding built word by word in registers "
+                                "+ Keccak-f[1600], one state per lane)") if fused else
                                "k_hc_perm_lane, leaf level (Keccak-f[1600], one state per lane)",
                      "per_launch": {"permutations": st["leaf_perms"], "nodes": st["leaf_nodes"],
                                     "ms": leaf_ms, "message_bytes": st["leaf_bytes"]},
