@@ -133,6 +133,7 @@ class Forest {
   bool hc_fuse_leaves_ = true;   // leaves of <= 2 Keccak blocks: one encode+hash kernel
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
+  hipEvent_t ev_meta_ = nullptr;               // the classification bins have reached the host
   OpStats stats_;
 };
 

@@ -78,6 +78,7 @@ Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
   if (hipMemset(d_cnt_, 0, sizeof(Counters)) != hipSuccess) return;
   if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return;
   if (hipEventCreate(&ev2_) != hipSuccess || hipEventCreate(&ev3_) != hipSuccess) return;
+  if (hipEventCreateWithFlags(&ev_meta_, hipEventDisableTiming) != hipSuccess) return;
   if (grow(tries_, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1), 0) != 0) return;
   if (hipMemset(tries_.p, 0, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1)) != hipSuccess) return;
   if (hipHostMalloc(&h_meta_, sizeof(HcMeta)) != hipSuccess) return;
@@ -104,6 +105,7 @@ Forest::~Forest() {
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (ev2_) (void)hipEventDestroy(ev2_);
   if (ev3_) (void)hipEventDestroy(ev3_);
+  if (ev_meta_) (void)hipEventDestroy(ev_meta_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -493,21 +495,33 @@ int Forest::hash(void *targets, int n_targets) {
     HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
                         encl, ws2_.p, stream_, lmeta_tmp, lmeta));
     HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipEventRecord(ev_meta_, stream_));
+    HIP_TRY(hipEventRecord(ev0_, stream_));
+    if (lmeta) {
+      // leaves of one or two blocks (the first two bins): launched before the
+      // bins reach the host, so the GPU hashes them while the host plans the
+      // other levels
+      HIP_TRY(hipEventRecord(ev2_, stream_));
+      HIP_TRY(hc_hash_leaves_direct(P, sorted, lmeta, (const HcMeta *)hc_meta_.p, n, encl, stream_));
+      HIP_TRY(hipEventRecord(ev3_, stream_));
+    }
+    HIP_TRY(hipEventSynchronize(ev_meta_));
     const HcMeta &M = *(const HcMeta *)h_meta_;
     if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
     uint64_t *scratch = (uint64_t *)hc_scratch_.p;
-    HIP_TRY(hipEventRecord(ev0_, stream_));
+    const uint32_t n_direct = lmeta ? M.hist[0] + M.hist[1] : 0u;
     uint32_t start = 0, n_leaf = 0;
     for (uint32_t lv = 0; lv < kHcLevels; lv++) {
       uint32_t cnt = 0;
       for (uint32_t c = 0; c < kHcClasses; c++) cnt += M.hist[lv * kHcClasses + c];
       if (cnt) {
         const bool leaves = lv == 0;
-        // leaves of one or two Keccak blocks (classes 0 and 1, binned first)
-        const uint32_t n_fused = leaves && lmeta ? M.hist[0] + M.hist[1] : 0u;
-        HIP_TRY(hc_hash_range(P, sorted + start, slot + start, cnt, scratch, encl + start, leaves, hc_spread_max_,
-                              stream_, leaves ? ev2_ : nullptr, leaves ? ev3_ : nullptr, n_fused, lmeta));
+        const uint32_t skip = leaves ? n_direct : 0u;  // already queued
+        const bool timed = leaves && !lmeta;
+        if (cnt > skip)
+          HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
+                                encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
+                                timed ? ev3_ : nullptr));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
         if (leaves) n_leaf = cnt;

@@ -23,23 +23,26 @@ size_t hc_temp_bytes(uint32_t n);
 // Bins dirty nodes by (level, block class): out = the list ordered by bin,
 // slot[i] = byte offset of sorted node i's message slot in the scratch buffer;
 // packed (n entries) is workspace.  With lmeta (2n words, lmeta_tmp 2n words
-// of workspace) the leaves of one or two Keccak blocks -- the first
-// hist[0] + hist[1] sorted entries -- get no scratch slot but a metadata
-// record lmeta[2i], lmeta[2i + 1] for the LDS leaf kernel.
+// of workspace) the leaves of one or two Keccak blocks whose key remainder
+// has at most 64 nibbles -- the first hist[0] + hist[1] sorted entries --
+// get no scratch slot but a metadata record lmeta[2i], lmeta[2i + 1] for
+// hc_hash_leaves_direct.
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp = nullptr,
                        uint64_t *lmeta = nullptr);
 // Encodes and hashes list[0..n) (one level: no node depends on another);
 // slot and enc_len are the level's slices of the classification arrays;
-// leaves = the leaf level (level 0: only leaf short nodes).  The first
-// n_fused leaves (those of at most two Keccak blocks, binned first, with
-// classification metadata lmeta) are encoded and hashed by one kernel
-// through LDS, without scratch slots.  Events, when given, are recorded
-// around that kernel, or else around the permutation kernel.
+// leaves = the leaf level (level 0: only leaf short nodes).  Events, when
+// given, are recorded around the permutation kernel.
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s,
-                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr, uint32_t n_fused = 0,
-                         const uint64_t *lmeta = nullptr);
+                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr);
+// Encodes and hashes the classification's first meta->hist[0] + hist[1]
+// sorted entries (leaves with metadata, see hc_classify) in one kernel; the
+// count is read on the device, n_upper (>= it) sizes the launch, so the
+// launch need not wait for the bins to reach the host.
+hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64_t *lmeta, const HcMeta *meta,
+                                 uint32_t n_upper, uint32_t *enc_len, hipStream_t s);
 // Adds the pass's permutations and encoding bytes (from the enc_len array of
 // the meta->nvalid hashed nodes) to the forest counters, and those of the
 // first n_leaf entries (the leaf level) to the leaf counters.

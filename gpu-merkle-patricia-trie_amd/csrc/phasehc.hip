@@ -169,9 +169,15 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
     if (r != REF_NULL) {
       uint32_t nb;
       uint32_t bin = bin_of(P, r, &nb);
+      if (lmeta_tmp && bin < kLdsLeafBins) {
+        uint64_t *m = lmeta_tmp + 2 * (size_t)i;
+        leaf_meta(P, r, m);
+        // a key remainder over the word builder's reach needs the generic
+        // encoder and a scratch slot: bin it with the three-block leaves
+        if (((m[0] >> 41) & 255u) > kLeafFastNibbles) bin = kLdsLeafBins;
+      }
       atomicAdd(&h[bin], 1u);
       pk = bin | (min(nb, kPackedNbMax) << 12);
-      if (lmeta_tmp && bin < kLdsLeafBins) leaf_meta(P, r, lmeta_tmp + 2 * (size_t)i);
     }
     packed[i] = pk;
   }
@@ -213,12 +219,10 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *__rest
       uint32_t nb = pk[q] >> 12;
       if (nb == kPackedNbMax) bin_of(P, r, &nb);  // saturated: recompute exactly
       out[pos] = r;
-      if (lmeta && (pk[q] & 0xfff) < kLdsLeafBins) {
-        const uint64_t m0 = lmeta_tmp[2 * (size_t)i];
-        lmeta[2 * (size_t)pos] = m0;
+      if (lmeta && (pk[q] & 0xfff) < kLdsLeafBins) {  // hashed by k_hc_leaf_direct: no scratch slot
+        lmeta[2 * (size_t)pos] = lmeta_tmp[2 * (size_t)i];
         lmeta[2 * (size_t)pos + 1] = lmeta_tmp[2 * (size_t)i + 1];
-        // the word builder needs no scratch slot; the generic encoder does
-        if (((m0 >> 41) & 255u) <= kLeafFastNibbles) nb = 0;
+        nb = 0;
       }
       slot[pos] = 136ull * nb;
     }
@@ -792,8 +796,11 @@ __global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list
 // word as it absorbs it: head word, value bytes (aligned value words shifted
 // by the value's offset in the message and masked to the value's bytes),
 // Keccak padding.  The state stays in registers as in k_hc_perm_lane.
-// Leaves the word builder does not cover (key remainder over 64 nibbles)
-// take the generic encoder into their scratch slot.
+// The kernel reads its leaf count (the first two bins) from the
+// classification on the device, so it can be launched before the host has
+// read the bins back.  Leaves with a key remainder over 64 nibbles are binned
+// out of its range; the only leaves here off the word builder's path are
+// those whose encoding is under 32 bytes (raw "hash", no permutation).
 
 // Aligned 8-byte load from global memory.
 typedef const __attribute__((address_space(1))) uint64_t gword_t;
@@ -831,32 +838,43 @@ __device__ __forceinline__ uint64_t funnel_bytes(uint64_t lo, uint64_t hi, uint3
   return ((uint64_t)r1 << 32) | r0;
 }
 
-// A leaf off the word builder's path: the generic encoder writes its padded
-// message to out (or, under 32 bytes, its raw encoding to the hash); returns
-// the blocks.  Kept out of line: it is rare and large.
-__device__ __noinline__ uint32_t leaf_generic(const Pools &P, ref_t r, uint64_t *out, uint32_t *enc) {
-  return encode_generic(P, r, out, enc);
+// A leaf whose encoding is under 32 bytes: the generic encoder writes the raw
+// encoding as its hash.  Returns the encoding length.  Kept out of line: it
+// is rare and large.
+__device__ __noinline__ uint32_t leaf_raw(const Pools &P, ref_t r) {
+  const uint32_t pay = node_payload(P, r);
+  const uint32_t e = rlp_hdr_len(pay) + pay;
+  if (e >= 32) {  // the binning keeps such leaves out of this kernel
+    raise_error(P.cnt, ERR_INTERNAL, 5);
+    return e;
+  }
+  uint32_t enc;
+  encode_generic(P, r, nullptr, &enc);
+  return enc;
 }
 
 constexpr uint32_t kLeafDirectHead = 7;  // LDS words per thread for the head (6 used; odd spreads banks)
 
 __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__restrict__ list,
                                                         const uint64_t *__restrict__ lmeta,
-                                                        const uint64_t *__restrict__ slot, uint32_t n,
-                                                        uint64_t *__restrict__ scratch,
+                                                        const uint32_t *__restrict__ hist,
                                                         uint32_t *__restrict__ enc_len) {
   __shared__ uint64_t head[256][kLeafDirectHead];
   const uint32_t i = grid_tid();
+  const uint32_t n = hist[0] + hist[1];  // leaves of one and two blocks, sorted first
   if (i >= n) return;  // no block barrier below
   const ref_t r = list[i];
   const uint64_t m0 = lmeta[2 * (size_t)i], va = lmeta[2 * (size_t)i + 1];
   const uint32_t vsz = (uint32_t)(m0 >> 50);
   const uint32_t vb0 = vsz == 1 ? (uint32_t)(gload(va & ~7ull) >> (8 * (va & 7))) & 0xffu : 0u;
   const LeafLayout l = leaf_layout(m0, vb0);
-  uint32_t enc = l.enc, nb = l.nb;
-  const uint64_t *gmsg = nullptr;  // generic leaves: the message in scratch
+  if (!l.fast) {
+    enc_len[i] = leaf_raw(P, r);
+    return;
+  }
+  const uint32_t enc = l.enc, nb = l.nb;
   uint64_t hw[kLeafHeadWords] = {0, 0, 0, 0, 0, 0};
-  if (l.fast) {
+  {
     const uint64_t ka = (uint64_t)(uintptr_t)P.keys + (m0 & ((1ull << 40) - 1));
     const uint32_t koff = (uint32_t)(ka & 7);
     const uint32_t ks1 = (uint32_t)(m0 >> 40) & 1u, term = (uint32_t)(m0 >> 49) & 1u;
@@ -871,17 +889,12 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
     leaf_head(hp, l.payload, kw, koff, ks1, l.cnt, term, vsz, l.raw, vb0);
 #pragma unroll
     for (uint32_t q = 0; q < kLeafHeadWords; q++) hw[q] = hp[q];
-  } else {
-    uint64_t *out = scratch + slot[i] / 8;
-    nb = leaf_generic(P, r, out, &enc);
-    gmsg = out;
   }
   enc_len[i] = enc;
-  if (nb == 0) return;  // raw encoding (< 32 bytes): the generic encoder set the hash
   // value geometry: message byte x is value byte x - vstart, i.e. byte
   // x + t0 from the aligned base V0; word w takes aligned words
   // jbase + w and jbase + w + 1 shifted by sh bytes
-  const bool hasv = l.fast && !l.raw && vsz > 0;
+  const bool hasv = !l.raw && vsz > 0;
   const uint32_t vstart = enc - vsz;
   const uint64_t V0 = va & ~7ull;
   const uint32_t u = (uint32_t)(va & 7);
@@ -895,7 +908,7 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
 #pragma unroll
   for (uint32_t b = 0; b < 2; b++) {
     if (b < nb) {
-      if (l.fast) {
+      {
         uint64_t W[18];
 #pragma unroll
         for (int q = 0; q < 18; q++) {
@@ -916,9 +929,6 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
           if (q == 16 && b + 1 == nb) v |= 0x80ull << 56;
           st[q] ^= v;
         }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 17; q++) st[q] ^= gmsg[17 * b + q];
       }
       keccak_f1600(st);
     }
@@ -1189,22 +1199,8 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
-                         hipEvent_t perm_end, uint32_t n_fused, const uint64_t *lmeta) {
+                         hipEvent_t perm_end) {
   if (n == 0) return hipSuccess;
-  if (leaves && n_fused && lmeta) {  // leaves of <= 2 Keccak blocks: encode and hash in one launch
-    if (perm_begin) {
-      hipError_t e = hipEventRecord(perm_begin, s);
-      if (e != hipSuccess) return e;
-    }
-    k_hc_leaf_direct<<<nblk(n_fused), kB, 0, s>>>(P, list, lmeta, slot, n_fused, scratch, enc_len);
-    GMPT_LAUNCH_CHECK();
-    if (perm_end) {
-      hipError_t e = hipEventRecord(perm_end, s);
-      if (e != hipSuccess) return e;
-    }
-    return hc_hash_range(P, list + n_fused, slot + n_fused, n - n_fused, scratch, enc_len + n_fused, true,
-                         spread_max, s, nullptr, nullptr, 0, nullptr);
-  }
   if (!leaves && n <= spread_max) {  // small inner level: encode and hash in one launch
     if (perm_begin) {
       hipError_t e = hipEventRecord(perm_begin, s);
@@ -1231,6 +1227,13 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
   GMPT_LAUNCH_CHECK();
   if (perm_end) return hipEventRecord(perm_end, s);
   return hipSuccess;
+}
+
+hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64_t *lmeta, const HcMeta *meta,
+                                 uint32_t n_upper, uint32_t *enc_len, hipStream_t s) {
+  if (n_upper == 0) return hipSuccess;
+  k_hc_leaf_direct<<<nblk(n_upper), kB, 0, s>>>(P, list, lmeta, meta->hist, enc_len);
+  return hipGetLastError();
 }
 
 hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
