@@ -44,7 +44,7 @@ size_t nu_ws_bytes(size_t n, bool tries) {
   size_t ne = 2 * n + 2;
   size_t k = (size_t)st_levels_for(ne);
   size_t b = 0;
-  b += n * (4 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
+  b += n * (4 + 8 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
   b += n * (4 * 7 + 1 + sizeof(Stop));
   b += ne * (4 + 4 + 4 + 4) + k * ne;
   b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
@@ -321,6 +321,7 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   size_t ne = 2 * (size_t)n + 2;
   a.klen = c.take<uint32_t>(n);
   a.kpoff = c.take<uint64_t>(n);
+  a.kref = c.take<uint64_t>(n);
   a.sk0 = c.take<uint64_t>(n);
   a.sk1 = c.take<uint64_t>(n);
   a.sv0 = c.take<uint32_t>(n);

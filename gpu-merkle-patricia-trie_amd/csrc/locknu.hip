@@ -314,9 +314,10 @@ __global__ __launch_bounds__(256) void k_lock_insert(NuArgs a, uint64_t key_cap,
   c.i = i;
   c.key_cap = key_cap;
   if (active) {
-    c.k.p = a.P.keys + a.kbase + a.kpoff[i];
-    c.k.len = a.klen[i];
-    c.key_off = a.kbase + a.kpoff[i];
+    const uint64_t kr = a.kref[i];
+    c.key_off = a.kbase + (kr & ((1ull << 48) - 1));
+    c.k.p = a.P.keys + c.key_off;
+    c.k.len = (uint32_t)(kr >> 48);
     c.trie = a.tid ? (uint32_t)a.tid[i] : 0u;
     c.seq = a.seq_base + i;
   }

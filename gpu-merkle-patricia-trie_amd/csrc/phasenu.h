@@ -52,6 +52,7 @@ struct NuArgs {
   // per input key
   uint32_t *klen;
   uint64_t *kpoff;       // relative to kbase after the scan
+  uint64_t *kref;        // kpoff | nibbles << 48: one load gives a key's place and length
   // sort
   uint64_t *sk0, *sk1;
   uint32_t *sv0, *sv1;

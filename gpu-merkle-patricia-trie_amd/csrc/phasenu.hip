@@ -36,10 +36,12 @@ struct KeyRef {
   __device__ __forceinline__ uint32_t nib(uint32_t j) const { return j + 1 == len ? kNibbleT : packed_nibble(p, j); }
 };
 
+constexpr uint64_t kKrefOff = (1ull << 48) - 1;
 __device__ __forceinline__ KeyRef key_of(const NuArgs &a, uint32_t idx) {
+  const uint64_t r = a.kref[idx];
   KeyRef k;
-  k.p = a.P.keys + a.kbase + a.kpoff[idx];
-  k.len = a.klen[idx];
+  k.p = a.P.keys + a.kbase + (r & kKrefOff);
+  k.len = (uint32_t)(r >> 48);
   return k;
 }
 
@@ -223,7 +225,9 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
   if (i >= a.n) return;
   uint32_t L = a.klen[i];
   const uint8_t *k = key_range_ok(a, i) ? stg.at(a, a.kidx[2 * i]) : a.kh;
-  uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + a.kpoff[i]);
+  const uint64_t koff = a.kpoff[i];
+  a.kref[i] = koff | ((uint64_t)L << 48);
+  uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + koff);
   uint32_t nb = (L / 2 + 8) & ~7u;
   uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
   for (uint32_t b = 0; b < nb; b += 8) {
@@ -276,18 +280,33 @@ __device__ __forceinline__ uint32_t trie_of(const NuArgs &a, uint32_t idx) {
 }
 
 __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, const uint32_t *sv, int full) {
-  uint32_t j = grid_tid();
-  if (j >= a.n) return;
-  uint32_t ib = sv[j];
+  const uint32_t j = grid_tid();
+  const bool active = j < a.n;
+  // key j's place and trie; key j - 1's come from the lane before (one
+  // random load per key instead of two), or a load on a wave's first lane
+  const uint32_t ib = active ? sv[j] : 0u;
+  const uint64_t rb = active ? a.kref[ib] : 0ull;
+  const uint32_t tb = active ? trie_of(a, ib) : 0u;
+  uint64_t ra = (uint64_t)__shfl_up((long long)rb, 1);
+  uint32_t ta = (uint32_t)__shfl_up((int)tb, 1);
+  if (!active) return;
+  if (__lane_id() == 0 && j > 0) {
+    const uint32_t ia = sv[j - 1];
+    ra = a.kref[ia];
+    ta = trie_of(a, ia);
+  }
   uint8_t fl = 0, l = 0;
   if (j == 0) {
     fl = 2;
   } else {
-    uint32_t ia = sv[j - 1];
-    if (trie_of(a, ia) != trie_of(a, ib)) {
+    if (ta != tb) {
       fl = 2;
     } else {
-      KeyRef ka = key_of(a, ia), kb = key_of(a, ib);
+      KeyRef ka, kb;
+      ka.p = a.P.keys + a.kbase + (ra & kKrefOff);
+      ka.len = (uint32_t)(ra >> 48);
+      kb.p = a.P.keys + a.kbase + (rb & kKrefOff);
+      kb.len = (uint32_t)(rb >> 48);
       uint32_t W = a.scal[SC_W];
       uint32_t mn = min(ka.len, kb.len);
       uint32_t p;
@@ -361,12 +380,14 @@ __global__ __launch_bounds__(256) void k_descend(NuArgs a) {
   ref_t node = REF_NULL, parent = REF_NULL;
   uint32_t pos = 0, pslot = 0;
   bool done = !active;
+  k.p = nullptr;
+  k.len = 0;
   if (active) {
     uint32_t idx = a.u_idx[i];
-    k = key_of(a, idx);
     uint32_t t = a.u_trie[i];
     if (t < a.ntries) {
       node = P.tries[t].root;
+      if (node != REF_NULL) k = key_of(a, idx);  // an empty trie needs no key bytes
     } else {
       done = true;
       raise_error(P.cnt, ERR_BAD_KEY, idx);
@@ -851,11 +872,11 @@ __device__ __forceinline__ ElemKey elem_key(const NuArgs &a, uint32_t e) {
     k.virt = true;
     k.uidx = src & 0x7fffffffu;
   } else {
-    uint32_t idx = a.u_idx[src];
-    k.off = a.kbase + a.kpoff[idx];
+    const uint64_t r = a.kref[a.u_idx[src]];
+    k.off = a.kbase + (r & kKrefOff);
     k.start = 0;
     k.origin = 0;
-    k.end = a.klen[idx];
+    k.end = (uint32_t)(r >> 48);
     k.term = 1;
     k.virt = false;
     k.uidx = src;
