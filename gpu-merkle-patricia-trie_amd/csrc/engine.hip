@@ -655,8 +655,8 @@ int Forest::autotune(const BatchDesc &sample, int mode, int64_t *chosen, double 
   for (uint32_t c : kCand) {
     t.hc_spread_max_ = c;
     t.hc_fuse_leaves_ = hc_fuse_leaves_;
-    double ms = 0;
-    for (int rep = 0; rep < 2; rep++) {  // the second run is timed (pools warm)
+    double ms = 1e30;
+    for (int rep = 0; rep < 4; rep++) {  // the first run warms the pools; the best of the other three counts
       int r;
       void *tg = nullptr;
       int nt = 0;
@@ -664,7 +664,7 @@ int Forest::autotune(const BatchDesc &sample, int mode, int64_t *chosen, double 
       r = mode == 0 ? t.puts_2phase(sample, &tg, &nt) : t.puts_latching(sample, &tg, &nt);
       if (r) return r;
       if ((r = t.hash(tg, nt))) return r;
-      ms = t.stats().last_hash_ms;
+      if (rep > 0) ms = std::min(ms, (double)t.stats().last_hash_ms);
     }
     if (ms < best) {
       best = ms;

@@ -941,118 +941,130 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
   node_done(P, r, 32);
 }
 
-// Keccak-f[1600] with one state spread over lanes 0..24 of a 32-lane group:
-// lane t holds word A[t % 5, t / 5] as (l, h) halves.  Lanes 25..31 compute
-// values nobody reads.  Shuffles stay inside the 32-lane group.
+// Keccak-f[1600] with one state spread over a wave: lane 8x + y holds word
+// A[x, y] (x, y < 5) as (l, h) halves; lanes 8x + 5 .. 8x + 7 and 40..63 hold
+// zero.  A column sits in one 8-lane half-row, so theta's column parity is a
+// three-step DPP xor butterfly (quad_perm, quad_perm, row_half_mirror) with
+// no LDS traffic; the parities of columns x - 1 and x + 1, and pi with chi's
+// two row neighbours, are gathered with ds_bpermute: 10 permutes per round.
 struct SpreadLane {
-  uint32_t t, rho, xm, xp, s0, s1, s2;
-  // pi moves A[x', y'] to B[y', 2x' + 3y']: lane (x, y) receives B[x, y] from
-  // lane x' + 5y' with x' = 3 (y - 3x) mod 5, y' = x
+  uint32_t valid, iota, rho, cm, cp, s0, s1, s2;  // cm .. s2: source byte addresses for ds_bpermute
+  // pi moves A[x', y'] to B[y', 2x' + 3y']: B[x, y] comes from A[3 (y - 3x) mod 5, x]
   static __device__ __forceinline__ uint32_t pi_src(uint32_t x, uint32_t y) {
-    return (3 * (y + 5 - (3 * x) % 5)) % 5 + 5 * x;
+    return 8 * ((3 * (y + 5 - (3 * x) % 5)) % 5) + x;
   }
-  __device__ __forceinline__ explicit SpreadLane(uint32_t gl) {
-    t = gl < 25 ? gl : 0;
-    uint32_t x = t % 5, y = t / 5;
-    rho = (uint32_t)kKeccak.rho[t];
-    xm = (x + 4) % 5;
-    xp = (x + 1) % 5;
-    s0 = pi_src(x, y);
-    s1 = pi_src((x + 1) % 5, y);
-    s2 = pi_src((x + 2) % 5, y);
+  __device__ __forceinline__ explicit SpreadLane(uint32_t lane) {
+    const uint32_t x0 = lane >> 3, y0 = lane & 7;
+    const bool ok = x0 < 5 && y0 < 5;
+    const uint32_t x = ok ? x0 : 0u, y = ok ? y0 : 0u;
+    valid = ok ? ~0u : 0u;
+    iota = lane == 0 ? ~0u : 0u;
+    rho = (uint32_t)kKeccak.rho[x + 5 * y];
+    cm = 4 * (8 * ((x + 4) % 5) + y);
+    cp = 4 * (8 * ((x + 1) % 5) + y);
+    s0 = 4 * pi_src(x, y);
+    s1 = 4 * pi_src((x + 1) % 5, y);
+    s2 = 4 * pi_src((x + 2) % 5, y);
+  }
+  // message word this lane absorbs (x + 5y < 17), or -1
+  __device__ __forceinline__ static int word_of(uint32_t lane) {
+    const uint32_t x = lane >> 3, y = lane & 7;
+    return x < 5 && y < 5 && x + 5 * y < 17 ? (int)(x + 5 * y) : -1;
   }
 };
 
-__device__ __forceinline__ uint32_t sh32(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src, 32); }
+__device__ __forceinline__ uint32_t bperm(uint32_t addr, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)v);
+}
+// xor over the lane's 8-lane half-row (all eight lanes get it)
+__device__ __forceinline__ uint32_t xor8(uint32_t v) {
+  v ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1, 0, 3, 2]
+  v ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2, 3, 0, 1]
+  v ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  return v;
+}
 
-// Two exchange stages per round: theta gathers columns x - 1 and x + 1
-// directly (no separate parity exchange); pi and chi's two row neighbours are
-// gathered together from the rho-rotated words.
 __device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const SpreadLane &c) {
-  const uint32_t iota = c.t == 0 ? ~0u : 0u;  // lane A[0, 0] takes the round constant
 #pragma unroll
   for (int ir = 0; ir < 24; ir++) {
     // theta: A ^= C[x - 1] ^ rotl(C[x + 1], 1)
-    uint32_t cml = xor3(xor3(sh32(l, c.xm), sh32(l, c.xm + 5), sh32(l, c.xm + 10)), sh32(l, c.xm + 15),
-                        sh32(l, c.xm + 20));
-    uint32_t cmh = xor3(xor3(sh32(h, c.xm), sh32(h, c.xm + 5), sh32(h, c.xm + 10)), sh32(h, c.xm + 15),
-                        sh32(h, c.xm + 20));
-    uint32_t cpl = xor3(xor3(sh32(l, c.xp), sh32(l, c.xp + 5), sh32(l, c.xp + 10)), sh32(l, c.xp + 15),
-                        sh32(l, c.xp + 20));
-    uint32_t cph = xor3(xor3(sh32(h, c.xp), sh32(h, c.xp + 5), sh32(h, c.xp + 10)), sh32(h, c.xp + 15),
-                        sh32(h, c.xp + 20));
+    const uint32_t cl = xor8(l), ch = xor8(h);
+    const uint32_t cml = bperm(c.cm, cl), cmh = bperm(c.cm, ch);
+    const uint32_t cpl = bperm(c.cp, cl), cph = bperm(c.cp, ch);
     l = xor3(l, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
     h = xor3(h, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
     // rho: rotate this lane's word by its own offset
-    uint32_t rr = c.rho & 31;
-    uint32_t a = c.rho >= 32 ? h : l, b = c.rho >= 32 ? l : h;
-    uint32_t nl = rr ? __builtin_amdgcn_alignbit(a, b, 32 - rr) : a;
-    uint32_t nh = rr ? __builtin_amdgcn_alignbit(b, a, 32 - rr) : b;
+    const uint32_t rr = c.rho & 31;
+    const uint32_t a = c.rho >= 32 ? h : l, b = c.rho >= 32 ? l : h;
+    const uint32_t nl = rr ? __builtin_amdgcn_alignbit(a, b, 32 - rr) : a;
+    const uint32_t nh = rr ? __builtin_amdgcn_alignbit(b, a, 32 - rr) : b;
     // pi + chi: B[x, y] ^ (~B[x + 1, y] & B[x + 2, y])
-    l = chi3(sh32(nl, c.s0), sh32(nl, c.s1), sh32(nl, c.s2));
-    h = chi3(sh32(nh, c.s0), sh32(nh, c.s1), sh32(nh, c.s2));
-    // iota (unrolled: the constants are immediates, no load per round)
-    l ^= (uint32_t)kKeccak.rc[ir] & iota;
-    h ^= (uint32_t)(kKeccak.rc[ir] >> 32) & iota;
+    l = chi3(bperm(c.s0, nl), bperm(c.s1, nl), bperm(c.s2, nl));
+    h = chi3(bperm(c.s0, nh), bperm(c.s1, nh), bperm(c.s2, nh));
+    // iota (unrolled: the constants are immediates); padding lanes back to zero
+    l = (l ^ ((uint32_t)kKeccak.rc[ir] & c.iota)) & c.valid;
+    h = (h ^ ((uint32_t)(kKeccak.rc[ir] >> 32) & c.iota)) & c.valid;
   }
 }
 
-// Sponge over nb padded blocks, state spread over a 32-lane group; every lane
-// of the group must call it with the same m and nb.
-__device__ __forceinline__ void sponge_spread(const uint64_t *m, uint32_t nb, uint32_t gl, const SpreadLane &c,
+// Sponge over nb padded blocks, state spread over the wave; every lane must
+// call it with the same m and nb.  Digest word q ends on lane 8q (q < 4).
+__device__ __forceinline__ void sponge_spread(const uint64_t *m, uint32_t nb, uint32_t lane, const SpreadLane &c,
                                               uint32_t &l, uint32_t &h) {
   l = 0;
   h = 0;
+  const int w = SpreadLane::word_of(lane);
   for (uint32_t b = 0; b < nb; b++) {
-    if (gl < 17) {
-      uint64_t w = m[17 * b + gl];
-      l ^= (uint32_t)w;
-      h ^= (uint32_t)(w >> 32);
+    if (w >= 0) {
+      const uint64_t v = m[17 * b + (uint32_t)w];
+      l ^= (uint32_t)v;
+      h ^= (uint32_t)(v >> 32);
     }
     keccak_spread(l, h, c);
   }
 }
+__device__ __forceinline__ bool spread_digest_lane(uint32_t lane) { return (lane & 7) == 0 && lane < 32; }
 
 __global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *list, const uint64_t *slot,
                                                         const uint32_t *enc_len, uint32_t n,
                                                         const uint64_t *scratch) {
-  const uint32_t gl = __lane_id() & 31;
-  const uint32_t i = blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32;
+  const uint32_t lane = __lane_id();
+  const uint32_t i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   uint32_t enc = i < n ? enc_len[i] : 0;
-  uint32_t nb = enc >= 32 ? keccak_blocks(enc) : 0;  // uniform over the group
-  SpreadLane c(gl);
+  uint32_t nb = enc >= 32 ? keccak_blocks(enc) : 0;  // uniform over the wave
+  SpreadLane c(lane);
   uint32_t l, h;
-  sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, gl, c, l, h);
+  sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, lane, c, l, h);
   if (nb == 0) return;
   ref_t r = list[i];
-  if (gl < 4) ((uint64_t *)node_hash(P, r))[gl] = ((uint64_t)h << 32) | l;
-  if (gl == 0) node_done(P, r, 32);
+  if (spread_digest_lane(lane)) ((uint64_t *)node_hash(P, r))[lane >> 3] = ((uint64_t)h << 32) | l;
+  if (lane == 0) node_done(P, r, 32);
 }
 
-// Small inner levels in one launch: one node per 32-lane group; lanes 0..15
-// encode it (encode_inner_node), then the whole group runs the sponge with the
-// state spread over 25 lanes.  The message passes through the node's scratch
-// slot, which the block barrier makes visible to the group (it stays in cache).
+// Small inner levels in one launch: one node per wave; lanes 0..15 encode it
+// (encode_inner_node), then the wave runs the sponge with the state spread
+// over it.  The message passes through the node's scratch slot, which the
+// block barrier makes visible to the wave (it stays in cache).
 __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restrict__ list,
                                                   const uint64_t *__restrict__ slot, uint32_t n,
                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
-  __shared__ uint64_t lds[256 / 32][kInnerWords];
-  const uint32_t gl = __lane_id() & 31, g = threadIdx.x / 32;
-  const uint32_t i = blockIdx.x * (256 / 32) + g;
+  __shared__ uint64_t lds[256 / 64][kInnerWords];
+  const uint32_t lane = __lane_id(), g = threadIdx.x / 64;
+  const uint32_t i = blockIdx.x * (256 / 64) + g;
   const bool active = i < n;
   const ref_t r = active ? list[i] : REF_NULL;
   uint64_t *out = active ? scratch + slot[i] / 8 : scratch;
-  uint32_t enc = encode_inner_node(P, r, active, gl < kInnerGroup, gl & (kInnerGroup - 1), lds[g], out);
-  enc = (uint32_t)__shfl((int)enc, 0, 32);
-  if (active && gl == 0) enc_len[i] = enc;
-  __syncthreads();  // the group's message words are visible to all its lanes
-  const uint32_t nb = active && enc >= 32 ? keccak_blocks(enc) : 0u;  // uniform over the group
-  SpreadLane c(gl);
+  uint32_t enc = encode_inner_node(P, r, active, lane < kInnerGroup, lane & (kInnerGroup - 1), lds[g], out);
+  enc = (uint32_t)__shfl((int)enc, 0, 64);
+  if (active && lane == 0) enc_len[i] = enc;
+  __syncthreads();  // the node's message words are visible to all its lanes
+  const uint32_t nb = active && enc >= 32 ? keccak_blocks(enc) : 0u;  // uniform over the wave
+  SpreadLane c(lane);
   uint32_t l, h;
-  sponge_spread(out, nb, gl, c, l, h);
+  sponge_spread(out, nb, lane, c, l, h);
   if (nb == 0) return;
-  if (gl < 4) ((uint64_t *)node_hash(P, r))[gl] = ((uint64_t)h << 32) | l;
-  if (gl == 0) node_done(P, r, 32);
+  if (spread_digest_lane(lane)) ((uint64_t *)node_hash(P, r))[lane >> 3] = ((uint64_t)h << 32) | l;
+  if (lane == 0) node_done(P, r, 32);
 }
 
 __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries) {
@@ -1099,13 +1111,13 @@ __global__ __launch_bounds__(256) void k_kb_lane(const uint64_t *slot, const uin
 
 __global__ __launch_bounds__(256) void k_kb_spread(const uint64_t *slot, const uint32_t *len, uint32_t n,
                                                    const uint64_t *scratch, uint8_t *out) {
-  const uint32_t gl = __lane_id() & 31;
-  const uint32_t i = blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32;
+  const uint32_t lane = __lane_id();
+  const uint32_t i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   uint32_t nb = i < n ? keccak_blocks(len[i]) : 0;
-  SpreadLane c(gl);
+  SpreadLane c(lane);
   uint32_t l, h;
-  sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, gl, c, l, h);
-  if (nb && gl < 4) ((uint64_t *)(out + 32 * (size_t)i))[gl] = ((uint64_t)h << 32) | l;
+  sponge_spread(nb ? scratch + slot[i] / 8 : scratch, nb, lane, c, l, h);
+  if (nb && spread_digest_lane(lane)) ((uint64_t *)(out + 32 * (size_t)i))[lane >> 3] = ((uint64_t)h << 32) | l;
 }
 
 // ------------------------------------------------------------- node export --
@@ -1206,7 +1218,7 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
       hipError_t e = hipEventRecord(perm_begin, s);
       if (e != hipSuccess) return e;
     }
-    k_hc_small<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+    k_hc_small<<<(n + 3) / 4, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
     GMPT_LAUNCH_CHECK();
     if (perm_end) return hipEventRecord(perm_end, s);
     return hipSuccess;
@@ -1221,7 +1233,7 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     if (e != hipSuccess) return e;
   }
   if (n <= spread_max)
-    k_hc_perm_spread<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
+    k_hc_perm_spread<<<(n + 3) / 4, kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
   else
     k_hc_perm_lane<<<nblk(n), kB, 0, s>>>(P, list, slot, enc_len, n, scratch);
   GMPT_LAUNCH_CHECK();
@@ -1292,7 +1304,7 @@ hipError_t keccak_batch(const uint8_t *data, const uint64_t *off, const uint32_t
   k_kb_pad<<<nblk(n), kB, 0, s>>>(data, off, len, n, slot, scratch);
   GMPT_LAUNCH_CHECK();
   if (spread)
-    k_kb_spread<<<(n + 7) / 8, kB, 0, s>>>(slot, len, n, scratch, out);
+    k_kb_spread<<<(n + 3) / 4, kB, 0, s>>>(slot, len, n, scratch, out);
   else
     k_kb_lane<<<nblk(n), kB, 0, s>>>(slot, len, n, scratch, out);
   return hipGetLastError();
