@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.h"
+#include "phasehc.h"
 #include "phasenu.h"
 
 namespace gmpt {
@@ -131,6 +132,7 @@ class Forest {
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
   bool hc_fuse_leaves_ = true;   // leaves of <= 2 Keccak blocks: one encode+hash kernel
+  LeafRecords rec_;              // bins the last build recorded for its new leaves (in ws_)
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   hipEvent_t ev_meta_ = nullptr;               // the classification bins have reached the host

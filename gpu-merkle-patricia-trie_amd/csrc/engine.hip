@@ -45,6 +45,7 @@ size_t nu_ws_bytes(size_t n, bool tries) {
   size_t k = (size_t)st_levels_for(ne);
   size_t b = 0;
   b += n * (4 + 8 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
+  b += ne * (4 + 16);  // leaf records
   b += n * (4 * 7 + 1 + sizeof(Stop));
   b += ne * (4 + 4 + 4 + 4) + k * ne;
   b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
@@ -300,6 +301,7 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
 int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid, void **temp) {
   const uint32_t n = d.n;
   int r;
+  rec_ = LeafRecords();  // the workspace is about to be reused and the trie changed
   if ((r = ensure_ws(ws_, nu_ws_bytes(n, has_tid)))) return r;
   Carver c(ws_.p);
   memset(&a, 0, sizeof(a));
@@ -353,6 +355,8 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.stop2 = c.take<Stop>(n);
   a.e_src = c.take<uint32_t>(ne);
   a.e_grp = c.take<uint32_t>(ne);
+  a.leaf_pk = c.take<uint32_t>(ne);
+  a.leaf_meta = c.take<uint64_t>(2 * ne);
   a.st_levels = st_levels_for(ne);
   a.st_stride = align_up(ne, 256);
   a.st = c.take<uint8_t>(a.st_stride * (size_t)a.st_levels);
@@ -456,6 +460,11 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     if ((r = sync_counters())) return r;
     key_used_ += *(const uint64_t *)(cnt_.scal + SC_KTOT);
     key_used_ = align_up(key_used_, 8);
+    // the new leaves' hashing bins, valid until the next change to the trie
+    rec_.pk = a.leaf_pk;
+    rec_.meta = a.leaf_meta;
+    rec_.lo = cnt_.scal[SC_DBASE];
+    rec_.n = cnt_.scal[SC_NE];
   }
   if (targets) *targets = dirty_.p;
   if (n_targets) *n_targets = (int)cnt_.n_dirty;
@@ -494,7 +503,7 @@ int Forest::hash(void *targets, int n_targets) {
       lmeta_tmp = (uint64_t *)hc_lmeta_tmp_.p;
     }
     HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
-                        encl, ws2_.p, stream_, lmeta_tmp, lmeta));
+                        encl, ws2_.p, stream_, lmeta_tmp, lmeta, rec_));
     HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipEventRecord(ev_meta_, stream_));
     HIP_TRY(hipEventRecord(ev0_, stream_));
@@ -534,6 +543,7 @@ int Forest::hash(void *targets, int n_targets) {
     stats_.leaf_nodes = n_leaf;
   }
   HIP_TRY(hc_roots(pools(), ntries_, stream_));  // also empties the dirty list
+  rec_ = LeafRecords();
   if ((r = sync_counters())) return r;
   // the counters are zeroed by the classification, so only a pass that ran sets them
   stats_.perms = n ? cnt_.n_perms : 0;
@@ -594,6 +604,7 @@ int Forest::reset_trie(uint32_t trie) {
     set_error("trie id out of range");
     return -3;
   }
+  rec_ = LeafRecords();
   HIP_TRY(hipMemsetAsync((TrieHdr *)tries_.p + trie, 0, sizeof(TrieHdr), stream_));
   HIP_TRY(hipStreamSynchronize(stream_));
   return 0;
@@ -602,6 +613,7 @@ int Forest::reset_trie(uint32_t trie) {
 int Forest::clear() {
   int r;
   if ((r = set_device())) return r;
+  rec_ = LeafRecords();
   HIP_TRY(hc_clear(d_cnt_, (TrieHdr *)tries_.p, ntries_, stream_));
   memset(&cnt_, 0, sizeof(cnt_));
   key_used_ = 0;

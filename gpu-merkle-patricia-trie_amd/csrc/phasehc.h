@@ -27,9 +27,17 @@ size_t hc_temp_bytes(uint32_t n);
 // has at most 64 nibbles -- the first hist[0] + hist[1] sorted entries --
 // get no scratch slot but a metadata record lmeta[2i], lmeta[2i + 1] for
 // hc_hash_leaves_direct.
+// Bins and leaf metadata recorded by a build for dirty-list entries
+// [lo, lo + n): pk[k] = bin | blocks << 12 or kLeafPackedNone, meta[2k..] as
+// leaf_meta_words.  Valid only while nothing has changed those nodes since.
+struct LeafRecords {
+  const uint32_t *pk = nullptr;
+  const uint64_t *meta = nullptr;
+  uint32_t lo = 0, n = 0;
+};
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp = nullptr,
-                       uint64_t *lmeta = nullptr);
+                       uint64_t *lmeta = nullptr, LeafRecords rec = LeafRecords());
 // Encodes and hashes list[0..n) (one level: no node depends on another);
 // slot and enc_len are the level's slices of the classification arrays;
 // leaves = the leaf level (level 0: only leaf short nodes).  Events, when

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 
+#include "hc_leaf.h"
 #include "keccak.h"
 #include "prims.h"
 
@@ -44,20 +45,6 @@ __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x 
 inline unsigned nblk(size_t n) { return (unsigned)((n + kB - 1) / kB); }
 
 
-__device__ __forceinline__ uint32_t be_bytes(uint64_t v) {
-  uint32_t n = 1;
-  while (v >>= 8) n++;
-  return n;
-}
-// RLP header length for a string/list payload of L bytes.
-__device__ __forceinline__ uint32_t rlp_hdr_len(uint32_t L) { return L < 56 ? 1u : 1u + be_bytes(L); }
-
-// write_bytes_size (rlp.cuh:36-51); first = the first byte (only used when L == 1)
-__device__ __forceinline__ uint32_t rlp_item_len(uint32_t L, uint32_t first) {
-  if (L == 1 && first <= 0x7f) return 1;
-  return rlp_hdr_len(L) + L;
-}
-
 // header bytes packed little-endian into a u64 (first byte lowest)
 __device__ __forceinline__ uint64_t rlp_hdr(uint32_t L, uint32_t small, uint32_t large, uint32_t *n) {
   if (L < 56) {
@@ -71,8 +58,6 @@ __device__ __forceinline__ uint64_t rlp_hdr(uint32_t L, uint32_t small, uint32_t
   return h;
 }
 
-// Keccak-256 blocks (rate 136 bytes) of a message of len bytes plus padding.
-__device__ __forceinline__ uint32_t keccak_blocks(uint32_t len) { return len / 136 + 1; }
 
 // ---------------------------------------------------------- classification --
 
@@ -85,14 +70,11 @@ __device__ uint32_t estimate_blocks(const Pools &P, ref_t r, bool *leaf) {
   *leaf = false;
   if (t == T_SHORT) {
     const Short &S = P.shrt[i];
-    uint32_t kc = (S.key_len - S.term) / 2 + 1;
-    uint32_t pay = rlp_item_len(kc, 0);
     if (ref_type(S.child) == T_VALUE) {
       *leaf = true;
-      pay += rlp_item_len(P.value[ref_idx(S.child)].size, 0x80);
-    } else {
-      pay += 33;
+      return leaf_blocks((uint32_t)S.key_len - S.term, P.value[ref_idx(S.child)].size);
     }
+    const uint32_t pay = rlp_item_len(((uint32_t)S.key_len - S.term) / 2 + 1, 0) + 33;
     len = rlp_hdr_len(pay) + pay;
   } else {
     const Branch &B = P.branch[i];
@@ -127,20 +109,14 @@ __device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r, uint32_t *bl
   return level * kHcClasses + cls;
 }
 
-// Leaf metadata for the LDS leaf kernel, recorded by the classification in
-// sorted order so that kernel's loads do not chase list -> short -> value:
-//   m0 = byte offset of the key's first packed byte in the key arena (40 bits)
-//        | key_start & 1 << 40 | key nibbles without terminator << 41 (8 bits)
-//        | term << 49 | value size << 50 (14 bits)
-//   m1 = device address of the value bytes
-// Only leaves of at most two Keccak blocks get one (value size < 272).
+// Leaf metadata (leaf_meta_words) for the leaf-level kernel, recorded by the
+// classification in sorted order so that kernel's loads do not chase
+// list -> short -> value.  Only leaves of at most two Keccak blocks get one
+// (value size < 272).
 __device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) {
   const Short &S = P.shrt[ref_idx(r)];
   const Value &V = P.value[ref_idx(S.child)];
-  const uint64_t cnt = (uint32_t)S.key_len - S.term;
-  m[0] = (S.key_off + S.key_start / 2) | ((uint64_t)(S.key_start & 1) << 40) | (cnt << 41) |
-         ((uint64_t)S.term << 49) | ((uint64_t)min(V.size, 16383u) << 50);
-  m[1] = V.addr;
+  leaf_meta_words(S.key_off, S.key_start, (uint32_t)S.key_len - S.term, S.term, V.size, V.addr, m);
 }
 constexpr uint32_t kLdsLeafBins = 2;  // level 0, classes 0 and 1: leaves of one or two blocks
 constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
@@ -155,23 +131,36 @@ __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
 // in `packed` (bin | blocks << 12, blocks saturated at 2^20 - 1); counts are
 // aggregated per block in LDS so each block issues one global atomic per bin
 // it touches (a global atomic per node serialises on the few hot bins).
-constexpr uint32_t kPackedNbMax = (1u << 20) - 1;
 
 __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restrict__ list, uint32_t n,
                                                  uint32_t *__restrict__ hist, uint32_t *__restrict__ packed,
-                                                 uint64_t *__restrict__ lmeta_tmp) {
+                                                 uint64_t *__restrict__ lmeta_tmp, LeafRecords rec) {
   __shared__ uint32_t h[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) h[b] = 0;
   __syncthreads();
   for (uint32_t i = blockIdx.x * kHcTile + threadIdx.x; i < min(n, (blockIdx.x + 1) * kHcTile); i += blockDim.x) {
     ref_t r = list[i];
     uint32_t pk = 0xffffffffu;  // builds leave null holes
+    // a leaf the last build created: its bin and metadata were recorded then
+    const uint32_t ri = i - rec.lo;
+    const uint32_t rpk = rec.pk && ri < rec.n ? rec.pk[ri] : kLeafPackedNone;
     if (r != REF_NULL) {
       uint32_t nb;
-      uint32_t bin = bin_of(P, r, &nb);
+      uint32_t bin;
+      if (rpk != kLeafPackedNone) {
+        bin = rpk & 0xfff;
+        nb = rpk >> 12;
+      } else {
+        bin = bin_of(P, r, &nb);
+      }
       if (lmeta_tmp && bin < kLdsLeafBins) {
         uint64_t *m = lmeta_tmp + 2 * (size_t)i;
-        leaf_meta(P, r, m);
+        if (rpk != kLeafPackedNone) {
+          m[0] = rec.meta[2 * (size_t)ri];
+          m[1] = rec.meta[2 * (size_t)ri + 1];
+        } else {
+          leaf_meta(P, r, m);
+        }
         // a key remainder over the word builder's reach needs the generic
         // encoder and a scratch slot: bin it with the three-block leaves
         if (((m[0] >> 41) & 255u) > kLeafFastNibbles) bin = kLdsLeafBins;
@@ -1187,7 +1176,7 @@ size_t hc_temp_bytes(uint32_t n) { return scan_temp_bytes(n > kHcBins ? n : kHcB
 
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp,
-                       uint64_t *lmeta) {
+                       uint64_t *lmeta, LeafRecords rec) {
   k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins);
   GMPT_LAUNCH_CHECK();
   // the pass's work counters (permutations, bytes, leaf permutations, leaf bytes)
@@ -1198,7 +1187,7 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
     return hipGetLastError();
   }
   unsigned tiles = (unsigned)((n + kHcTile - 1) / kHcTile);
-  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist, packed, lmeta ? lmeta_tmp : nullptr);
+  k_hc_hist<<<tiles, kB, 0, s>>>(P, list, n, meta->hist, packed, lmeta ? lmeta_tmp : nullptr, rec);
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(meta->hist, cursor, kHcBins, &meta->nvalid, temp, s);
   if (e != hipSuccess) return e;

@@ -53,6 +53,8 @@ struct NuArgs {
   uint32_t *klen;
   uint64_t *kpoff;       // relative to kbase after the scan
   uint64_t *kref;        // kpoff | nibbles << 48: one load gives a key's place and length
+  uint32_t *leaf_pk;     // per element: the new leaf's hashing bin (hc_leaf.h), or kLeafPackedNone
+  uint64_t *leaf_meta;   // per element: the new leaf's metadata words (two)
   // sort
   uint64_t *sk0, *sk1;
   uint32_t *sv0, *sv1;

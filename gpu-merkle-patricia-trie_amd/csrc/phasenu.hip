@@ -14,6 +14,7 @@
 //      split joins its group as one more element ("virtual" element)
 // The result is the same trie the sequential insert produces.
 #include <limits.h>
+#include "hc_leaf.h"
 #include "phasenu.h"
 #include "prims.h"
 
@@ -959,7 +960,8 @@ __global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
 
 // Elements: new keys become a leaf short node + value node; a split short
 // node's remainder is re-attached below the new branch.
-__device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32_t idx) {
+__device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32_t idx, uint64_t *addr = nullptr,
+                                           uint32_t *size = nullptr) {
   Value &V = a.P.value[vid];
   int64_t vs = a.vidx[2 * idx], ve = a.vidx[2 * idx + 1];
   V.addr = a.vbase + (uint64_t)vs;
@@ -967,6 +969,8 @@ __device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32
   V.size = (uint32_t)(ve - vs + 1);
   V.pad = 0;
   V.seq = 0;
+  if (addr) *addr = V.addr;
+  if (size) *size = V.size;
 }
 
 __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
@@ -991,28 +995,38 @@ __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
     start = Lp + 1;
   }
   ref_t child;
+  uint64_t va = 0;
+  uint32_t vsz = 0;
   if (ek.virt) {
     child = a.P.shrt[ek.uidx].child;
   } else {
     uint32_t vid = a.base_value + e;
     if (vid >= a.P.cap_value) {
       raise_error(a.P.cnt, ERR_POOL, 3);
+      a.leaf_pk[e] = kLeafPackedNone;
       return;
     }
-    init_value(a, vid, a.u_idx[ek.uidx]);
+    init_value(a, vid, a.u_idx[ek.uidx], &va, &vsz);
     child = make_ref(T_VALUE, vid);
   }
   ref_t me = child, sh = REF_NULL;
+  uint32_t pk = kLeafPackedNone;
   if (start < ek.end) {
     uint32_t sid = a.base_short + e;
     if (sid >= a.P.cap_short) {
       raise_error(a.P.cnt, ERR_POOL, 2);
     } else {
-      init_short(a.P.shrt[sid], ek.off, ek.start + (start - ek.origin), ek.end - start, start, ek.term, child);
+      const uint32_t ks = ek.start + (start - ek.origin), kl = ek.end - start;
+      init_short(a.P.shrt[sid], ek.off, ks, kl, start, ek.term, child);
       sh = make_ref(T_SHORT, sid);
       me = sh;
+      if (!ek.virt) {  // a new leaf: record its hashing bin and metadata
+        pk = leaf_packed(kl - ek.term, vsz);
+        leaf_meta_words(ek.off, ks, kl - ek.term, ek.term, vsz, va, a.leaf_meta + 2 * (size_t)e);
+      }
     }
   }
+  a.leaf_pk[e] = pk;
   put_dirty(a, q.dbase + e, sh);
   if (is_root)
     install(a, gi, me);
