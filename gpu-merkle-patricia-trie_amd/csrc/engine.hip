@@ -458,8 +458,13 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     HIP_TRY(nu_emit(a, temp, stream_));
     HIP_TRY(nu_build(a, temp, stream_));
     if ((r = sync_counters())) return r;
-    key_used_ += *(const uint64_t *)(cnt_.scal + SC_KTOT);
-    key_used_ = align_up(key_used_, 8);
+    // the batch's packed keys stay only if a new node refers to them: a batch
+    // of updates alone (no elements) leaves them unused, so the arena keeps
+    // its place (keys of repeated updates do not accumulate)
+    if (cnt_.scal[SC_NE] != 0) {
+      key_used_ += *(const uint64_t *)(cnt_.scal + SC_KTOT);
+      key_used_ = align_up(key_used_, 8);
+    }
     // the new leaves' hashing bins, valid until the next change to the trie
     rec_.pk = a.leaf_pk;
     rec_.meta = a.leaf_meta;

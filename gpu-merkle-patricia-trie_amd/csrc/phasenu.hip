@@ -808,7 +808,7 @@ __global__ __launch_bounds__(256) void k_split_reps(NuArgs a, uint32_t ne_max) {
 //   leaf / remainder short of element e: base_short + e
 //   extension short of branch rank r   : base_short + NE + r
 //   value of new-key element e         : base_value + e
-//   value of update i (unique index)   : base_value + upd_off + i
+//   (an updated key's value node is rewritten in place)
 // Dirty-list entries: dbase + e (element short), dbase + NE + r (branch),
 // dbase + NE + R + r (extension); REF_NULL where no node was made.
 struct Slots {
@@ -1036,23 +1036,21 @@ __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
 
 // Existing keys: swap in a new value node (runs before the links, so a leaf
 // that is also split carries the new value into its remainder).
-__global__ __launch_bounds__(256) void k_updates(NuArgs a, uint32_t upd_off) {
+// The key's value node is rewritten in place (no node is allocated per
+// update; only this key refers to it).
+__global__ __launch_bounds__(256) void k_updates(NuArgs a) {
   uint32_t i = grid_tid();
   uint32_t m = a.scal[SC_M_UNIQ];
   if (i >= m) return;
   Stop st = a.stop[i];
   if (!is_update(st.kind)) return;
-  uint32_t vid = a.base_value + upd_off + i;
-  if (vid >= a.P.cap_value) {
-    raise_error(a.P.cnt, ERR_POOL, 3);
+  const ref_t vr = st.kind == STOP_LEAF_UPDATE ? a.P.shrt[ref_idx(st.stop)].child
+                                               : a.P.branch[ref_idx(st.stop)].child[16];
+  if (ref_type(vr) != T_VALUE) {
+    raise_error(a.P.cnt, ERR_INTERNAL, 13);
     return;
   }
-  init_value(a, vid, a.u_idx[i]);
-  ref_t vr = make_ref(T_VALUE, vid);
-  if (st.kind == STOP_LEAF_UPDATE)
-    a.P.shrt[ref_idx(st.stop)].child = vr;
-  else
-    a.P.branch[ref_idx(st.stop)].child[16] = vr;
+  init_value(a, ref_idx(vr), a.u_idx[i]);
 }
 
 // Records this build's slot bases before it starts and advances the pool and
@@ -1062,12 +1060,12 @@ __global__ void k_build_begin(NuArgs a, const uint32_t *nrep_total) {
   a.scal[SC_NREP] = *nrep_total;
   a.scal[SC_DBASE] = a.P.cnt->n_dirty;
 }
-__global__ void k_build_end(NuArgs a, uint32_t upd_off) {
+__global__ void k_build_end(NuArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t ne = a.scal[SC_NE], nrep = a.scal[SC_NREP], m = a.scal[SC_M_UNIQ];
+  uint32_t ne = a.scal[SC_NE], nrep = a.scal[SC_NREP];
   a.P.cnt->n_branch = a.base_branch + nrep;
   a.P.cnt->n_short = a.base_short + ne + nrep;
-  a.P.cnt->n_value = a.base_value + upd_off + m;
+  a.P.cnt->n_value = a.base_value + ne;
   a.P.cnt->n_dirty = a.scal[SC_DBASE] + ne + 2 * nrep;
 }
 
@@ -1255,7 +1253,7 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
     k_st_levels<<<nblk(ne_max), kB, 0, s>>>(a, k - 1, cnt);
     GMPT_LAUNCH_CHECK();
   }
-  k_updates<<<nblk(a.n), kB, 0, s>>>(a, ne_max);
+  k_updates<<<nblk(a.n), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a, ne_max);
   GMPT_LAUNCH_CHECK();
@@ -1269,7 +1267,7 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
   GMPT_LAUNCH_CHECK();
   k_link_elems<<<nblk(ne_max), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
-  k_build_end<<<1, 64, 0, s>>>(a, ne_max);
+  k_build_end<<<1, 64, 0, s>>>(a);
   return hipGetLastError();
 }
 

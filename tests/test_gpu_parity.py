@@ -159,6 +159,44 @@ def test_incremental_batches(gmpt, workloads, oracle, modes):
     assert t.get_values(sample) == o.gets(sample)
 
 
+def test_repeated_updates_rewrite_values_in_place(gmpt, workloads, oracle):
+    """PhaseNU update-only batches rewrite the keys' value nodes in place and
+    keep no packed keys: across five rounds of updates the pools grow by the
+    staged value bytes only, and every root and value matches the oracle."""
+    rng = np.random.default_rng(61)
+    base = _rand_batch(workloads, 61, 4000, 30, 90)
+    t = gmpt.MPT(device=0)
+    o = oracle.OracleMPT()
+    _put(t, base, "2phase")
+    o.puts(base)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    used0 = t.stats()["pool_bytes_used"]
+    staged = 0
+    for r in range(5):
+        flat, lens = workloads.random_values(rng, base.n, 20, 140)
+        upd = workloads.batch_from_hex([base.key(i) for i in range(base.n)],
+                                       [flat[int(lens[:i].sum()):int(lens[:i + 1].sum())].tobytes()
+                                        for i in range(base.n)])
+        upd = upd.subset(rng.permutation(upd.n))
+        _put(t, upd, "2phase")
+        o.puts(upd)
+        o.hash()
+        assert t.get_root_hash() == o.root(), r
+        staged += int(lens.sum()) + 8 + 256  # one staged value buffer per batch (8-byte slack, 256-byte aligned)
+    grown = t.stats()["pool_bytes_used"] - used0
+    assert grown <= staged, (grown, staged)
+    sample = base.subset(range(0, base.n, 13))
+    assert t.get_values(sample) == o.gets(sample)
+    # new keys after the updates still land next to the kept keys
+    fresh = _rand_batch(workloads, 62, 3000, 30, 90)
+    _put(t, fresh, "2phase")
+    o.puts(fresh)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    assert t.get_values(fresh) == o.gets(fresh)
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_many_small_batches_split_shorts(gmpt, workloads, oracle, mode):
     # keys sharing long prefixes split existing short nodes at every depth
