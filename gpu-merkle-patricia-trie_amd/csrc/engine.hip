@@ -524,22 +524,24 @@ int Forest::hash(void *targets, int n_targets) {
     const HcMeta &M = *(const HcMeta *)h_meta_;
     if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
     uint64_t *scratch = (uint64_t *)hc_scratch_.p;
-    const uint32_t n_direct = lmeta ? M.hist[0] + M.hist[1] : 0u;
+    uint32_t n_direct = 0;  // the leaves of level 0, all hashed by the direct kernel
+    if (lmeta)
+      for (uint32_t c = 0; c < kHcClasses; c++) n_direct += M.hist[c];
     uint32_t start = 0, n_leaf = 0;
     for (uint32_t lv = 0; lv < kHcLevels; lv++) {
       uint32_t cnt = 0;
       for (uint32_t c = 0; c < kHcClasses; c++) cnt += M.hist[lv * kHcClasses + c];
       if (cnt) {
-        const bool leaves = lv == 0;
-        const uint32_t skip = leaves ? n_direct : 0u;  // already queued
-        const bool timed = leaves && !lmeta;
+        const bool leaves = lv <= 1;  // level 1: leaves the direct kernel does not take
+        const uint32_t skip = lv == 0 ? n_direct : 0u;  // already queued
+        const bool timed = lv == 0 && !lmeta;
         if (cnt > skip)
           HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
                                 encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
                                 timed ? ev3_ : nullptr));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
-        if (leaves) n_leaf = cnt;
+        if (leaves) n_leaf += cnt;
       }
       start += cnt;
     }

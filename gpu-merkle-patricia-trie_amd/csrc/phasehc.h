@@ -23,10 +23,11 @@ size_t hc_temp_bytes(uint32_t n);
 // Bins dirty nodes by (level, block class): out = the list ordered by bin,
 // slot[i] = byte offset of sorted node i's message slot in the scratch buffer;
 // packed (n entries) is workspace.  With lmeta (2n words, lmeta_tmp 2n words
-// of workspace) the leaves of one or two Keccak blocks whose key remainder
-// has at most 64 nibbles -- the first hist[0] + hist[1] sorted entries --
-// get no scratch slot but a metadata record lmeta[2i], lmeta[2i + 1] for
-// hc_hash_leaves_direct.
+// of workspace) the leaves whose key remainder has at most 64 nibbles and
+// whose value is under 16,383 bytes -- level 0, the first hist[0 .. 8)
+// sorted entries -- get no scratch slot but a metadata record lmeta[2i],
+// lmeta[2i + 1] for hc_hash_leaves_direct; other leaves are binned at
+// level 1 (no inner node can be there).
 // Bins and leaf metadata recorded by a build for dirty-list entries
 // [lo, lo + n): pk[k] = bin | blocks << 12 or kLeafPackedNone, meta[2k..] as
 // leaf_meta_words.  Valid only while nothing has changed those nodes since.
@@ -45,10 +46,10 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s,
                          hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr);
-// Encodes and hashes the classification's first meta->hist[0] + hist[1]
-// sorted entries (leaves with metadata, see hc_classify) in one kernel; the
-// count is read on the device, n_upper (>= it) sizes the launch, so the
-// launch need not wait for the bins to reach the host.
+// Encodes and hashes the classification's level-0 entries (leaves with
+// metadata, see hc_classify; their count is the sum of meta->hist[0 .. 8))
+// in one kernel; the count is read on the device, n_upper (>= it) sizes the
+// launch, so the launch need not wait for the bins to reach the host.
 hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64_t *lmeta, const HcMeta *meta,
                                  uint32_t n_upper, uint32_t *enc_len, hipStream_t s);
 // Adds the pass's permutations and encoding bytes (from the enc_len array of

@@ -118,8 +118,13 @@ __device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) 
   const Value &V = P.value[ref_idx(S.child)];
   leaf_meta_words(S.key_off, S.key_start, (uint32_t)S.key_len - S.term, S.term, V.size, V.addr, m);
 }
-constexpr uint32_t kLdsLeafBins = 2;  // level 0, classes 0 and 1: leaves of one or two blocks
-constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
+// Level 0 holds the leaves k_hc_leaf_direct hashes (key remainder <= 64
+// nibbles, value < 16,383 bytes); other leaves go to level 1, which no inner
+// node uses (a node at nibble depth 255 cannot exist), and take the encode +
+// scratch + permutation kernels.
+constexpr uint32_t kLdsLeafBins = kHcClasses;  // the bins of level 0
+constexpr uint32_t kLeafFastNibbles = 64;      // key nibbles a leaf head holds in registers
+constexpr uint32_t kLeafDirectMaxValue = 16382; // value bytes the metadata holds (14 bits, saturated above)
 
 __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
   uint32_t i = grid_tid();
@@ -161,9 +166,9 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
         } else {
           leaf_meta(P, r, m);
         }
-        // a key remainder over the word builder's reach needs the generic
-        // encoder and a scratch slot: bin it with the three-block leaves
-        if (((m[0] >> 41) & 255u) > kLeafFastNibbles) bin = kLdsLeafBins;
+        // a key remainder or value over the word builder's reach needs the
+        // generic encoder and a scratch slot: level 1, same class
+        if (((m[0] >> 41) & 255u) > kLeafFastNibbles || (m[0] >> 50) > kLeafDirectMaxValue) bin += kHcClasses;
       }
       atomicAdd(&h[bin], 1u);
       pk = bin | (min(nb, kPackedNbMax) << 12);
@@ -811,7 +816,7 @@ __device__ __forceinline__ LeafLayout leaf_layout(uint64_t m0, uint32_t vb0) {
   l.payload = key_item + (l.raw ? 1u : rlp_hdr_len(l.vsz) + l.vsz);
   l.enc = rlp_hdr_len(l.payload) + l.payload;
   l.nb = keccak_blocks(l.enc);
-  l.fast = l.cnt <= kLeafFastNibbles && l.enc >= 32 && l.nb <= 2;
+  l.fast = l.cnt <= kLeafFastNibbles && l.enc >= 32 && l.vsz <= kLeafDirectMaxValue;
   return l;
 }
 
@@ -850,7 +855,9 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
                                                         uint32_t *__restrict__ enc_len) {
   __shared__ uint64_t head[256][kLeafDirectHead];
   const uint32_t i = grid_tid();
-  const uint32_t n = hist[0] + hist[1];  // leaves of one and two blocks, sorted first
+  uint32_t n = 0;  // the leaves of level 0, sorted first
+#pragma unroll
+  for (uint32_t c = 0; c < kLdsLeafBins; c++) n += hist[c];
   if (i >= n) return;  // no block barrier below
   const ref_t r = list[i];
   const uint64_t m0 = lmeta[2 * (size_t)i], va = lmeta[2 * (size_t)i + 1];
@@ -894,33 +901,31 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
   uint64_t st[25];
 #pragma unroll
   for (int q = 0; q < 25; q++) st[q] = 0;
+#pragma unroll 1
+  for (uint32_t b = 0; b < nb; b++) {
+    {
+      uint64_t W[18];
 #pragma unroll
-  for (uint32_t b = 0; b < 2; b++) {
-    if (b < nb) {
-      {
-        uint64_t W[18];
-#pragma unroll
-        for (int q = 0; q < 18; q++) {
-          const int32_t j = clamp_i(jbase + (int32_t)(17 * b) + q, 0, J);  // clamped into the value
-          W[q] = hasv ? gload(V0 + 8 * (uint64_t)(uint32_t)j) : 0ull;
-        }
-#pragma unroll
-        for (int q = 0; q < 17; q++) {
-          const int32_t w = 17 * (int32_t)b + q;
-          // value bytes: message bytes [max(vstart, 8w), min(enc, 8w + 8)) of this word
-          const int32_t lo_b = clamp_i((int32_t)vstart - 8 * w, 0, 8);
-          const int32_t hi_b = clamp_i((int32_t)enc - 8 * w, 0, 8);
-          const uint64_t f = funnel_bytes(W[q], W[q + 1], sh);
-          const uint64_t mk = hasv && hi_b > lo_b ? (~0ull << (8 * lo_b)) & (~0ull >> (64 - 8 * hi_b)) : 0ull;
-          uint64_t v = f & mk;
-          if (w < (int32_t)kLeafHeadWords) v |= hw[w];
-          if ((int32_t)(enc >> 3) == w) v |= 0x01ull << (8 * (enc & 7));  // Keccak padding
-          if (q == 16 && b + 1 == nb) v |= 0x80ull << 56;
-          st[q] ^= v;
-        }
+      for (int q = 0; q < 18; q++) {
+        const int32_t j = clamp_i(jbase + (int32_t)(17 * b) + q, 0, J);  // clamped into the value
+        W[q] = hasv ? gload(V0 + 8 * (uint64_t)(uint32_t)j) : 0ull;
       }
-      keccak_f1600(st);
+#pragma unroll
+      for (int q = 0; q < 17; q++) {
+        const int32_t w = 17 * (int32_t)b + q;
+        // value bytes: message bytes [max(vstart, 8w), min(enc, 8w + 8)) of this word
+        const int32_t lo_b = clamp_i((int32_t)vstart - 8 * w, 0, 8);
+        const int32_t hi_b = clamp_i((int32_t)enc - 8 * w, 0, 8);
+        const uint64_t f = funnel_bytes(W[q], W[q + 1], sh);
+        const uint64_t mk = hasv && hi_b > lo_b ? (~0ull << (8 * lo_b)) & (~0ull >> (64 - 8 * hi_b)) : 0ull;
+        uint64_t v = f & mk;
+        if (q < (int)kLeafHeadWords && b == 0) v |= hw[q];  // the head lies in the first block
+        if ((int32_t)(enc >> 3) == w) v |= 0x01ull << (8 * (enc & 7));  // Keccak padding
+        if (q == 16 && b + 1 == nb) v |= 0x80ull << 56;
+        st[q] ^= v;
+      }
     }
+    keccak_f1600(st);
   }
   uint64_t *hp = (uint64_t *)node_hash(P, r);
   hp[0] = st[0];
