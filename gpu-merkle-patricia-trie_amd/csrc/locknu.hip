@@ -37,8 +37,10 @@ inline unsigned nblk(size_t n) { return (unsigned)((n + kB - 1) / kB); }
 // agent-scope load.  Such loads read memory in the order they return, so the
 // optimistic read protocol only needs each load of a step to have returned
 // before the next step's loads issue: load_order() waits for outstanding
-// loads.  An acquire load would also invalidate the CU's caches at every
-// version check.
+// loads (a version read, then the contents it guards, then the re-check).
+// An acquire load would also invalidate the CU's caches at every version
+// check.  Key bytes in the arena never change while the insert runs and are
+// read with plain loads.
 template <typename T>
 __device__ __forceinline__ T ld(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -48,66 +50,111 @@ __device__ __forceinline__ void load_order() {
   __builtin_amdgcn_s_waitcnt(0);
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
-// version read before a node's contents
-__device__ __forceinline__ uint64_t ver_read(uint64_t *l) {
-  uint64_t v = ld(l);
-  load_order();
-  return v;
+// Writes that other lanes may read while the insert runs are agent-scope
+// stores too: they are written through to the point of coherence, so once a
+// lane's stores have completed (store_order() waits for them) every other
+// lane's coherent loads see them.  Publishing a node therefore takes: the
+// node's stores, store_order(), the reference store, store_order(), the
+// unlock.  Release and acquire orderings would instead write back and
+// invalidate the whole L2 of the CU's die at every latch.
+template <typename T, typename U>
+__device__ __forceinline__ void st(T *p, U v) {
+  __hip_atomic_store(p, (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// version re-check after the contents were read
-__device__ __forceinline__ bool ver_check(uint64_t *l, uint64_t v) {
-  load_order();
-  return ld(l) == v;
-}
+__device__ __forceinline__ void store_order() { load_order(); }
+// The latch holder reads nothing it did not validate before the upgrade, and
+// stores only after the upgrade returned, so the upgrade needs no ordering.
 __device__ __forceinline__ bool ver_upgrade(uint64_t *l, uint64_t v) {
   uint64_t exp = v;
-  return __hip_atomic_compare_exchange_strong(l, &exp, v + 2, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+  return __hip_atomic_compare_exchange_strong(l, &exp, v + 2, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
 }
+// Callers run store_order() first, so the latch's writes are visible before
+// the new version is.
 __device__ __forceinline__ void ver_unlock(uint64_t *l) {
-  __hip_atomic_fetch_add(l, (uint64_t)2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(l, (uint64_t)2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void ver_unlock_obsolete(uint64_t *l) {
-  __hip_atomic_fetch_add(l, (uint64_t)3, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(l, (uint64_t)3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-struct KeyRef {
-  const uint8_t *p;
-  uint32_t len;
-  __device__ __forceinline__ uint32_t nib(uint32_t j) const { return j + 1 == len ? kNibbleT : packed_nibble(p, j); }
-};
+// Packed key bytes as big-endian nibble words: 16 nibbles from nibble s of
+// the key at p (8-byte aligned), first nibble in bits 63..60.  Reads only the
+// aligned 8-byte words that hold nibbles below lim (the caller's bound on
+// valid nibbles, s < lim), so every read stays inside the key's slot.
+__device__ __forceinline__ uint64_t nib_word(const uint8_t *p, uint32_t s, uint32_t lim) {
+  const uint32_t wi = s >> 4, sh = (s & 15u) * 4u;
+  const uint64_t *w = (const uint64_t *)p;
+  uint64_t hi = __builtin_bswap64(w[wi]);
+  if (sh == 0) return hi;
+  uint64_t lo = 16u * (wi + 1) < lim ? __builtin_bswap64(w[wi + 1]) : 0ull;
+  return (hi << sh) | (lo >> (64u - sh));
+}
 
 struct Ctx {
-  KeyRef k;
-  uint32_t i;       // batch position
+  const uint8_t *kp;  // packed nibbles of the key (8-byte aligned)
+  uint32_t len;       // hex length, terminator included
+  uint32_t i;         // batch position
   uint32_t trie;
   uint64_t seq;
-  uint64_t key_off; // this key's packed bytes in the key arena
-  uint64_t key_cap; // key arena bytes in use (bounds for optimistic reads)
+  uint64_t key_off;   // this key's packed bytes in the key arena
+  uint64_t key_cap;   // key arena bytes in use (bounds for optimistic reads)
+  uint32_t mark_from; // first node start position this lane marks (see mark())
+  // the packed byte holding nibble j, loaded ahead of its use (0 when nibble
+  // j is the terminator or past it)
+  __device__ __forceinline__ uint32_t byte_at(uint32_t j) const { return j + 1 < len ? kp[j >> 1] : 0u; }
+  __device__ __forceinline__ uint32_t nib(uint32_t j, uint32_t b) const {
+    return j + 1 == len ? kNibbleT : (j & 1) ? (b & 15u) : (b >> 4);
+  }
+  __device__ __forceinline__ uint32_t nib(uint32_t j) const {
+    return j + 1 == len ? kNibbleT : packed_nibble(kp, j);
+  }
 };
 
 __device__ __forceinline__ void init_value(const NuArgs &a, const Ctx &c, uint32_t vid) {
-  Value &V = a.P.value[vid];
-  int64_t vs = a.vidx[2 * c.i], ve = a.vidx[2 * c.i + 1];
-  V.addr = a.vbase + (uint64_t)vs;
-  V.hp = a.hps ? a.hps[c.i] : 0ull;
-  V.size = (uint32_t)(ve - vs + 1);
-  V.pad = 0;
-  V.seq = c.seq;
+  uint64_t *w = (uint64_t *)&a.P.value[vid];
+  const int64_t vs = a.vidx[2 * c.i], ve = a.vidx[2 * c.i + 1];
+  st(w + 0, a.vbase + (uint64_t)vs);                  // addr
+  st(w + 1, a.hps ? (uint64_t)a.hps[c.i] : 0ull);     // hp
+  st(w + 2, (uint64_t)(uint32_t)(ve - vs + 1));       // size, pad
+  st(w + 3, c.seq);                                   // seq
 }
 
+// A fresh short node, written as its eight words (layout of Short).
 __device__ __forceinline__ void init_short(Short &S, uint64_t key_off, uint32_t key_start, uint32_t key_len,
-                                           uint32_t depth, uint8_t term, ref_t child) {
+                                           uint32_t depth, uint32_t term, ref_t child) {
   uint64_t *w = (uint64_t *)&S;
 #pragma unroll
-  for (int q = 0; q < 8; q++) w[q] = 0;
-  S.key_off = key_off;
-  S.child = child;
-  S.key_start = (uint16_t)key_start;
-  S.key_len = (uint16_t)key_len;
-  S.depth = (uint16_t)depth;
-  S.term = term;
-  S.dirty = 1;
+  for (int q = 0; q < 4; q++) st(w + q, 0ull);        // hash
+  st(w + 4, key_off);
+  st(w + 5, (uint64_t)child | (uint64_t)key_start << 32 | (uint64_t)key_len << 48);
+  st(w + 6, (uint64_t)depth | (uint64_t)term << 24 | 1ull << 32);  // hash_size 0, dirty 1
+  st(w + 7, 0ull);                                    // lock
+}
+
+// A fresh branch at depth d with two children, written as its sixteen words
+// (layout of Branch).
+__device__ __forceinline__ void init_branch(Branch &B, uint32_t d, uint32_t s0, ref_t r0, uint32_t s1, ref_t r1) {
+  uint64_t *w = (uint64_t *)&B;
+#pragma unroll
+  for (int q = 0; q < 4; q++) st(w + q, 0ull);        // hash
+#pragma unroll
+  for (uint32_t q = 0; q < 9; q++) {                  // child[2q], child[2q + 1] (child[16], depth)
+    const uint32_t lo = 2 * q, hi = 2 * q + 1;
+    uint64_t x = (uint64_t)(lo == s0 ? r0 : lo == s1 ? r1 : 0u);
+    x |= q < 8 ? (uint64_t)(hi == s0 ? r0 : hi == s1 ? r1 : 0u) << 32 : (uint64_t)d << 32;
+    st(w + 4 + q, x);
+  }
+  st(w + 13, 1ull);                                   // dirty 1
+  st(w + 14, 0ull);                                   // lock
+  st(w + 15, 0ull);
+}
+
+// Publishes a node reference once the node's own stores have completed.
+__device__ __forceinline__ void publish(ref_t *slot, ref_t r) {
+  store_order();
+  st(slot, r);
+  store_order();
 }
 
 // New leaf for the key's nibbles [start, len): a short node over a value node,
@@ -116,182 +163,235 @@ __device__ __forceinline__ ref_t make_leaf(const NuArgs &a, const Ctx &c, uint32
   uint32_t vid = a.base_value + c.i;
   init_value(a, c, vid);
   ref_t v = make_ref(T_VALUE, vid);
-  *made_short = start < c.k.len;
+  *made_short = start < c.len;
   if (!*made_short) return v;
   uint32_t sid = a.base_short + 2 * c.i;
-  init_short(a.P.shrt[sid], c.key_off, start, c.k.len - start, start, 1, v);
+  init_short(a.P.shrt[sid], c.key_off, start, c.len - start, start, 1, v);
   return make_ref(T_SHORT, sid);
 }
 
-enum { ATT_DONE = 0, ATT_RESTART = 1, ATT_FAIL = 2 };
+enum { ATT_DONE = 0, ATT_RESTART = 1 };
 
-// One optimistic attempt.  Returns ATT_DONE, or ATT_RESTART when a version
-// check or latch upgrade failed (nothing was modified).  Path nodes are marked
-// dirty on the way down; the nodes a finished insert modified or created are
-// reported through mk for the caller to mark.
-struct Marks {
-  ref_t existing[2];   // existing nodes whose encoding changes
-  ref_t fresh[3];      // new short/branch nodes
-  int ne, nf;
-};
+// Dirty marks without atomics.  The keys are sorted and unique, so the lanes
+// whose keys reach a node (every key with the node's nibble prefix) form one
+// contiguous range of lanes; the first of them is the one lane whose key
+// shares fewer nibbles with its predecessor's than the node's start position
+// (the first key of a trie shares none).  That lane alone marks the node: a
+// node flagged clean is flagged and staged for re-hashing.  Nodes this insert
+// creates are born flagged and are staged by their creator.
+__device__ __forceinline__ void mark(const Pools &P, const Ctx &c, DirtyStage &ds, ref_t node, uint32_t flag,
+                                     uint32_t pos) {
+  if (flag != 0 || pos < c.mark_from) return;
+  __hip_atomic_store(node_dirty_flag(P, node), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  ds.push(P, node);
+}
 
-__device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk, DirtyStage &ds) {
+// Issues the loads of a node's version word (and a branch's dirty flag) for
+// the next synchronisation point; other references get version 0.
+__device__ __forceinline__ void head_issue(const Pools &P, ref_t r, uint64_t &v, uint32_t &flag) {
+  const uint32_t ty = ref_type(r), idx = ref_idx(r);
+  v = 0;
+  flag = 1;
+  if (ty == T_BRANCH && idx < P.cap_branch) {
+    v = ld(&P.branch[idx].lock);
+    flag = ld(&P.branch[idx].dirty);
+  } else if (ty == T_SHORT && idx < P.cap_short) {
+    v = ld(&P.shrt[idx].lock);
+  }
+}
+
+// One optimistic attempt (optimistic lock coupling).  Returns ATT_DONE, or
+// ATT_RESTART when a version check or latch upgrade failed (nothing was
+// modified).  Each step costs two waits for memory: the node's contents, then
+// the node's version re-check together with the child's version (the child
+// reference was read under the node's version, so the re-check also covers
+// it).  Nodes are never reused while the insert runs and a replaced node is
+// left obsolete (bit 0), so a stale child reference always fails its own
+// version check.  Path nodes are marked on the way down (a mark left by an
+// abandoned attempt only costs a redundant re-hash).
+__device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage &ds) {
   const Pools &P = a.P;
-  mk.ne = mk.nf = 0;
   TrieHdr *H = &P.tries[c.trie];
   uint64_t *plock = &H->lock;
-  uint64_t pver = ver_read(plock);
+  uint64_t pver = ld(plock);
+  load_order();
   if (pver & 3) return ATT_RESTART;
   ref_t *pslot = &H->root;
   ref_t parent = REF_NULL;
   ref_t node = ld(pslot);
-  if (!ver_check(plock, pver)) return ATT_RESTART;
-  uint32_t pos = 0;
+  load_order();
+  uint64_t v;
+  uint32_t flag;
+  uint32_t pos = 0, kb;  // kb: the key byte holding nibble pos
+  {
+    uint64_t hv = ld(plock);
+    head_issue(P, node, v, flag);
+    kb = c.byte_at(0);
+    load_order();
+    if (hv != pver || (v & 3)) return ATT_RESTART;
+  }
   for (int guard = 0; guard < 2 * kMaxKeyNibbles + 4; guard++) {
-    uint32_t ty = ref_type(node), idx = ref_idx(node);
+    const uint32_t ty = ref_type(node), idx = ref_idx(node);
     if (node == REF_NULL) {
       if (!ver_upgrade(plock, pver)) return ATT_RESTART;
       bool ms;
       ref_t leaf = make_leaf(a, c, pos, &ms);
-      __hip_atomic_store(pslot, leaf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      publish(pslot, leaf);
       ver_unlock(plock);
-      if (ms) mk.fresh[mk.nf++] = leaf;
-      if (parent != REF_NULL) mk.existing[mk.ne++] = parent;
+      if (ms) ds.push(P, leaf);
       return ATT_DONE;
     }
     if (ty == T_BRANCH) {
-      if (idx >= P.cap_branch) return ATT_RESTART;
+      if (idx >= P.cap_branch || pos >= c.len) return ATT_RESTART;
       Branch *B = &P.branch[idx];
-      uint64_t v = ver_read(&B->lock);
-      if (v & 3) return ATT_RESTART;
-      if (!ver_check(plock, pver)) return ATT_RESTART;
-      if (pos >= c.k.len) return ATT_RESTART;
-      uint32_t nib = c.k.nib(pos);
-      ref_t child = ld(&B->child[nib]);
-      if (!ver_check(&B->lock, v)) return ATT_RESTART;
-      // on the key's path: re-hash (a mark left by an abandoned attempt only
-      // costs a redundant re-hash)
-      ds.mark(P, node);
+      const uint32_t nib = c.nib(pos, kb);
+      const ref_t child = ld(&B->child[nib]);
+      load_order();
+      uint64_t cv;
+      uint32_t cflag;
+      const uint64_t bv = ld(&B->lock);
+      head_issue(P, child, cv, cflag);
+      kb = c.byte_at(pos + 1);
+      load_order();
+      if (bv != v) return ATT_RESTART;
+      mark(P, c, ds, node, flag, pos);
+      if (cv & 3) return ATT_RESTART;
       parent = node;
       plock = &B->lock;
       pver = v;
       pslot = &B->child[nib];
       node = child;
+      v = cv;
+      flag = cflag;
       pos += 1;
       continue;
     }
     if (ty == T_SHORT) {
       if (idx >= P.cap_short) return ATT_RESTART;
       Short *S = &P.shrt[idx];
-      uint64_t v = ver_read(&S->lock);
-      if (v & 3) return ATT_RESTART;
-      if (!ver_check(plock, pver)) return ATT_RESTART;
-      // fields read as the two 64-bit words that hold them (offsets 40, 48)
-      uint64_t koff = ld(&S->key_off);
-      uint64_t w40 = ld((const uint64_t *)((const uint8_t *)S + 40));
-      uint64_t w48 = ld((const uint64_t *)((const uint8_t *)S + 48));
-      ref_t child = (ref_t)(w40 & 0xffffffffu);
-      uint32_t kstart = (uint32_t)((w40 >> 32) & 0xffffu), klen = (uint32_t)(w40 >> 48);
-      uint32_t depth = (uint32_t)(w48 & 0xffffu), term = (uint32_t)((w48 >> 24) & 0xffu);
-      if (klen == 0 || klen > kMaxKeyNibbles || depth != pos || koff + (kstart + klen) / 2 + 1 > c.key_cap)
+      // fields read as the words that hold them (offsets 32, 40, 48)
+      const uint64_t koff = ld(&S->key_off);
+      const uint64_t w40 = ld((const uint64_t *)((const uint8_t *)S + 40));
+      const uint64_t w48 = ld((const uint64_t *)((const uint8_t *)S + 48));
+      load_order();
+      const ref_t child = (ref_t)(w40 & 0xffffffffu);
+      const uint32_t kstart = (uint32_t)((w40 >> 32) & 0xffffu), klen = (uint32_t)(w40 >> 48);
+      const uint32_t depth = (uint32_t)(w48 & 0xffffu), term = (uint32_t)((w48 >> 24) & 0xffu);
+      const uint32_t sflag = (uint32_t)(w48 >> 32);
+      if (klen == 0 || klen > kMaxKeyNibbles || depth != pos || term > 1 || pos >= c.len || koff > c.key_cap ||
+          (uint64_t)(kstart + klen) / 2 + 1 > c.key_cap - koff)
         return ATT_RESTART;
-      const uint8_t *sk = P.keys + koff;
+      const uint8_t *sk = P.keys + (koff & ~7ull);
+      // match the node's key against the key from pos, 16 nibbles per load
+      const uint32_t sreal = klen - term, kleft = c.len - 1 - pos;
+      const uint32_t mreal = min(sreal, kleft);
       uint32_t p = 0;
-      while (p < klen && pos + p < c.k.len) {
-        uint32_t sn = (term && p + 1 == klen) ? kNibbleT : packed_nibble(sk, kstart + p);
-        if (sn != c.k.nib(pos + p)) break;
-        p++;
+      while (p < mreal) {
+        uint64_t x = nib_word(sk, kstart + p, kstart + sreal) ^ nib_word(c.kp, pos + p, c.len - 1);
+        const uint32_t cnt = min(16u, mreal - p);
+        if (cnt < 16) x &= ~0ull << (64u - 4u * cnt);
+        if (x) {
+          p += (uint32_t)__clzll((long long)x) >> 2;
+          break;
+        }
+        p += cnt;
       }
-      if (!ver_check(&S->lock, v)) return ATT_RESTART;
+      if (p == mreal && p == sreal && term && p == kleft) p++;  // both at the terminator
       if (p == klen) {
         if (ref_type(child) == T_VALUE) {
           // the key exists: swap the value if this write is later.  A lane
           // whose write is already superseded finishes without the latch.
-          uint32_t cv = ref_idx(child);
-          if (cv < P.cap_value) {
-            uint64_t cur = ld(&P.value[cv].seq);
-            if (!ver_check(&S->lock, v)) return ATT_RESTART;
-            if (cur >= c.seq) return ATT_DONE;
+          const uint32_t cv = ref_idx(child);
+          uint64_t cur = 0;
+          const uint64_t sv = ld(&S->lock);
+          if (cv < P.cap_value) cur = ld(&P.value[cv].seq);
+          load_order();
+          if (sv != v) return ATT_RESTART;
+          if (cv < P.cap_value && cur >= c.seq) {
+            mark(P, c, ds, node, sflag, pos);
+            return ATT_DONE;
           }
           if (!ver_upgrade(&S->lock, v)) return ATT_RESTART;
-          uint32_t vi = ref_idx(child);
-          bool newer = vi >= P.cap_value || P.value[vi].seq < c.seq;
-          if (newer) {
-            uint32_t vid = a.base_value + c.i;
+          if (cv >= P.cap_value || ld(&P.value[cv].seq) < c.seq) {
+            const uint32_t vid = a.base_value + c.i;
             init_value(a, c, vid);
-            __hip_atomic_store(&S->child, make_ref(T_VALUE, vid), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            publish(&S->child, make_ref(T_VALUE, vid));
           }
           ver_unlock(&S->lock);
-          mk.existing[mk.ne++] = node;
+          mark(P, c, ds, node, sflag, pos);
           return ATT_DONE;
         }
-        ds.mark(P, node);
+        uint64_t cv;
+        uint32_t cflag;
+        const uint64_t sv = ld(&S->lock);
+        head_issue(P, child, cv, cflag);
+        kb = c.byte_at(pos + klen);
+        load_order();
+        if (sv != v) return ATT_RESTART;
+        mark(P, c, ds, node, sflag, pos);
+        if (cv & 3) return ATT_RESTART;
         parent = node;
         plock = &S->lock;
         pver = v;
         pslot = &S->child;
         node = child;
+        v = cv;
+        flag = cflag;
         pos += klen;
         continue;
       }
-      if (pos + p >= c.k.len) return ATT_RESTART;
-      // split S at p: a new branch at depth pos + p
-      uint32_t d = pos + p;
-      uint32_t snib = (term && p + 1 == klen) ? kNibbleT : packed_nibble(sk, kstart + p);
-      uint32_t knib = c.k.nib(d);
+      if (pos + p >= c.len) return ATT_RESTART;
+      // split S at p: a new branch at depth pos + p (the latch upgrades
+      // validate everything read above)
+      const uint32_t d = pos + p;
+      const uint32_t snib = (term && p + 1 == klen) ? kNibbleT : packed_nibble(sk, kstart + p);
+      const uint32_t knib = c.nib(d);
       if (p == 0 && !ver_upgrade(plock, pver)) return ATT_RESTART;
       if (!ver_upgrade(&S->lock, v)) {
         if (p == 0) ver_unlock(plock);
         return ATT_RESTART;
       }
-      uint32_t bid = a.base_branch + c.i;
-      Branch &Bn = P.branch[bid];
-      {
-        uint64_t *w = (uint64_t *)&Bn;
-#pragma unroll
-        for (int q = 0; q < 16; q++) w[q] = 0;
-      }
-      Bn.depth = (uint16_t)d;
-      Bn.dirty = 1;
+      const uint32_t bid = a.base_branch + c.i;
       ref_t rest = child;
       if (p + 1 < klen) {
-        uint32_t rid = a.base_short + 2 * c.i + 1;
-        init_short(P.shrt[rid], koff, kstart + p + 1, klen - p - 1, d + 1, (uint8_t)term, child);
+        const uint32_t rid = a.base_short + 2 * c.i + 1;
+        init_short(P.shrt[rid], koff, kstart + p + 1, klen - p - 1, d + 1, term, child);
         rest = make_ref(T_SHORT, rid);
-        mk.fresh[mk.nf++] = rest;
+        ds.push(P, rest);
       }
-      Bn.child[snib] = rest;
       bool ms;
-      ref_t leaf = make_leaf(a, c, d + 1, &ms);
-      Bn.child[knib] = leaf;
-      if (ms) mk.fresh[mk.nf++] = leaf;
-      ref_t bref = make_ref(T_BRANCH, bid);
-      mk.fresh[mk.nf++] = bref;
+      const ref_t leaf = make_leaf(a, c, d + 1, &ms);
+      if (ms) ds.push(P, leaf);
+      init_branch(P.branch[bid], d, snib, rest, knib, leaf);
+      const ref_t bref = make_ref(T_BRANCH, bid);
+      ds.push(P, bref);
       if (p == 0) {
-        __hip_atomic_store(pslot, bref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        publish(pslot, bref);
         ver_unlock_obsolete(&S->lock);
         ver_unlock(plock);
-        if (parent != REF_NULL) mk.existing[mk.ne++] = parent;
       } else {
-        S->key_len = (uint16_t)p;
-        S->term = 0;
-        __hip_atomic_store(&S->child, bref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // S keeps its first p nibbles and leads to the new branch: term
+        // cleared (its own byte: the dirty word beside it may be written by
+        // the lane that marks S), then child and key length in one word
+        st(&S->term, (uint8_t)0);
+        store_order();
+        st((uint64_t *)((uint8_t *)S + 40), (uint64_t)bref | (uint64_t)kstart << 32 | (uint64_t)p << 48);
+        store_order();
         ver_unlock(&S->lock);
-        mk.existing[mk.ne++] = node;
+        mark(P, c, ds, node, sflag, pos);
       }
       return ATT_DONE;
     }
     // a value node in a branch's slot 16: the key exists
     if (parent == REF_NULL || ref_type(parent) != T_BRANCH) return ATT_RESTART;
     if (!ver_upgrade(plock, pver)) return ATT_RESTART;
-    uint32_t vi = ref_idx(node);
-    if (vi >= P.cap_value || P.value[vi].seq < c.seq) {
-      uint32_t vid = a.base_value + c.i;
+    const uint32_t vi = ref_idx(node);
+    if (vi >= P.cap_value || ld(&P.value[vi].seq) < c.seq) {
+      const uint32_t vid = a.base_value + c.i;
       init_value(a, c, vid);
-      __hip_atomic_store(pslot, make_ref(T_VALUE, vid), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      publish(pslot, make_ref(T_VALUE, vid));
     }
     ver_unlock(plock);
-    mk.existing[mk.ne++] = parent;
     return ATT_DONE;
   }
   return ATT_RESTART;
@@ -301,7 +401,7 @@ __device__ int attempt(const NuArgs &a, const Ctx &c, Marks &mk, DirtyStage &ds)
 // write of each distinct key, scal[SC_M_UNIQ] of them).
 constexpr uint32_t kDirtyStage = 4096;  // staged dirty-list entries per block
 
-__global__ __launch_bounds__(256) void k_lock_insert(NuArgs a, uint64_t key_cap, uint32_t max_trips) {
+__global__ __launch_bounds__(256, 5) void k_lock_insert(NuArgs a, uint64_t key_cap, uint32_t max_trips) {
   __shared__ ref_t dbuf[kDirtyStage];
   __shared__ uint32_t dcnt;
   if (threadIdx.x == 0) dcnt = 0;
@@ -313,31 +413,29 @@ __global__ __launch_bounds__(256) void k_lock_insert(NuArgs a, uint64_t key_cap,
   Ctx c;
   c.i = i;
   c.key_cap = key_cap;
+  c.kp = a.P.keys;
+  c.len = 1;
+  c.trie = 0;
+  c.seq = 0;
+  c.key_off = 0;
+  c.mark_from = 0;
   if (active) {
     const uint64_t kr = a.kref[i];
     c.key_off = a.kbase + (kr & ((1ull << 48) - 1));
-    c.k.p = a.P.keys + c.key_off;
-    c.k.len = (uint32_t)(kr >> 48);
+    c.kp = a.P.keys + c.key_off;
+    c.len = (uint32_t)(kr >> 48);
     c.trie = a.tid ? (uint32_t)a.tid[i] : 0u;
     c.seq = a.seq_base + i;
+    c.mark_from = a.u_first[j] ? 0u : (uint32_t)a.u_lcp[j] + 1u;
   }
   bool done = !active;
   uint32_t restarts = 0;
-  Marks mk;
-  mk.ne = mk.nf = 0;
   for (uint32_t trip = 0; trip < max_trips; trip++) {
-    int res = ATT_RESTART;
     if (!done) {
-      res = attempt(a, c, mk, ds);
-      if (res == ATT_DONE)
+      if (attempt(a, c, ds) == ATT_DONE)
         done = true;
       else
         restarts++;
-    }
-    if (res == ATT_DONE) {
-      // the modified existing nodes, then the new nodes
-      for (int q = 0; q < mk.ne; q++) ds.mark(a.P, mk.existing[q]);
-      for (int q = 0; q < mk.nf; q++) ds.push(a.P, mk.fresh[q]);
     }
     if (__ballot(!done) == 0) break;
   }

@@ -434,3 +434,24 @@ def test_long_values(gmpt, workloads, oracle, mode):
     t = gmpt.MPT(device=0)
     _put(t, allb, mode)
     assert t.get_root_hash() == oracle.root_of(allb)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_skewed_user_keys_on_preloaded_trie(gmpt, workloads, oracle, mode):
+    """The skewed-update shape at reduced size: 'user' + 19-digit keys (a
+    31-nibble shared prefix, then digit branches under one-nibble extensions),
+    a preloaded trie, then Zipf(0.99) overwrites and inserts with many
+    repeated keys, so concurrent inserts meet on shared branches."""
+    pre, ops = workloads.config3_lock(seed=7, n_preload=60_000, n_ops=40_000, universe=150_000, value_len=60)
+    t = gmpt.MPT(device=0)
+    _put(t, pre, "2phase")
+    o = oracle.OracleMPT()
+    o.puts(pre)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    _put(t, ops, mode)
+    o.puts(ops)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    sample = ops.subset(range(0, ops.n, 97))
+    assert t.get_values(sample) == o.gets(sample)
