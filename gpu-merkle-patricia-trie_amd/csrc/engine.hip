@@ -374,29 +374,41 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
 // Sorts the packed keys (by trie, then nibble path) and keeps the last
 // occurrence of each key: u_idx[0 .. scal[SC_M_UNIQ]) in sorted order.  pass 0
 // sorts by the windows' leading bits with a fix-up of equal runs, pass 1 by
-// all 64 window bits, pass 2 by the whole keys.  No host synchronisation:
-// the caller reads flag_fallback at its next one and, if a flag is set, runs
-// next_pass() and repeats the work that followed.
+// every window bit that varies across the batch, pass 2 by the whole keys.
+// Passes 0 and 2 need no host synchronisation: the caller reads
+// flag_fallback at its next one and, if a flag is set, runs next_pass() and
+// repeats the work that followed.
 int Forest::sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int *cur) {
   uint64_t *sk = nullptr;
   uint32_t *sv = nullptr;
   int nwords = 0;
+  int r;
   if (pass == 2) {
-    int r;
     if ((r = sync_counters())) return r;
     uint32_t W = cnt_.scal[SC_W], maxlen = cnt_.scal[SC_MAXLEN];
     nwords = (int)((maxlen > W ? maxlen - W : 1) + 15) / 16;
   }
-  HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, pass == 1, cur));
-  HIP_TRY(nu_adjacent(a, sk, sv, pass == 2 ? 1 : 0, stream_));
+  uint64_t vmask = 0, sk_mask = 0;
+  if (pass == 1 && 4ull * ((a.n + 255) / 256) <= a.n) {
+    // keys with long shared stretches inside the window (ASCII digits, fixed
+    // fields) sort by fewer radix digits once the constant bits are dropped
+    HIP_TRY(nu_window_mask(a, *cur ? a.sk1 : a.sk0, stream_));
+    if ((r = sync_counters())) return r;
+    const uint64_t wor = *(const uint64_t *)(cnt_.scal + SC_WOR), wand = *(const uint64_t *)(cnt_.scal + SC_WAND);
+    const uint64_t m = wor & ~wand;
+    const int pop = __builtin_popcountll(m);
+    if (pop >= 1 && pop <= 56) vmask = m;  // saves at least one 8-bit digit
+  }
+  HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, pass == 1, cur, vmask, &sk_mask));
+  HIP_TRY(nu_adjacent(a, sk, sv, pass == 2 ? 1 : 0, stream_, sk_mask));
   HIP_TRY(nu_unique(a, sv, temp, stream_));
   return 0;
 }
 
 // After a synchronisation: the sort pass to redo with, or -1 when the last
 // sort stands.  A run of equal leading window bits too long to fix up asks
-// for all 64 window bits; two distinct keys with equal windows ask for a
-// sort by whole keys.
+// for a sort by every varying window bit; two distinct keys with equal
+// windows ask for a sort by whole keys.
 int Forest::next_pass(int pass) {
   uint32_t f = cnt_.flag_fallback;
   if (f == 0 || pass == 2) return -1;

@@ -91,7 +91,8 @@ struct NuArgs {
 // device scalar slots in NuArgs::scal
 enum {
   SC_W = 0, SC_MAXLEN = 1, SC_M_UNIQ = 2, SC_NGROUP = 3, SC_NE = 4, SC_KTOT = 6 /* u64 */, SC_MERGE = 8,
-  SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11, SC_COUNT = 16
+  SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11,
+  SC_WOR = 12 /* u64: OR of the windows */, SC_WAND = 14 /* u64: AND of the windows */, SC_COUNT = 16
 };
 
 // Zeroes the call's device scalars (SC_W starts at all ones for its minimum).
@@ -101,9 +102,19 @@ hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s);
 // with a fix-up of equal runs, or by all 64 bits when win_all_bits) or by the
 // whole keys (full = number of 16-nibble words), then by trie.  *cur_io (if
 // given) is the ping-pong buffer holding the current order, in and out.
+// With win_all_bits and a mask of the window bits that vary across the batch
+// (vmask, from nu_window_mask), the windows are first compressed to those
+// bits (an order-preserving bit gather to the top) and sorted by them only;
+// *sk_mask receives the mask the returned keys are compressed with (0: raw
+// windows), for nu_adjacent.
 hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
-                   uint32_t **sv, bool win_all_bits = false, int *cur_io = nullptr);
-hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s);
+                   uint32_t **sv, bool win_all_bits = false, int *cur_io = nullptr, uint64_t vmask = 0,
+                   uint64_t *sk_mask = nullptr);
+// OR and AND of the n windows sk[0 .. n) into scal[SC_WOR], scal[SC_WAND]
+// (per-block partials in a.ecount, which needs 4 words per 256 keys).
+hipError_t nu_window_mask(const NuArgs &a, const uint64_t *sk, hipStream_t s);
+hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s,
+                       uint64_t sk_mask = 0);
 hipError_t nu_unique(const NuArgs &a, const uint32_t *sv, void *temp, hipStream_t s);
 // Descends every unique key and groups them by stop point; sets SC_MERGE when
 // a split group is interrupted by keys below the split node (nu_regroup).

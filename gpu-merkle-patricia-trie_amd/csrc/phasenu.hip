@@ -280,7 +280,34 @@ __device__ __forceinline__ uint32_t trie_of(const NuArgs &a, uint32_t idx) {
   return a.tid ? (uint32_t)a.tid[idx] : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, const uint32_t *sv, int full) {
+// Bit gather / scatter over a mask that is the same for every lane (the loop
+// runs once per mask bit, lowest first).
+__device__ __forceinline__ uint64_t gather_bits(uint64_t x, uint64_t m) {
+  uint64_t r = 0;
+  for (uint32_t k = 0; m; k++) {
+    const uint64_t b = m & (0ull - m);
+    r |= (uint64_t)((x & b) != 0) << k;
+    m ^= b;
+  }
+  return r;
+}
+__device__ __forceinline__ uint64_t scatter_bits(uint64_t x, uint64_t m) {
+  uint64_t r = 0;
+  while (m) {
+    const uint64_t b = m & (0ull - m);
+    if (x & 1) r |= b;
+    x >>= 1;
+    m ^= b;
+  }
+  return r;
+}
+
+// Adjacent keys of the sorted order: LCP, duplicate and new-trie flags.
+// With cmask != 0 the keys are windows compressed to the bits of cmask
+// (k_compress_windows): the XOR of two is scattered back to window bits
+// before its leading zeros give the first differing nibble.
+__global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, const uint32_t *sv, int full,
+                                                  uint64_t cmask) {
   const uint32_t j = grid_tid();
   const bool active = j < a.n;
   // key j's place and trie; key j - 1's come from the lane before (one
@@ -313,6 +340,7 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
       uint32_t p;
       bool ident;
       uint64_t x = full ? 0ull : (sk[j - 1] ^ sk[j]);
+      if (x != 0 && cmask != 0) x = scatter_bits(x >> (64 - __popcll(cmask)), cmask);
       if (x != 0) {
         p = W + (uint32_t)(__clzll((long long)x) >> 2);
         if (ka.len != kb.len) p = min(p, mn - 1);
@@ -1133,10 +1161,66 @@ __global__ __launch_bounds__(256) void k_fix_runs(NuArgs a, uint64_t *sk, uint32
   }
 }
 
+// OR and AND of the windows per block (no global atomics: one slot pair per
+// block), then of the block results.
+__device__ __forceinline__ void or_and_block(uint64_t &o, uint64_t &n) {
+  __shared__ uint64_t so[4], sn[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    o |= (uint64_t)__shfl_xor((unsigned long long)o, off);
+    n &= (uint64_t)__shfl_xor((unsigned long long)n, off);
+  }
+  if (__lane_id() == 0) {
+    so[threadIdx.x >> 6] = o;
+    sn[threadIdx.x >> 6] = n;
+  }
+  __syncthreads();
+  o = so[0] | so[1] | so[2] | so[3];
+  n = sn[0] & sn[1] & sn[2] & sn[3];
+}
+__global__ __launch_bounds__(256) void k_wmask_part(uint32_t n, const uint64_t *sk, uint64_t *part) {
+  const uint32_t j = grid_tid();
+  uint64_t o = 0, m = ~0ull;
+  if (j < n) o = m = sk[j];
+  or_and_block(o, m);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = o;
+    part[2 * blockIdx.x + 1] = m;
+  }
+}
+__global__ __launch_bounds__(256) void k_wmask_final(uint32_t nb, const uint64_t *part, uint32_t *scal) {
+  uint64_t o = 0, m = ~0ull;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    o |= part[2 * b];
+    m &= part[2 * b + 1];
+  }
+  or_and_block(o, m);
+  if (threadIdx.x == 0) {
+    *(uint64_t *)(scal + SC_WOR) = o;
+    *(uint64_t *)(scal + SC_WAND) = m;
+  }
+}
+// Keeps the window bits of vmask, in order, at the top of the key.
+__global__ __launch_bounds__(256) void k_compress_windows(uint32_t n, uint64_t *sk, uint64_t vmask) {
+  const uint32_t j = grid_tid();
+  if (j >= n) return;
+  sk[j] = gather_bits(sk[j], vmask) << (64 - __popcll(vmask));
+}
+
 }  // namespace
 
+hipError_t nu_window_mask(const NuArgs &a, const uint64_t *sk, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const unsigned nb = nblk(a.n);
+  if (4ull * nb > a.n) return hipErrorInvalidValue;  // partials do not fit in ecount
+  uint64_t *part = (uint64_t *)a.ecount;
+  k_wmask_part<<<nb, kB, 0, s>>>(a.n, sk, part);
+  GMPT_LAUNCH_CHECK();
+  k_wmask_final<<<1, kB, 0, s>>>(nb, part, a.scal);
+  return hipGetLastError();
+}
+
 hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStream_t s, uint64_t **sk,
-                   uint32_t **sv, bool win_all_bits, int *cur_io) {
+                   uint32_t **sv, bool win_all_bits, int *cur_io, uint64_t vmask, uint64_t *sk_mask) {
   uint64_t *k[2] = {a.sk0, a.sk1};
   uint32_t *v[2] = {a.sv0, a.sv1};
   int cur = cur_io ? *cur_io : 0, which = 0;
@@ -1149,7 +1233,14 @@ hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStrea
     k_fix_runs<<<nblk(a.n), kB, 0, s>>>(a, k[cur], v[cur]);
     GMPT_LAUNCH_CHECK();
   } else if (!full) {
-    e = radix_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, 0, 64, &which, temp, s);
+    // compressed to the varying bits (all of them: no fix-up needed), or all 64
+    const int pop = vmask ? __builtin_popcountll(vmask) : 64;
+    const int lo = pop < 64 ? 64 - 8 * ((pop + 7) / 8) : 0;
+    if (pop < 64) {
+      k_compress_windows<<<nblk(a.n), kB, 0, s>>>(a.n, k[cur], vmask);
+      GMPT_LAUNCH_CHECK();
+    }
+    e = radix_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, lo, 64, &which, temp, s);
     if (e != hipSuccess) return e;
     cur ^= which;
   } else {
@@ -1181,11 +1272,15 @@ hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStrea
   *sk = k[cur];
   *sv = v[cur];
   if (cur_io) *cur_io = cur;
+  // the trie pass regathered raw windows
+  if (sk_mask) *sk_mask = (!full && win_all_bits && vmask && __builtin_popcountll(vmask) < 64 && trie_bits == 0)
+                              ? vmask : 0ull;
   return hipSuccess;
 }
 
-hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s) {
-  k_adjacent<<<nblk(a.n), kB, 0, s>>>(a, sk, sv, full);
+hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s,
+                       uint64_t sk_mask) {
+  k_adjacent<<<nblk(a.n), kB, 0, s>>>(a, sk, sv, full, sk_mask);
   return hipGetLastError();
 }
 

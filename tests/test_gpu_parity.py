@@ -422,6 +422,32 @@ def test_long_runs_of_equal_leading_window_bits(gmpt, workloads, oracle):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_long_runs_in_several_tries(gmpt, workloads, oracle, mode):
+    """The re-sort by the window's varying bits (here 1 + 40 of 64) and the
+    whole-key sort when the batch spans tries (the trie pass follows the
+    window sort), with repeated keys in each trie."""
+    rng = np.random.default_rng(34)
+    batches = []
+    for tr in range(3):
+        n = 1500
+        keys = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+        keys[:, 0] = rng.integers(0, 2, n) * 0x10
+        keys[:, 1:3] = 0x5a
+        keys[: n // 3, 3:8] = 0x77
+        keys = np.unique(keys, axis=0)
+        keys = np.concatenate([keys, keys[rng.integers(0, len(keys), 200)]])  # repeats: last write wins
+        flat, lens = workloads.random_values(rng, len(keys), 40, 90)
+        batches.append(workloads.batch_from_fixed(keys, flat, lens))
+    allb = workloads.concat_batches(batches, trie_ids=[0, 1, 2])
+    allb = allb.subset(rng.permutation(allb.n))
+    t = gmpt.MPT(device=0, num_tries=3)
+    _put(t, allb, mode)
+    for i in range(3):
+        sub = allb.subset(np.nonzero(allb.trie_ids == i)[0])
+        assert t.get_root_hash(i) == oracle.root_of(sub), i
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_long_values(gmpt, workloads, oracle, mode):
     """Leaves of many Keccak blocks (values up to 3,000 bytes) and a few
     single-byte and empty values."""
