@@ -236,6 +236,7 @@ int gmpt_get_stats(gmpt_forest *f, gmpt_stats *out) {
   out->leaf_perms = s.leaf_perms;
   out->leaf_bytes = s.leaf_bytes;
   out->leaf_nodes = s.leaf_nodes;
+  out->ext_direct_nodes = s.ext_direct_nodes;
   out->pool_bytes_used = f->f->pool_bytes_used();
   out->pool_bytes_reserved = f->f->pool_bytes_reserved();
   return 0;

@@ -49,6 +49,7 @@ struct OpStats {
   uint64_t leaf_perms = 0;     // permutations of the leaf level
   uint64_t leaf_bytes = 0;     // encoding bytes of the leaf level
   uint32_t leaf_nodes = 0;
+  uint32_t ext_direct_nodes = 0;  // extensions hashed by the leaf word builder
 };
 
 class Forest {

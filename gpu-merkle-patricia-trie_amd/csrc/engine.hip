@@ -500,6 +500,7 @@ int Forest::hash(void *targets, int n_targets) {
   stats_.hash_launches = 0;
   stats_.hashed_nodes = 0;
   stats_.leaf_nodes = 0;
+  stats_.ext_direct_nodes = 0;
   stats_.last_hash_ms = 0;
   if (n > 0) {
     Pools P = pools();
@@ -547,12 +548,14 @@ int Forest::hash(void *targets, int n_targets) {
         const bool leaves = lv <= 1;  // level 1: leaves the direct kernel does not take
         const uint32_t skip = lv == 0 ? n_direct : 0u;  // already queued
         const bool timed = lv == 0 && !lmeta;
+        const uint32_t n_ext = leaves ? 0u : M.hist[lv * kHcClasses + kHcClasses - 1];
         if (cnt > skip)
           HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
                                 encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
-                                timed ? ev3_ : nullptr));
+                                timed ? ev3_ : nullptr, n_ext));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
+        if (!leaves && n_ext >= kExtDirectMin && cnt > hc_spread_max_) stats_.ext_direct_nodes += n_ext;
         if (leaves) n_leaf += cnt;
       }
       start += cnt;

@@ -10,6 +10,8 @@ constexpr uint32_t kHcLevels = kHcMaxDepth + 2;
 constexpr uint32_t kHcBins = kHcLevels * kHcClasses;
 // Levels with at most this many nodes spread each Keccak state over 25 lanes.
 constexpr uint32_t kHcSpreadMaxDefault = 1024;
+// Extensions per inner level that get their own (leaf word builder) kernel.
+constexpr uint32_t kExtDirectMin = 8192;
 
 // Classification results, read back by the host in one copy.
 struct HcMeta {
@@ -41,11 +43,13 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
                        uint64_t *lmeta = nullptr, LeafRecords rec = LeafRecords());
 // Encodes and hashes list[0..n) (one level: no node depends on another);
 // slot and enc_len are the level's slices of the classification arrays;
-// leaves = the leaf level (level 0: only leaf short nodes).  Events, when
-// given, are recorded around the permutation kernel.
+// leaves = the leaf levels (0 and 1: only leaf short nodes).  Of an inner
+// level, the last n_ext entries are its extension class, hashed by the leaf
+// word builder when the level is above spread_max.  Events, when given, are
+// recorded around the permutation kernel.
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s,
-                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr);
+                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr, uint32_t n_ext = 0);
 // Encodes and hashes the classification's level-0 entries (leaves with
 // metadata, see hc_classify; their count is the sum of meta->hist[0 .. 8))
 // in one kernel; the count is read on the device, n_upper (>= it) sizes the

@@ -94,11 +94,21 @@ __device__ uint32_t estimate_blocks(const Pools &P, ref_t r, bool *leaf) {
   return keccak_blocks(len);
 }
 
+constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
+// Inner nodes have at most 5 blocks, so an inner level's last class is free
+// for the extensions k_hc_ext_direct hashes: a short node over a branch with
+// a key of 1 .. 64 nibbles.
+constexpr uint32_t kExtClass = kHcClasses - 1;
+__device__ __forceinline__ bool is_direct_ext(const Short &S) {
+  return !S.term && ref_type(S.child) == T_BRANCH && S.key_len >= 1 && S.key_len <= kLeafFastNibbles;
+}
+
 __device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r, uint32_t *blocks) {
   bool leaf;
   uint32_t nb = estimate_blocks(P, r, &leaf);
   *blocks = nb;
   uint32_t cls = nb > kHcClasses ? kHcClasses - 1 : nb - 1;
+  if (!leaf && ref_type(r) == T_SHORT && is_direct_ext(P.shrt[ref_idx(r)])) cls = kExtClass;
   uint32_t level;
   if (leaf) {
     level = 0;
@@ -123,7 +133,6 @@ __device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) 
 // node uses (a node at nibble depth 255 cannot exist), and take the encode +
 // scratch + permutation kernels.
 constexpr uint32_t kLdsLeafBins = kHcClasses;  // the bins of level 0
-constexpr uint32_t kLeafFastNibbles = 64;      // key nibbles a leaf head holds in registers
 constexpr uint32_t kLeafDirectMaxValue = 16382; // value bytes the metadata holds (14 bits, saturated above)
 
 __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
@@ -655,6 +664,20 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
   }
 }
 
+// An extension (is_direct_ext) encodes like a leaf whose value is its child's
+// hash buffer: [compact key, string of the child's digest, or of its raw
+// encoding when that is under 32 bytes].  Fills that leaf's metadata words
+// (the string's bytes are read from the child node) and returns true for
+// such a node.
+__device__ __forceinline__ bool ext_direct_meta(const Pools &P, ref_t r, uint64_t m[2]) {
+  if (ref_type(r) != T_SHORT) return false;
+  const Short &S = P.shrt[ref_idx(r)];
+  if (!is_direct_ext(S)) return false;
+  const Branch &B = P.branch[ref_idx(S.child)];
+  leaf_meta_words(S.key_off, S.key_start, S.key_len, 0, B.hash_size, (uint64_t)(uintptr_t)B.hash, m);
+  return true;
+}
+
 constexpr uint32_t kInnerGroup = 16;              // lanes per node (one per nibble slot)
 constexpr uint32_t kInnerNodes = 256 / kInnerGroup;
 constexpr uint32_t kInnerWords = 80;              // 640 LDS bytes per node >= 4 Keccak blocks
@@ -849,25 +872,15 @@ __device__ __noinline__ uint32_t leaf_raw(const Pools &P, ref_t r) {
 
 constexpr uint32_t kLeafDirectHead = 7;  // LDS words per thread for the head (6 used; odd spreads banks)
 
-__global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__restrict__ list,
-                                                        const uint64_t *__restrict__ lmeta,
-                                                        const uint32_t *__restrict__ hist,
-                                                        uint32_t *__restrict__ enc_len) {
-  __shared__ uint64_t head[256][kLeafDirectHead];
-  const uint32_t i = grid_tid();
-  uint32_t n = 0;  // the leaves of level 0, sorted first
-#pragma unroll
-  for (uint32_t c = 0; c < kLdsLeafBins; c++) n += hist[c];
-  if (i >= n) return;  // no block barrier below
-  const ref_t r = list[i];
-  const uint64_t m0 = lmeta[2 * (size_t)i], va = lmeta[2 * (size_t)i + 1];
+// Encodes and hashes one node shaped [compact key, string] from its metadata
+// words m0 (leaf_meta_words) and va (the string's bytes), building each
+// message word in registers; hp is the thread's LDS row for the message head.
+// Returns the encoding length.
+__device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, uint64_t m0, uint64_t va, uint64_t *hp) {
   const uint32_t vsz = (uint32_t)(m0 >> 50);
   const uint32_t vb0 = vsz == 1 ? (uint32_t)(gload(va & ~7ull) >> (8 * (va & 7))) & 0xffu : 0u;
   const LeafLayout l = leaf_layout(m0, vb0);
-  if (!l.fast) {
-    enc_len[i] = leaf_raw(P, r);
-    return;
-  }
+  if (!l.fast) return leaf_raw(P, r);
   const uint32_t enc = l.enc, nb = l.nb;
   uint64_t hw[kLeafHeadWords] = {0, 0, 0, 0, 0, 0};
   {
@@ -879,14 +892,12 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
     uint64_t kw[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) kw[q] = (uint32_t)q < nkw ? gload(ka - koff + 8 * q) : 0ull;
-    uint64_t *hp = head[threadIdx.x];
 #pragma unroll
     for (uint32_t q = 0; q < kLeafHeadWords; q++) hp[q] = 0;
     leaf_head(hp, l.payload, kw, koff, ks1, l.cnt, term, vsz, l.raw, vb0);
 #pragma unroll
     for (uint32_t q = 0; q < kLeafHeadWords; q++) hw[q] = hp[q];
   }
-  enc_len[i] = enc;
   // value geometry: message byte x is value byte x - vstart, i.e. byte
   // x + t0 from the aligned base V0; word w takes aligned words
   // jbase + w and jbase + w + 1 shifted by sh bytes
@@ -927,11 +938,57 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
     }
     keccak_f1600(st);
   }
-  uint64_t *hp = (uint64_t *)node_hash(P, r);
-  hp[0] = st[0];
-  hp[1] = st[1];
-  hp[2] = st[2];
-  hp[3] = st[3];
+  uint64_t *h = (uint64_t *)node_hash(P, r);
+  h[0] = st[0];
+  h[1] = st[1];
+  h[2] = st[2];
+  h[3] = st[3];
+  node_done(P, r, 32);
+  return enc;
+}
+
+__global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__restrict__ list,
+                                                        const uint64_t *__restrict__ lmeta,
+                                                        const uint32_t *__restrict__ hist,
+                                                        uint32_t *__restrict__ enc_len) {
+  __shared__ uint64_t head[256][kLeafDirectHead];
+  const uint32_t i = grid_tid();
+  uint32_t n = 0;  // the leaves of level 0, sorted first
+#pragma unroll
+  for (uint32_t c = 0; c < kLdsLeafBins; c++) n += hist[c];
+  if (i >= n) return;  // no block barrier below
+  enc_len[i] = hash_leaf_shaped(P, list[i], lmeta[2 * (size_t)i], lmeta[2 * (size_t)i + 1], head[threadIdx.x]);
+}
+
+// The extension class of an inner level (kExtClass): each extension encodes
+// like a leaf (ext_direct_meta), so the leaf word builder hashes it with no
+// scratch message.  A node that is not such an extension (none is binned
+// here) would take the generic encoder and its scratch slot.
+__global__ __launch_bounds__(256) void k_hc_ext_direct(Pools P, const ref_t *__restrict__ list,
+                                                       const uint64_t *__restrict__ slot, uint32_t n,
+                                                       uint64_t *__restrict__ scratch,
+                                                       uint32_t *__restrict__ enc_len) {
+  __shared__ uint64_t head[256][kLeafDirectHead];
+  const uint32_t i = grid_tid();
+  if (i >= n) return;  // no block barrier below
+  const ref_t r = list[i];
+  uint64_t m[2];
+  if (ext_direct_meta(P, r, m)) {
+    enc_len[i] = hash_leaf_shaped(P, r, m[0], m[1], head[threadIdx.x]);
+    return;
+  }
+  uint32_t enc;
+  uint64_t *msg = scratch + slot[i] / 8;
+  encode_generic(P, r, msg, &enc);
+  enc_len[i] = enc;
+  if (enc < 32) return;
+  uint64_t st[25];
+  keccak_lane(msg, keccak_blocks(enc), st);
+  uint64_t *h = (uint64_t *)node_hash(P, r);
+  h[0] = st[0];
+  h[1] = st[1];
+  h[2] = st[2];
+  h[3] = st[3];
   node_done(P, r, 32);
 }
 
@@ -1205,7 +1262,7 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
-                         hipEvent_t perm_end) {
+                         hipEvent_t perm_end, uint32_t n_ext) {
   if (n == 0) return hipSuccess;
   if (!leaves && n <= spread_max) {  // small inner level: encode and hash in one launch
     if (perm_begin) {
@@ -1216,6 +1273,16 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     GMPT_LAUNCH_CHECK();
     if (perm_end) return hipEventRecord(perm_end, s);
     return hipSuccess;
+  }
+  // the level's extension class, sorted last: its own launch when it is large
+  // enough to fill the GPU (a few thousand ride along in the branch kernels
+  // at no cost, but would pay a launch's latency)
+  if (!leaves && n_ext >= kExtDirectMin) {
+    const uint32_t m = n - n_ext;
+    k_hc_ext_direct<<<nblk(n_ext), kB, 0, s>>>(P, list + m, slot + m, n_ext, scratch, enc_len + m);
+    GMPT_LAUNCH_CHECK();
+    n = m;
+    if (n == 0) return hipSuccess;
   }
   if (leaves)
     k_hc_encode_leaf<<<(n + kLeafNodes - 1) / kLeafNodes, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);

@@ -35,7 +35,7 @@ class Stats(ctypes.Structure):
                 ("enc_bytes", ctypes.c_uint64), ("pool_bytes_used", ctypes.c_uint64),
                 ("pool_bytes_reserved", ctypes.c_uint64), ("leaf_perm_ms", ctypes.c_double),
                 ("leaf_perms", ctypes.c_uint64), ("leaf_bytes", ctypes.c_uint64), ("leaf_nodes", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32)]
+                ("ext_direct_nodes", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -64,7 +64,7 @@ typedef struct gmpt_stats {
   uint64_t leaf_perms;            /* its permutations */
   uint64_t leaf_bytes;            /* encoding bytes of the leaf level */
   uint32_t leaf_nodes;            /* leaf nodes hashed */
-  uint32_t pad;
+  uint32_t ext_direct_nodes;      /* extension nodes hashed by the leaf word builder (last hash) */
 } gmpt_stats;
 
 /* Creates a forest of num_tries empty tries on device `device`.

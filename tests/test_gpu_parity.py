@@ -476,6 +476,7 @@ def test_skewed_user_keys_on_preloaded_trie(gmpt, workloads, oracle, mode):
     o.hash()
     assert t.get_root_hash() == o.root()
     _put(t, ops, mode)
+    assert t.stats()["ext_direct_nodes"] >= 8192  # one-nibble extensions hashed as leaf-shaped nodes
     o.puts(ops)
     o.hash()
     assert t.get_root_hash() == o.root()
