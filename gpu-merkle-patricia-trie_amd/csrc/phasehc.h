@@ -11,7 +11,7 @@ constexpr uint32_t kHcBins = kHcLevels * kHcClasses;
 // Levels with at most this many nodes spread each Keccak state over 25 lanes.
 constexpr uint32_t kHcSpreadMaxDefault = 1024;
 // Extensions per inner level that get their own (leaf word builder) kernel.
-constexpr uint32_t kExtDirectMin = 8192;
+constexpr uint32_t kExtDirectMin = 16384;
 
 // Classification results, read back by the host in one copy.
 struct HcMeta {
