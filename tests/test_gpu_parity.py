@@ -482,3 +482,34 @@ def test_skewed_user_keys_on_preloaded_trie(gmpt, workloads, oracle, mode):
     assert t.get_root_hash() == o.root()
     sample = ops.subset(range(0, ops.n, 97))
     assert t.get_values(sample) == o.gets(sample)
+
+
+@pytest.mark.parametrize("seed", [61, 62, 63])
+def test_latched_inserts_split_fresh_nodes(gmpt, workloads, oracle, seed):
+    """Keys built so that concurrent inserts of one wavefront split the same
+    fresh short nodes at many depths: clusters of keys sharing prefixes of
+    random lengths, inserted with LockNU into an empty trie and then into the
+    result, checked against the oracle after each batch."""
+    rng = np.random.default_rng(seed)
+    keys = []
+    for _ in range(300):
+        base = rng.integers(0, 256, 32, dtype=np.uint8)
+        for _ in range(int(rng.integers(2, 40))):
+            k = base.copy()
+            cut = int(rng.integers(1, 32))
+            k[cut:] = rng.integers(0, 256, 32 - cut, dtype=np.uint8)
+            keys.append(k)
+    keys = np.unique(np.array(keys), axis=0)
+    keys = keys[rng.permutation(len(keys))]
+    half = len(keys) // 2
+    t = gmpt.MPT(device=0)
+    o = oracle.OracleMPT()
+    for part in (keys[:half], keys[half:]):
+        flat, lens = workloads.random_values(rng, len(part), 1, 120)
+        b = workloads.batch_from_fixed(part, flat, lens)
+        _put(t, b, "latching")
+        o.puts(b)
+        o.hash()
+        assert t.get_root_hash() == o.root()
+    sample = workloads.batch_from_fixed(keys[::11], *workloads.random_values(rng, len(keys[::11]), 1, 2))
+    assert t.get_values(sample) == o.gets(sample)
