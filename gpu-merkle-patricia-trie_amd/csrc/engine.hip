@@ -34,20 +34,13 @@ struct Carver {
   }
 };
 
-int st_levels_for(size_t ne) {
-  int k = 1;
-  while (((size_t)1 << k) < ne) k++;
-  return k + 1;
-}
-
 size_t nu_ws_bytes(size_t n, bool tries) {
   size_t ne = 2 * n + 2;
-  size_t k = (size_t)st_levels_for(ne);
   size_t b = 0;
   b += n * (4 + 8 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
   b += ne * (4 + 16);  // leaf records
   b += n * (4 * 7 + 1 + sizeof(Stop));
-  b += ne * (4 + 4 + 4 + 4) + k * ne;
+  b += ne * (4 + 4 + 4 + 4) + mt_bytes(align_up(ne, 256));
   b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
   b += 64 * 256;  // alignment slack for ~40 carves
   return b;
@@ -357,9 +350,9 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.e_grp = c.take<uint32_t>(ne);
   a.leaf_pk = c.take<uint32_t>(ne);
   a.leaf_meta = c.take<uint64_t>(2 * ne);
-  a.st_levels = st_levels_for(ne);
   a.st_stride = align_up(ne, 256);
-  a.st = c.take<uint8_t>(a.st_stride * (size_t)a.st_levels);
+  a.st_levels = mt_levels(a.st_stride);
+  a.st = c.take<uint8_t>(mt_bytes(a.st_stride));
   a.repof = c.take<uint32_t>(ne);
   a.bid = c.take<uint32_t>(ne);
   *temp = c.take<uint8_t>(std::max(radix_temp_bytes(n), scan_temp_bytes(ne)));

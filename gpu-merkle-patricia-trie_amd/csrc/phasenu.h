@@ -78,8 +78,8 @@ struct NuArgs {
   // E elements (M <= 2m)
   uint32_t *e_src;       // bit31: virtual (short node idx) else unique index
   uint32_t *e_grp;
-  uint8_t *st;           // sparse table, levels x stride bytes; level 0 = split-point lcp+1
-  size_t st_stride;
+  uint8_t *st;           // minimum tree over the split points (mt_* below); level 0 = split-point lcp+1
+  size_t st_stride;      // bytes of level 0 (a multiple of 256)
   int st_levels;
   uint32_t *repof;       // per split point: representative split point (or ~0u)
   uint32_t *bid;         // per split point: branch slot (reps only)
@@ -87,6 +87,25 @@ struct NuArgs {
   uint64_t seq_base;     // batch order stamp of key 0
   uint32_t base_branch, base_short, base_value;  // pre-assigned node slots
 };
+
+// Minimum tree geometry: level 0 holds `stride` bytes (one per split
+// point); level k holds one byte per 16 of level k - 1, padded to 16 bytes,
+// right after it; the top level fits one 16-byte block.
+__host__ __device__ inline size_t mt_level_len(size_t stride, int k) {
+  size_t len = stride;
+  for (int q = 0; q < k; q++) len = ((len / 16 + 15) / 16) * 16;
+  return len;
+}
+__host__ __device__ inline int mt_levels(size_t stride) {
+  int k = 1;
+  while (mt_level_len(stride, k - 1) > 16) k++;
+  return k;
+}
+__host__ __device__ inline size_t mt_bytes(size_t stride) {
+  size_t b = 0;
+  for (int k = 0; k < mt_levels(stride); k++) b += mt_level_len(stride, k);
+  return b;
+}
 
 // device scalar slots in NuArgs::scal
 enum {

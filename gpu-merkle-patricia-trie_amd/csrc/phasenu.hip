@@ -686,7 +686,7 @@ __global__ __launch_bounds__(256) void k_emit_count(NuArgs a) {
   a.ecount[i] = c;
 }
 
-// ecount[] holds the exclusive scan.  Level 0 of the sparse table gets, at
+// ecount[] holds the exclusive scan.  Level 0 of the minimum tree gets, at
 // split point q (between E[q] and E[q+1]), lcp(E[q], E[q+1]) + 1, or 0 across
 // a group boundary.
 __global__ __launch_bounds__(256) void k_emit_write(NuArgs a) {
@@ -744,68 +744,114 @@ __global__ __launch_bounds__(256) void k_emit_write(NuArgs a) {
 
 // ------------------------------------------------------------- 4. build --
 
-// Sparse-table levels k0 + 1 .. k0 + cnt (cnt <= 4) from level k0: entry q
-// of level l is min(SP[q .. q + 2^l)), stored where q + 2^l <= nsp.
-__global__ __launch_bounds__(256) void k_st_levels(NuArgs a, int k0, int cnt) {
-  uint32_t q = grid_tid();
-  uint32_t ne = a.scal[SC_NE];
-  uint32_t nsp = ne > 0 ? ne - 1 : 0;
-  const size_t h = (size_t)1 << k0;
-  if (q + 2 * h > nsp) return;
-  const uint8_t *lo = a.st + (size_t)k0 * a.st_stride;
-  uint32_t mn = lo[q];  // min of lo[q + m h] for m < have
-  uint32_t have = 1;
-  for (int l = 1; l <= cnt; l++) {
-    size_t span = h << l;  // level k0 + l covers span entries
-    if (q + span > nsp) return;
-    for (uint32_t m = have; m < 2 * have; m++) mn = min(mn, (uint32_t)lo[q + m * h]);
-    a.st[(size_t)(k0 + l) * a.st_stride + q] = (uint8_t)mn;
-    have *= 2;
+// Split-point LCP minima (SP[q] = lcp + 1 of split point q, 0 between
+// groups) as a 16-ary minimum tree: entry b of level k is the minimum of
+// entries 16b .. 16b + 15 of level k - 1.  A nearest-smaller query reads one
+// 16-byte block per level it climbs and descends: for uniform keys the
+// answer is almost always in the query's own block or the next one up,
+// instead of one dependent byte load per level of a sparse table (21 levels
+// for 2M split points).
+constexpr int kMtMaxLevels = 8;  // 16^8 split points
+
+// Level k from level k - 1, one entry per thread (entries past the level's
+// valid count are never read by a query).
+__global__ __launch_bounds__(256) void k_mt_level(NuArgs a, int k) {
+  const uint32_t b = grid_tid();
+  const uint32_t ne = a.scal[SC_NE];
+  uint32_t cnt = ne > 0 ? ne - 1 : 0;  // valid entries of level k - 1
+  size_t off = 0;
+  for (int q = 0; q < k - 1; q++) {
+    off += mt_level_len(a.st_stride, q);
+    cnt = (cnt + 15) / 16;
   }
+  if ((size_t)16 * b >= cnt) return;
+  const uint8_t *lo = a.st + off + (size_t)16 * b;
+  const uint4 v = *(const uint4 *)lo;
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  uint32_t mn = 255;
+#pragma unroll
+  for (uint32_t i = 0; i < 16; i++)
+    if (16 * b + i < cnt) mn = min(mn, (w[i >> 2] >> (8 * (i & 3))) & 255u);
+  a.st[off + mt_level_len(a.st_stride, k - 1) + b] = (uint8_t)mn;
 }
 
 struct ST {
   const uint8_t *t;
-  size_t stride;
+  uint32_t off[kMtMaxLevels];  // byte offset of level k
+  uint32_t cnt[kMtMaxLevels];  // valid entries of level k
   int levels;
   uint32_t nsp;
-  __device__ __forceinline__ uint8_t at(int k, uint32_t q) const { return t[(size_t)k * stride + q]; }
+  __device__ __forceinline__ uint8_t at(int k, uint32_t q) const { return t[off[k] + q]; }
+  // 16-bit mask of the valid entries of the block of level k at base
+  // (a multiple of 16) whose value is below lim (lim <= 256)
+  __device__ __forceinline__ uint32_t below(int k, uint32_t base, uint32_t lim) const {
+    const uint4 v = *(const uint4 *)(t + off[k] + base);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) m |= (uint32_t)(((w[i >> 2] >> (8 * (i & 3))) & 255u) < lim) << i;
+    const uint32_t valid = cnt[k] - base;  // > 0 for every block a query reads
+    return valid >= 16 ? m : m & ((1u << valid) - 1);
+  }
+  // rightmost (leftmost) entry below lim under level-k entry p, at level 0
+  __device__ __forceinline__ uint32_t down_last(int k, uint32_t p, uint32_t lim) const {
+    for (; k > 0; k--) {
+      const uint32_t base = 16 * p;
+      p = base + 31 - __clz(below(k - 1, base, lim));
+    }
+    return p;
+  }
+  __device__ __forceinline__ uint32_t down_first(int k, uint32_t p, uint32_t lim) const {
+    for (; k > 0; k--) {
+      const uint32_t base = 16 * p;
+      p = base + __ffs(below(k - 1, base, lim)) - 1;
+    }
+    return p;
+  }
+  // largest q <= j0 with SP[q] < lim, or -1 (j0 may be -1)
+  __device__ int last_below(int64_t j0, uint32_t lim) const {
+    int64_t idx = j0;
+    for (int k = 0; k < levels && idx >= 0; k++) {
+      const uint32_t base = (uint32_t)idx & ~15u;
+      const uint32_t m = below(k, base, lim) & (0xffffu >> (15 - ((uint32_t)idx - base)));
+      if (m) return (int)down_last(k, base + 31 - __clz(m), lim);
+      idx = (int64_t)(base / 16) - 1;
+    }
+    return -1;
+  }
+  // smallest q >= j0 with SP[q] < lim, or nsp
+  __device__ uint32_t first_below(uint32_t j0, uint32_t lim) const {
+    uint32_t idx = j0;
+    for (int k = 0; k < levels && idx < cnt[k]; k++) {
+      const uint32_t base = idx & ~15u;
+      const uint32_t m = below(k, base, lim) & (0xffffu << (idx - base)) & 0xffffu;
+      if (m) return down_first(k, base + __ffs(m) - 1, lim);
+      idx = base / 16 + 1;
+    }
+    return nsp;
+  }
   // largest q < j with SP[q] < L, or -1
-  __device__ int prev_less(uint32_t j, uint32_t L) const {
-    int64_t pos = j;
-    for (int k = levels - 1; k >= 0; k--) {
-      int64_t s = 1ll << k;
-      if (pos - s >= 0 && at(k, (uint32_t)(pos - s)) >= L) pos -= s;
-    }
-    return (int)(pos - 1);
-  }
+  __device__ int prev_less(uint32_t j, uint32_t L) const { return last_below((int64_t)j - 1, L); }
   // smallest q > j with SP[q] < L, or nsp
-  __device__ uint32_t next_less(uint32_t j, uint32_t L) const {
-    uint64_t pos = (uint64_t)j + 1;
-    for (int k = levels - 1; k >= 0; k--) {
-      uint64_t s = 1ull << k;
-      if (pos + s <= nsp && at(k, (uint32_t)pos) >= L) pos += s;
-    }
-    return (uint32_t)pos;
-  }
+  __device__ uint32_t next_less(uint32_t j, uint32_t L) const { return first_below(j + 1, L); }
   // smallest q >= j with SP[q] <= L, or nsp
-  __device__ uint32_t next_leq(uint32_t j, uint32_t L) const {
-    uint64_t pos = j;
-    for (int k = levels - 1; k >= 0; k--) {
-      uint64_t s = 1ull << k;
-      if (pos + s <= nsp && at(k, (uint32_t)pos) > L) pos += s;
-    }
-    return (uint32_t)pos;
-  }
+  __device__ uint32_t next_leq(uint32_t j, uint32_t L) const { return first_below(j, L + 1); }
 };
 
 __device__ __forceinline__ ST st_of(const NuArgs &a) {
   ST s;
   s.t = a.st;
-  s.stride = a.st_stride;
-  s.levels = a.st_levels;
   uint32_t ne = a.scal[SC_NE];
   s.nsp = ne > 0 ? ne - 1 : 0;
+  s.levels = min(a.st_levels, kMtMaxLevels);
+  uint32_t off = 0, cnt = s.nsp;
+#pragma unroll
+  for (int k = 0; k < kMtMaxLevels; k++) {
+    s.off[k] = off;
+    s.cnt[k] = cnt;
+    off += (uint32_t)mt_level_len(a.st_stride, k);
+    cnt = (cnt + 15) / 16;
+  }
   return s;
 }
 
@@ -1343,9 +1389,10 @@ hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s) {
 
 hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
   uint32_t ne_max = 2 * a.n + 2;
-  for (int k = 1; k < a.st_levels; k += 4) {
-    int cnt = a.st_levels - k < 4 ? a.st_levels - k : 4;
-    k_st_levels<<<nblk(ne_max), kB, 0, s>>>(a, k - 1, cnt);
+  size_t cnt = ne_max;  // bound on the entries of each level
+  for (int k = 1; k < a.st_levels; k++) {
+    cnt = (cnt + 15) / 16;
+    k_mt_level<<<nblk(cnt), kB, 0, s>>>(a, k);
     GMPT_LAUNCH_CHECK();
   }
   k_updates<<<nblk(a.n), kB, 0, s>>>(a);
