@@ -495,12 +495,13 @@ int Forest::hash(void *targets, int n_targets) {
   stats_.leaf_nodes = 0;
   stats_.ext_direct_nodes = 0;
   stats_.last_hash_ms = 0;
+  unsigned long long *parts = nullptr;  // the pass's work counts (hc_count), folded by hc_roots
   if (n > 0) {
     Pools P = pools();
     if ((r = ensure_ws(hc_sorted_, sizeof(ref_t) * (size_t)n))) return r;
     if ((r = ensure_ws(hc_slot_, sizeof(uint64_t) * (size_t)n))) return r;
     if ((r = ensure_ws(hc_enc_, sizeof(uint32_t) * (size_t)n))) return r;
-    if ((r = ensure_ws(hc_meta_, sizeof(HcMeta)))) return r;
+    if ((r = ensure_ws(hc_meta_, sizeof(HcMeta) + 32 * kHcCountParts))) return r;
     if ((r = ensure_ws(hc_cursor_, sizeof(uint32_t) * kHcBins))) return r;
     if ((r = ensure_ws(ws2_, hc_temp_bytes(n)))) return r;
     ref_t *sorted = (ref_t *)hc_sorted_.p;
@@ -554,10 +555,11 @@ int Forest::hash(void *targets, int n_targets) {
       start += cnt;
     }
     HIP_TRY(hipEventRecord(ev1_, stream_));
-    HIP_TRY(hc_count(P, encl, (const HcMeta *)hc_meta_.p, n, n_leaf, stream_));
+    parts = (unsigned long long *)((uint8_t *)hc_meta_.p + sizeof(HcMeta));
+    HIP_TRY(hc_count(encl, (const HcMeta *)hc_meta_.p, n, n_leaf, parts, stream_));
     stats_.leaf_nodes = n_leaf;
   }
-  HIP_TRY(hc_roots(pools(), ntries_, stream_));  // also empties the dirty list
+  HIP_TRY(hc_roots(pools(), ntries_, parts, stream_));  // also empties the dirty list
   rec_ = LeafRecords();
   if ((r = sync_counters())) return r;
   // the counters are zeroed by the classification, so only a pass that ran sets them

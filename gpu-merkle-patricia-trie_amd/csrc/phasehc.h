@@ -56,15 +56,18 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
 // launch, so the launch need not wait for the bins to reach the host.
 hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64_t *lmeta, const HcMeta *meta,
                                  uint32_t n_upper, uint32_t *enc_len, hipStream_t s);
-// Adds the pass's permutations and encoding bytes (from the enc_len array of
-// the meta->nvalid hashed nodes) to the forest counters, and those of the
-// first n_leaf entries (the leaf level) to the leaf counters.
-hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
-                    hipStream_t s);
+// The pass's permutations and encoding bytes (from the enc_len array of the
+// meta->nvalid hashed nodes), and those of the first n_leaf entries (the
+// leaf level), as kHcCountParts per-block partial sums (4 words each) in
+// parts; hc_roots adds them to the forest counters.
+constexpr uint32_t kHcCountParts = 256;
+hipError_t hc_count(const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
+                    unsigned long long *parts, hipStream_t s);
 // Zeroes the forest's counters and every trie header (empty tries).
 hipError_t hc_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries, hipStream_t s);
-// Copies every trie's root hash into its header and empties the dirty list.
-hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s);
+// Copies every trie's root hash into its header and empties the dirty list;
+// with parts (from hc_count), adds the pass's work counts to the counters.
+hipError_t hc_roots(const Pools &P, uint32_t ntries, const unsigned long long *parts, hipStream_t s);
 // Node export: encoding lengths, then encodings at offs and 32-byte hash slots.
 hipError_t hc_enc_len(const Pools &P, const ref_t *refs, uint32_t m, uint64_t *len, hipStream_t s);
 hipError_t hc_enc_write(const Pools &P, const ref_t *refs, uint32_t m, const uint64_t *offs, uint8_t *encs,

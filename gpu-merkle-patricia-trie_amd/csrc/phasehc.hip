@@ -463,20 +463,35 @@ __device__ __forceinline__ uint64_t low_bytes(uint64_t v, uint32_t n) { return n
 // Work counters of a hash pass: one reduction over the encoding lengths
 // after the last level (per-wave atomics on one counter serialise when waves
 // are many).
-__global__ __launch_bounds__(256) void k_hc_count(Counters *cnt, const uint32_t *__restrict__ enc_len,
+// (one partial per block: atomics from every block on four counters
+// serialise)
+__global__ __launch_bounds__(256) void k_hc_count(unsigned long long *parts, const uint32_t *__restrict__ enc_len,
                                                   const uint32_t *__restrict__ nvalid, uint32_t n_leaf) {
   __shared__ unsigned long long sh[4][4];
   const uint32_t n = *nvalid;
   // perms, bytes over all entries; perms, bytes over the leaf entries
   unsigned long long v[4] = {0, 0, 0, 0};
-  for (uint32_t i = grid_tid(); i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t e = enc_len[i];
-    const uint32_t p = e >= 32 ? keccak_blocks(e) : 0u;
-    v[0] += p;
-    v[1] += e;
-    if (i < n_leaf) {
-      v[2] += p;
-      v[3] += e;
+  // four entries per load (enc_len is 16-byte aligned)
+  const uint32_t stride = 4 * gridDim.x * blockDim.x;
+  for (uint32_t i0 = 4 * grid_tid(); i0 < n; i0 += stride) {
+    uint32_t e4[4];
+    if (i0 + 4 <= n) {
+      const uint4 q = *(const uint4 *)(enc_len + i0);
+      e4[0] = q.x, e4[1] = q.y, e4[2] = q.z, e4[3] = q.w;
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++) e4[k] = i0 + k < n ? enc_len[i0 + k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      const uint32_t e = e4[k];
+      const uint32_t p = e >= 32 ? keccak_blocks(e) : 0u;
+      v[0] += p;
+      v[1] += e;
+      if (i0 + k < n_leaf) {
+        v[2] += p;
+        v[3] += e;
+      }
     }
   }
 #pragma unroll
@@ -486,14 +501,9 @@ __global__ __launch_bounds__(256) void k_hc_count(Counters *cnt, const uint32_t 
 #pragma unroll
     for (int q = 0; q < 4; q++) sh[threadIdx.x >> 6][q] = v[q];
   __syncthreads();
-  if (threadIdx.x < 4) {
-    unsigned long long t = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
-    unsigned long long *dst = threadIdx.x == 0 ? &cnt->n_perms
-                              : threadIdx.x == 1 ? &cnt->enc_bytes
-                              : threadIdx.x == 2 ? &cnt->leaf_perms
-                                                 : &cnt->leaf_bytes;
-    if (t) atomicAdd(dst, t);
-  }
+  if (threadIdx.x < 4)
+    parts[4 * blockIdx.x + threadIdx.x] =
+        sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
 // Writes the list header, the compact-key item and the value header (or a
@@ -1118,9 +1128,30 @@ __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restri
   if (lane == 0) node_done(P, r, 32);
 }
 
-__global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries) {
+__global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries, const unsigned long long *parts) {
   uint32_t t = grid_tid();
   if (t == 0) P.cnt->n_dirty = 0;  // every listed node has been hashed
+  if (parts && blockIdx.x == 0) {  // the pass's work counts (uniform branch)
+    static_assert(kHcCountParts == 256, "one partial per thread");
+    __shared__ unsigned long long sh[4][4];
+    unsigned long long v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      v[q] = parts[4 * threadIdx.x + q];
+      for (int off = 32; off > 0; off >>= 1) v[q] += (unsigned long long)__shfl_xor((long long)v[q], off);
+    }
+    if (__lane_id() == 0)
+#pragma unroll
+      for (int q = 0; q < 4; q++) sh[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      unsigned long long *dst = threadIdx.x == 0 ? &P.cnt->n_perms
+                                : threadIdx.x == 1 ? &P.cnt->enc_bytes
+                                : threadIdx.x == 2 ? &P.cnt->leaf_perms
+                                                   : &P.cnt->leaf_bytes;
+      *dst += sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    }
+  }
   if (t >= ntries) return;
   TrieHdr &T = P.tries[t];
   ref_t r = T.root;
@@ -1309,10 +1340,9 @@ hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64
   return hipGetLastError();
 }
 
-hipError_t hc_count(const Pools &P, const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
-                    hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_hc_count<<<(unsigned)std::min<size_t>(nblk(n), 1024), kB, 0, s>>>(P.cnt, enc_len, &meta->nvalid, n_leaf);
+hipError_t hc_count(const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
+                    unsigned long long *parts, hipStream_t s) {
+  k_hc_count<<<kHcCountParts, kB, 0, s>>>(parts, enc_len, &meta->nvalid, n_leaf);
   return hipGetLastError();
 }
 
@@ -1331,8 +1361,8 @@ hipError_t hc_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t hc_roots(const Pools &P, uint32_t ntries, hipStream_t s) {
-  k_hc_roots<<<nblk(ntries), kB, 0, s>>>(P, ntries);
+hipError_t hc_roots(const Pools &P, uint32_t ntries, const unsigned long long *parts, hipStream_t s) {
+  k_hc_roots<<<nblk(ntries), kB, 0, s>>>(P, ntries, parts);
   return hipGetLastError();
 }
 
