@@ -132,7 +132,7 @@ class Forest {
   DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_, hc_lmeta_, hc_lmeta_tmp_;
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
-  bool hc_fuse_leaves_ = true;   // leaves of <= 2 Keccak blocks: one encode+hash kernel
+  bool hc_fuse_leaves_ = true;   // level-0 leaves: one encode+hash kernel (k_hc_leaf_direct)
   LeafRecords rec_;              // bins the last build recorded for its new leaves (in ws_)
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel

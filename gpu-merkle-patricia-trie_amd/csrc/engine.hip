@@ -520,9 +520,8 @@ int Forest::hash(void *targets, int n_targets) {
     HIP_TRY(hipEventRecord(ev_meta_, stream_));
     HIP_TRY(hipEventRecord(ev0_, stream_));
     if (lmeta) {
-      // leaves of one or two blocks (the first two bins): launched before the
-      // bins reach the host, so the GPU hashes them while the host plans the
-      // other levels
+      // the leaves of level 0 (its bins): launched before the bins reach the
+      // host, so the GPU hashes them while the host plans the other levels
       HIP_TRY(hipEventRecord(ev2_, stream_));
       HIP_TRY(hc_hash_leaves_direct(P, sorted, lmeta, (const HcMeta *)hc_meta_.p, n, encl, stream_));
       HIP_TRY(hipEventRecord(ev3_, stream_));

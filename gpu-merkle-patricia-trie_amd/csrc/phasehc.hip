@@ -121,8 +121,7 @@ __device__ __forceinline__ uint32_t bin_of(const Pools &P, ref_t r, uint32_t *bl
 
 // Leaf metadata (leaf_meta_words) for the leaf-level kernel, recorded by the
 // classification in sorted order so that kernel's loads do not chase
-// list -> short -> value.  Only leaves of at most two Keccak blocks get one
-// (value size < 272).
+// list -> short -> value.
 __device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) {
   const Short &S = P.shrt[ref_idx(r)];
   const Value &V = P.value[ref_idx(S.child)];
@@ -815,19 +814,20 @@ __global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list
   node_done(P, r, 32);
 }
 
-// Leaf level, leaves of at most two Keccak blocks: encode and hash in one
-// launch, one thread per leaf, without a scratch message.  The leaf's
+// Leaf level 0 (key remainder <= 64 nibbles, value < 16,383 bytes, any
+// number of Keccak blocks): encode and hash in one launch, one thread per
+// leaf, without a scratch message.  The leaf's
 // metadata comes from the classification (leaf_meta, sorted order), so the
 // loads are at most two deep; the thread builds the message head (list
 // header, compact key, value header) once, then computes each 8-byte message
 // word as it absorbs it: head word, value bytes (aligned value words shifted
 // by the value's offset in the message and masked to the value's bytes),
 // Keccak padding.  The state stays in registers as in k_hc_perm_lane.
-// The kernel reads its leaf count (the first two bins) from the
-// classification on the device, so it can be launched before the host has
-// read the bins back.  Leaves with a key remainder over 64 nibbles are binned
-// out of its range; the only leaves here off the word builder's path are
-// those whose encoding is under 32 bytes (raw "hash", no permutation).
+// The kernel reads its leaf count (level 0's bins) from the classification
+// on the device, so it can be launched before the host has read the bins
+// back.  Other leaves are binned at level 1; the only leaves here off the
+// word builder's path are those whose encoding is under 32 bytes (raw
+// "hash", no permutation).
 
 // Aligned 8-byte load from global memory.
 typedef const __attribute__((address_space(1))) uint64_t gword_t;

@@ -130,9 +130,10 @@ int gmpt_synchronize(gmpt_forest *f);
  * dirty nodes run Keccak with one state spread over 25 lanes (latency), larger
  * levels one state per lane (throughput).  Default 1024, or the
  * GMPT_HC_SPREAD_MAX environment variable.  "hc_fuse_leaves" (0 or 1,
- * default 1): leaves of at most two Keccak blocks are encoded and hashed by
- * one kernel through LDS instead of an encode kernel, an HBM scratch buffer
- * and a permutation kernel.  No reference counterpart. */
+ * default 1): leaves whose key remainder has at most 64 nibbles and whose
+ * value is under 16,383 bytes are encoded and hashed by one kernel, the
+ * message built in registers, instead of an encode kernel, an HBM scratch
+ * buffer and a permutation kernel.  No reference counterpart. */
 int gmpt_set_option(gmpt_forest *f, const char *name, int64_t value);
 int gmpt_get_option(gmpt_forest *f, const char *name, int64_t *value);
 /* Autotuner: inserts `sample` (mode 0 PhaseNU, 1 LockNU) into a scratch forest
