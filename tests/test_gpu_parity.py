@@ -513,3 +513,30 @@ def test_latched_inserts_split_fresh_nodes(gmpt, workloads, oracle, seed):
         assert t.get_root_hash() == o.root()
     sample = workloads.batch_from_fixed(keys[::11], *workloads.random_values(rng, len(keys[::11]), 1, 2))
     assert t.get_values(sample) == o.gets(sample)
+
+
+@pytest.mark.parametrize("modes", [("2phase", "latching"), ("latching", "latching"), ("latching", "2phase")])
+def test_prefix_keys_incremental(gmpt, workloads, oracle, modes):
+    """Keys that are prefixes of other keys (values in branch slot 16): a
+    second batch overwrites some of them, adds new prefixes of existing keys
+    and extensions of existing keys, each mode on top of the other."""
+    rng = random.Random(71)
+    keys = set()
+    while len(keys) < 1200:
+        keys.add(bytes(rng.choice([0, 1, 0x10, 0xfe]) for _ in range(rng.randint(1, 5))))
+    keys = sorted(keys)
+    rng.shuffle(keys)
+    first, rest = keys[:700], keys[700:]
+    second = rest + rng.sample(first, 200)
+    rng.shuffle(second)
+    t = gmpt.MPT(device=0)
+    o = oracle.OracleMPT()
+    for ks, mode in ((first, modes[0]), (second, modes[1])):
+        vals = [bytes([rng.getrandbits(8)]) * rng.randint(1, 50) for _ in ks]
+        b = workloads.batch_from_bytes(ks, vals)
+        _put(t, b, mode)
+        o.puts(b)
+        o.hash()
+        assert t.get_root_hash() == o.root()
+    allb = workloads.batch_from_bytes(keys, [b"x"] * len(keys))
+    assert t.get_values(allb) == o.gets(allb)
