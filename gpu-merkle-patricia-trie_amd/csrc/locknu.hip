@@ -38,9 +38,9 @@ inline unsigned nblk(size_t n) { return (unsigned)((n + kB - 1) / kB); }
 // optimistic read protocol only needs each load of a step to have returned
 // before the next step's loads issue: load_order() waits for outstanding
 // loads (a version read, then the contents it guards, then the re-check).
-// An acquire load would also invalidate the CU's caches at every version
-// check.  Key bytes in the arena never change while the insert runs and are
-// read with plain loads.
+// An acquire load would also invalidate the L2 of the CU's die at every
+// version check.  Key bytes in the arena never change while the insert runs
+// and are read with plain loads.
 template <typename T>
 __device__ __forceinline__ T ld(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
