@@ -91,7 +91,7 @@ struct Counters {
   unsigned long long enc_bytes;  // PhaseHC: bytes of node encodings produced
   unsigned long long leaf_perms; // PhaseHC: permutations of the leaf level
   unsigned long long leaf_bytes; // PhaseHC: encoding bytes of the leaf level
-  uint32_t scal[16];             // the current insert's device scalars (PhaseNU / LockNU)
+  uint32_t scal[20];             // the current insert's device scalars (PhaseNU / LockNU)
 };
 
 enum : uint32_t {

@@ -309,6 +309,7 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.kh_bytes = d.keys_bytes;
   a.v_bytes = d.values_total;
   a.kbase = key_used_;
+  a.key_cap = keys_.cap;
   a.vbase = vbase;
   a.base_branch = cnt_.n_branch;
   a.base_short = cnt_.n_short;

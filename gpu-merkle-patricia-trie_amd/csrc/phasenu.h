@@ -53,6 +53,7 @@ struct NuArgs {
   uint32_t *klen;
   uint64_t *kpoff;       // relative to kbase after the scan
   uint64_t *kref;        // kpoff | nibbles << 48: one load gives a key's place and length
+  uint64_t key_cap;      // key arena bytes allocated (bounds the speculative packing)
   uint32_t *leaf_pk;     // per element: the new leaf's hashing bin (hc_leaf.h), or kLeafPackedNone
   uint64_t *leaf_meta;   // per element: the new leaf's metadata words (two)
   // sort
@@ -111,7 +112,8 @@ __host__ __device__ inline size_t mt_bytes(size_t stride) {
 enum {
   SC_W = 0, SC_MAXLEN = 1, SC_M_UNIQ = 2, SC_NGROUP = 3, SC_NE = 4, SC_KTOT = 6 /* u64 */, SC_MERGE = 8,
   SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11,
-  SC_WOR = 12 /* u64: OR of the windows */, SC_WAND = 14 /* u64: AND of the windows */, SC_COUNT = 16
+  SC_WOR = 12 /* u64: OR of the windows */, SC_WAND = 14 /* u64: AND of the windows */,
+  SC_UNIFORM = 16 /* 1: every key has key 0's length (packed by the validation pass) */, SC_COUNT = 20
 };
 
 // Zeroes the call's device scalars (SC_W starts at all ones for its minimum).

@@ -125,14 +125,35 @@ __device__ __forceinline__ KeyStager stage_keys(const NuArgs &a, uint8_t *lds) {
   return st;
 }
 
+// Packed slot bytes of a key of L hex bytes (terminator included): two
+// nibbles per byte, padded to whole 8-byte words, at least one zero byte.
+__device__ __forceinline__ uint32_t slot_bytes(uint32_t L) { return (L / 2 + 8) & ~7u; }
+
+// Writes the key's nibbles (nn = L - 1 of them, the terminator left out) two
+// per byte, high nibble first, zero past nn, over slot_bytes(L) bytes.
+__device__ __forceinline__ void pack_key(uint64_t *dst, const uint8_t *k, uint32_t L) {
+  const uint32_t nb = slot_bytes(L);
+  const uint32_t nn = L - 1;
+  for (uint32_t b = 0; b < nb; b += 8) {
+    uint32_t o = 2 * b;  // first nibble of this 8-byte word
+    uint64_t h0 = o < nn ? ldn(k + o, min(8u, nn - o)) : 0ull;
+    uint64_t h1 = o + 8 < nn ? ldn(k + o + 8, min(8u, nn - o - 8)) : 0ull;
+    dst[b / 8] = (uint64_t)pack8(h0) | ((uint64_t)pack8(h1) << 32);
+  }
+}
+
 // Validates every key, and reduces per block the common prefix with key 0
-// and the longest key (no global atomics: one slot per block).
+// and the longest key (no global atomics: one slot per block).  A key as long
+// as key 0 is also packed at once to slot i of a layout with equal slots: when
+// every key has that length (bit 31 of blk_len stays clear in every block),
+// the packing pass only builds the sort windows.
 __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, uint32_t *blk_len) {
   __shared__ uint32_t sw[4], sl[4];
   __shared__ __align__(16) uint8_t lds[kKeyStage + 16];
   const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
-  uint32_t lcp0 = UINT_MAX, len = 0;
+  uint32_t lcp0 = UINT_MAX, len = 0, odd = 0;
+  const uint32_t L0 = key_range_ok(a, 0) ? (uint32_t)(a.kidx[1] - a.kidx[0] + 1) : 0u;
   if (i < a.n) {
     bool ok = key_range_ok(a, i);
     int32_t L = 1;
@@ -177,43 +198,63 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
     }
     len = (uint32_t)L;
     a.klen[i] = len;
-    a.kpoff[i] = (uint64_t)((len / 2 + 8) & ~7u);  // packed slot bytes (>= 8)
+    const uint32_t slot = slot_bytes(len);
+    a.kpoff[i] = slot;
+    odd = len != L0;
+    if (ok && !odd && a.kbase + (uint64_t)(i + 1) * slot <= a.key_cap) {
+      const uint64_t koff = (uint64_t)i * slot;
+      pack_key((uint64_t *)(a.P.keys + a.kbase + koff), stg.at(a, a.kidx[2 * i]), len);
+      a.kref[i] = koff | ((uint64_t)len << 48);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     lcp0 = min(lcp0, (uint32_t)__shfl_xor(lcp0, off));
     len = max(len, (uint32_t)__shfl_xor(len, off));
+    odd |= (uint32_t)__shfl_xor(odd, off);
   }
   if (__lane_id() == 0) {
     sw[threadIdx.x >> 6] = lcp0;
-    sl[threadIdx.x >> 6] = len;
+    sl[threadIdx.x >> 6] = len | (odd << 31);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     blk_w[blockIdx.x] = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
-    blk_len[blockIdx.x] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
+    const uint32_t m = sl[0] | sl[1] | sl[2] | sl[3];
+    const uint32_t l = max(max(sl[0] & 0xFFFFu, sl[1] & 0xFFFFu), max(sl[2] & 0xFFFFu, sl[3] & 0xFFFFu));
+    blk_len[blockIdx.x] = l | (m & 0x80000000u);
   }
 }
 
 __global__ __launch_bounds__(256) void k_keys_reduce(NuArgs a, const uint32_t *blk_w, const uint32_t *blk_len,
                                                      uint32_t nb) {
-  __shared__ uint32_t sw[4], sl[4];
-  uint32_t w = UINT_MAX, l = 0;
+  __shared__ uint32_t sw[4], sl[4], so[4];
+  uint32_t w = UINT_MAX, l = 0, odd = 0;
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
     w = min(w, blk_w[b]);
-    l = max(l, blk_len[b]);
+    l = max(l, blk_len[b] & 0xFFFFu);
+    odd |= blk_len[b] >> 31;
   }
   for (int off = 32; off > 0; off >>= 1) {
     w = min(w, (uint32_t)__shfl_xor(w, off));
     l = max(l, (uint32_t)__shfl_xor(l, off));
+    odd |= (uint32_t)__shfl_xor(odd, off);
   }
   if (__lane_id() == 0) {
     sw[threadIdx.x >> 6] = w;
     sl[threadIdx.x >> 6] = l;
+    so[threadIdx.x >> 6] = odd;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     a.scal[SC_W] = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
     a.scal[SC_MAXLEN] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
+    // every key as long as key 0, and every one of them packed: the offsets
+    // are i * slot, so the offset scan is skipped
+    const uint32_t L0 = key_range_ok(a, 0) ? (uint32_t)(a.kidx[1] - a.kidx[0] + 1) : 0u;
+    const uint64_t ktot = (uint64_t)a.n * slot_bytes(L0);
+    const bool uni = (so[0] | so[1] | so[2] | so[3]) == 0 && a.n > 0 && L0 > 0 && a.kbase + ktot <= a.key_cap;
+    a.scal[SC_UNIFORM] = uni;
+    if (uni) *(uint64_t *)(a.scal + SC_KTOT) = ktot;
   }
 }
 
@@ -221,6 +262,23 @@ __global__ __launch_bounds__(256) void k_keys_reduce(NuArgs a, const uint32_t *b
 // 16 nibbles from the batch's common prefix W, terminator and padding as 0.
 __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
   __shared__ __align__(16) uint8_t lds[kKeyStage + 16];
+  if (a.scal[SC_UNIFORM]) {
+    // keys already packed in equal slots: the window is read from the packed
+    // bytes (nibble W sits in 8-byte word W / 16 of the slot, and the slot
+    // holds byte W / 2 since W <= L)
+    const uint32_t i = grid_tid();
+    if (i >= a.n || !a.sk0) return;
+    const uint32_t L = a.klen[i];
+    const uint32_t slot = slot_bytes(L);
+    const uint64_t *src = (const uint64_t *)(a.P.keys + a.kbase + (uint64_t)i * slot);
+    const uint32_t W = a.scal[SC_W];
+    const uint32_t q = W / 16, d = W % 16;
+    const uint64_t hi = __builtin_bswap64(src[q]);
+    const uint64_t lo = (q + 1) * 8 < slot ? __builtin_bswap64(src[q + 1]) : 0ull;
+    a.sk0[i] = d ? (hi << (4 * d)) | (lo >> (64 - 4 * d)) : hi;
+    a.sv0[i] = i;
+    return;
+  }
   const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
   if (i >= a.n) return;
@@ -228,15 +286,8 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
   const uint8_t *k = key_range_ok(a, i) ? stg.at(a, a.kidx[2 * i]) : a.kh;
   const uint64_t koff = a.kpoff[i];
   a.kref[i] = koff | ((uint64_t)L << 48);
-  uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + koff);
-  uint32_t nb = (L / 2 + 8) & ~7u;
-  uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
-  for (uint32_t b = 0; b < nb; b += 8) {
-    uint32_t o = 2 * b;  // first nibble of this 8-byte word
-    uint64_t h0 = o < nn ? ldn(k + o, min(8u, nn - o)) : 0ull;
-    uint64_t h1 = o + 8 < nn ? ldn(k + o + 8, min(8u, nn - o - 8)) : 0ull;
-    dst[b / 8] = (uint64_t)pack8(h0) | ((uint64_t)pack8(h1) << 32);
-  }
+  pack_key((uint64_t *)(a.P.keys + a.kbase + koff), k, L);
+  const uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
   if (!a.sk0) return;  // LockNU packs keys without sorting them
   uint32_t W = a.scal[SC_W];
   uint64_t h0 = W < nn ? ldn(k + W, min(8u, nn - W)) : 0ull;
@@ -1165,7 +1216,8 @@ hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s) {
   GMPT_LAUNCH_CHECK();
   k_keys_reduce<<<1, kB, 0, s>>>(a, a.ecount, a.grp, nblk(a.n));
   GMPT_LAUNCH_CHECK();
-  hipError_t e = scan_exclusive_u64(a.kpoff, a.kpoff, a.n, (uint64_t *)(a.scal + SC_KTOT), temp, s);
+  hipError_t e =
+      scan_exclusive_u64(a.kpoff, a.kpoff, a.n, (uint64_t *)(a.scal + SC_KTOT), temp, s, a.scal + SC_UNIFORM);
   if (e != hipSuccess) return e;
   k_keys_pack<<<nblk(a.n), kB, 0, s>>>(a);
   return hipGetLastError();

@@ -19,7 +19,9 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 constexpr size_t kScanDirectTiles = 2048;  // up to this many tiles: two launches per scan
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, size_t n, T *__restrict__ sums) {
+__global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, size_t n, T *__restrict__ sums,
+                                                     const uint32_t *skip) {
+  if (skip && *skip) return;  // uniform over the grid
   __shared__ T sh[4];
   size_t base = (size_t)blockIdx.x * kScanTile;
   T acc = 0;
@@ -39,7 +41,8 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, s
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *__restrict__ out, size_t n,
                                                    const T *__restrict__ offsets, const T *__restrict__ tile_sums,
-                                                   T *__restrict__ total) {
+                                                   T *__restrict__ total, const uint32_t *skip) {
+  if (skip && *skip) return;  // uniform over the grid
   __shared__ T tile[kScanTile];
   __shared__ T sh[4];
   size_t base = (size_t)blockIdx.x * kScanTile;
@@ -92,27 +95,28 @@ size_t scan_temp_elems(size_t n) {
 }
 
 template <typename T>
-hipError_t scan_impl(const T *in, T *out, size_t n, T *total, T *temp, hipStream_t s) {
+hipError_t scan_impl(const T *in, T *out, size_t n, T *total, T *temp, hipStream_t s,
+                     const uint32_t *skip = nullptr) {
   if (n == 0) {
     if (total) return hipMemsetAsync(total, 0, sizeof(T), s);
     return hipSuccess;
   }
   size_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 1) {
-    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, nullptr, total);
+    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, nullptr, total, skip);
     return hipGetLastError();
   }
   T *sums = temp;
   T *rest = temp + nb + 1;
-  k_scan_reduce<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, sums);
+  k_scan_reduce<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, sums, skip);
   GMPT_LAUNCH_CHECK();
   if (nb <= kScanDirectTiles) {  // each tile adds up the sums before it
-    k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, nullptr, sums, total);
+    k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, nullptr, sums, total, skip);
     return hipGetLastError();
   }
-  hipError_t e = scan_impl<T>(sums, sums, nb, nullptr, rest, s);
+  hipError_t e = scan_impl<T>(sums, sums, nb, nullptr, rest, s, skip);
   if (e != hipSuccess) return e;
-  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, nullptr, total);
+  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, nullptr, total, skip);
   return hipGetLastError();
 }
 
@@ -263,8 +267,8 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint3
 }
 
 hipError_t scan_exclusive_u64(const uint64_t *in, uint64_t *out, size_t n, uint64_t *total, void *temp,
-                              hipStream_t s) {
-  return scan_impl<uint64_t>(in, out, n, total, (uint64_t *)temp, s);
+                              hipStream_t s, const uint32_t *skip) {
+  return scan_impl<uint64_t>(in, out, n, total, (uint64_t *)temp, s, skip);
 }
 
 hipError_t radix_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int begin_bit,
