@@ -145,38 +145,62 @@ __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
 // aggregated per block in LDS so each block issues one global atomic per bin
 // it touches (a global atomic per node serialises on the few hot bins).
 
+// A leaf whose key remainder or value is beyond the word builder's reach
+// (metadata word m0) needs the generic encoder and a scratch slot.
+__device__ __forceinline__ bool leaf_too_long(uint64_t m0) {
+  return ((m0 >> 41) & 255u) > kLeafFastNibbles || (m0 >> 50) > kLeafDirectMaxValue;
+}
+
+// Each thread takes kHcTile / 256 entries; their list and leaf-record loads
+// are issued together before any is used (the block count is kept low for
+// the global histogram, so each thread's loads must overlap).
+constexpr int kHcPer = (int)(kHcTile / 256);
+
 __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restrict__ list, uint32_t n,
                                                  uint32_t *__restrict__ hist, uint32_t *__restrict__ packed,
                                                  uint64_t *__restrict__ lmeta_tmp, LeafRecords rec) {
   __shared__ uint32_t h[kHcBins];
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) h[b] = 0;
-  __syncthreads();
-  for (uint32_t i = blockIdx.x * kHcTile + threadIdx.x; i < min(n, (blockIdx.x + 1) * kHcTile); i += blockDim.x) {
-    ref_t r = list[i];
-    uint32_t pk = 0xffffffffu;  // builds leave null holes
-    // a leaf the last build created: its bin and metadata were recorded then
+  const uint32_t i0 = blockIdx.x * kHcTile + threadIdx.x;
+  ref_t r[kHcPer];
+  uint32_t rpk[kHcPer];
+  uint4 rm[kHcPer];  // a recorded leaf's metadata
+#pragma unroll
+  for (int q = 0; q < kHcPer; q++) {
+    const uint32_t i = i0 + 256u * q;
     const uint32_t ri = i - rec.lo;
-    const uint32_t rpk = rec.pk && ri < rec.n ? rec.pk[ri] : kLeafPackedNone;
-    if (r != REF_NULL) {
-      uint32_t nb;
-      uint32_t bin;
-      if (rpk != kLeafPackedNone) {
-        bin = rpk & 0xfff;
-        nb = rpk >> 12;
+    r[q] = i < n ? list[i] : REF_NULL;  // builds leave null holes
+    // a leaf the last build created: its bin and metadata were recorded then
+    rpk[q] = i < n && rec.pk && ri < rec.n ? rec.pk[ri] : kLeafPackedNone;
+  }
+#pragma unroll
+  for (int q = 0; q < kHcPer; q++) {
+    const uint32_t ri = i0 + 256u * q - rec.lo;
+    rm[q] = lmeta_tmp && rpk[q] != kLeafPackedNone ? *(const uint4 *)(rec.meta + 2 * (size_t)ri)
+                                                   : make_uint4(0, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kHcPer; q++) {
+    const uint32_t i = i0 + 256u * q;
+    if (i >= n) break;
+    uint32_t pk = 0xffffffffu;
+    if (r[q] != REF_NULL) {
+      uint32_t nb, bin;
+      if (rpk[q] != kLeafPackedNone) {
+        bin = rpk[q] & 0xfff;
+        nb = rpk[q] >> 12;
       } else {
-        bin = bin_of(P, r, &nb);
+        bin = bin_of(P, r[q], &nb);
       }
       if (lmeta_tmp && bin < kLdsLeafBins) {
         uint64_t *m = lmeta_tmp + 2 * (size_t)i;
-        if (rpk != kLeafPackedNone) {
-          m[0] = rec.meta[2 * (size_t)ri];
-          m[1] = rec.meta[2 * (size_t)ri + 1];
+        if (rpk[q] != kLeafPackedNone) {
+          *(uint4 *)m = rm[q];
         } else {
-          leaf_meta(P, r, m);
+          leaf_meta(P, r[q], m);
         }
-        // a key remainder or value over the word builder's reach needs the
-        // generic encoder and a scratch slot: level 1, same class
-        if (((m[0] >> 41) & 255u) > kLeafFastNibbles || (m[0] >> 50) > kLeafDirectMaxValue) bin += kHcClasses;
+        if (leaf_too_long(m[0])) bin += kHcClasses;
       }
       atomicAdd(&h[bin], 1u);
       pk = bin | (min(nb, kPackedNbMax) << 12);
