@@ -540,3 +540,39 @@ def test_prefix_keys_incremental(gmpt, workloads, oracle, modes):
         assert t.get_root_hash() == o.root()
     allb = workloads.batch_from_bytes(keys, [b"x"] * len(keys))
     assert t.get_values(allb) == o.gets(allb)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_equal_length_key_packing_edges(gmpt, workloads, oracle, mode):
+    """Batches whose keys all have key 0's length are packed by the validation
+    pass (equal slots, sort windows read from the packed bytes); one key of
+    another length anywhere, key 0 included, sends the batch down the general
+    packing.  Cases: every key identical (the common prefix is the whole key),
+    a shared prefix of 37 nibbles (the window starts mid-word), key 0 longer
+    than the rest, one shorter key in the middle, and keys whose length is
+    odd in nibbles."""
+    rng = random.Random(71)
+
+    def vals(n):
+        return [bytes([rng.getrandbits(8)]) * rng.randint(1, 90) for _ in range(n)]
+
+    same = [bytes(range(32))] * 500
+    pre = bytes(rng.getrandbits(8) for _ in range(18)) + b"\x50"  # 37 shared nibbles + 1
+    shared = list({pre + bytes(rng.getrandbits(8) for _ in range(13)) for _ in range(3000)})
+    uni = list({bytes(rng.getrandbits(8) for _ in range(20)) for _ in range(3000)})
+    long0 = [bytes(rng.getrandbits(8) for _ in range(21))] + uni
+    mid = uni[:1500] + [uni[1500][:7]] + uni[1501:]
+    cases = [("identical", same), ("shared-37", shared), ("key0-longer", long0), ("one-short", mid)]
+    for name, keys in cases:
+        b = workloads.batch_from_bytes(keys, vals(len(keys)))
+        t = _build(gmpt, b, mode)
+        assert t.get_root_hash() == oracle.root_of(b), name
+        last = {}
+        for i in range(b.n):
+            last[b.key(i)] = b.value(i)
+        assert t.get_values(b) == [last[b.key(i)] for i in range(b.n)], name
+    # odd nibble counts: hex keys of 2 x 9 + 1 nibbles cut to 18 (17 + terminator)
+    hk = list({bytes(rng.randrange(16) for _ in range(17)) + b"\x10" for _ in range(2500)})
+    b = workloads.batch_from_hex(hk, vals(len(hk)))
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash() == oracle.root_of(b)
