@@ -39,6 +39,7 @@ size_t nu_ws_bytes(size_t n, bool tries) {
   size_t b = 0;
   b += n * (4 + 8 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
   b += ne * (4 + 16);  // leaf records
+  b += n * 8 * 5;      // sorted krefs; unique-order krefs and windows (two copies each)
   b += n * (4 * 7 + 1 + sizeof(Stop));
   b += ne * (4 + 4 + 4 + 4) + mt_bytes(align_up(ne, 256));
   b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
@@ -340,6 +341,11 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.ecount = c.take<uint32_t>(n);
   a.merge = c.take<uint32_t>(n);
   a.u_idx2 = c.take<uint32_t>(n);
+  a.skref = c.take<uint64_t>(n);
+  a.u_kref = c.take<uint64_t>(n);
+  a.u_kref2 = c.take<uint64_t>(n);
+  a.u_win = c.take<uint64_t>(n);
+  a.u_win2 = c.take<uint64_t>(n);
   a.u_trie2 = c.take<uint32_t>(n);
   a.perm0 = c.take<uint32_t>(n);
   a.perm1 = c.take<uint32_t>(n);
@@ -395,7 +401,10 @@ int Forest::sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int
   }
   HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, pass == 1, cur, vmask, &sk_mask));
   HIP_TRY(nu_adjacent(a, sk, sv, pass == 2 ? 1 : 0, stream_, sk_mask));
-  HIP_TRY(nu_unique(a, sv, temp, stream_));
+  // the sorted keys are the plain windows unless the whole keys were sorted
+  // or the windows were compressed to their varying bits
+  a.u_win_ok = pass != 2 && sk_mask == 0;
+  HIP_TRY(nu_unique(a, sk, sv, temp, stream_));
   return 0;
 }
 

@@ -74,6 +74,12 @@ struct NuArgs {
   uint32_t *ecount;      // E entries emitted per unique key, scanned to positions
   uint32_t *merge;       // per run: run whose group it joins (split groups around a subtree)
   uint32_t *u_idx2, *u_trie2, *perm0, *perm1, *gkey0, *gkey1;
+  // each unique key's kref and sort window, gathered in unique order so the
+  // build reads them in sequence (u_win holds the 16 nibbles from W only when
+  // u_win_ok: the last sort's keys were the plain windows)
+  uint64_t *skref;       // per sorted position: kref of that key
+  uint64_t *u_kref, *u_kref2, *u_win, *u_win2;
+  uint32_t u_win_ok;
   uint8_t *u_lcp2;
   Stop *stop2;
   // E elements (M <= 2m)
@@ -136,7 +142,7 @@ hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStrea
 hipError_t nu_window_mask(const NuArgs &a, const uint64_t *sk, hipStream_t s);
 hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s,
                        uint64_t sk_mask = 0);
-hipError_t nu_unique(const NuArgs &a, const uint32_t *sv, void *temp, hipStream_t s);
+hipError_t nu_unique(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, void *temp, hipStream_t s);
 // Descends every unique key and groups them by stop point; sets SC_MERGE when
 // a split group is interrupted by keys below the split node (nu_regroup).
 hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s);

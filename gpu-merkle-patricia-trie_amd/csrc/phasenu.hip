@@ -369,6 +369,7 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
   uint64_t ra = (uint64_t)__shfl_up((long long)rb, 1);
   uint32_t ta = (uint32_t)__shfl_up((int)tb, 1);
   if (!active) return;
+  a.skref[j] = rb;
   if (__lane_id() == 0 && j > 0) {
     const uint32_t ia = sv[j - 1];
     ra = a.kref[ia];
@@ -418,7 +419,8 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
 // After the inclusive scan of run starts: run r spans sorted positions with
 // run[j] == r+1.  The first position carries the LCP to the previous distinct
 // key, the last position the surviving (last-written) input index.
-__global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint32_t *sv, const uint32_t *run_excl) {
+__global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint64_t *sk, const uint32_t *sv,
+                                                const uint32_t *run_excl) {
   uint32_t j = grid_tid();
   if (j >= a.n) return;
   bool first = (a.flags[j] & 1) == 0;
@@ -431,7 +433,11 @@ __global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint32_t *sv, co
     a.u_first[r] = (a.flags[j] >> 1) & 1;
     a.u_trie[r] = trie_of(a, sv[j]);
   }
-  if (last) a.u_idx[r] = sv[j];
+  if (last) {
+    a.u_idx[r] = sv[j];
+    a.u_kref[r] = a.skref[j];
+    if (a.u_win_ok) a.u_win[r] = sk[j];
+  }
   if (j + 1 == a.n) a.scal[SC_M_UNIQ] = r + 1;
 }
 
@@ -683,6 +689,8 @@ __global__ __launch_bounds__(256) void k_permute(NuArgs a, const uint32_t *perm,
   if (i >= m) return;
   uint32_t p = perm[i];
   a.u_idx2[i] = a.u_idx[p];
+  a.u_kref2[i] = a.u_kref[p];
+  if (a.u_win_ok) a.u_win2[i] = a.u_win[p];
   a.u_trie2[i] = a.u_trie[p];
   a.u_lcp2[i] = a.u_lcp[p];
   a.stop2[i] = a.stop[p];
@@ -982,7 +990,9 @@ struct ElemKey {
   uint32_t end;       // absolute end of the element's path (exclusive)
   uint8_t term;       // path ends with the terminator
   bool virt;
+  bool has_win;       // win holds nibbles W .. W + 15 of the key
   uint32_t uidx;      // unique index (new key) or short node idx (virtual)
+  uint64_t win;
 };
 
 __device__ __forceinline__ ElemKey elem_key(const NuArgs &a, uint32_t e) {
@@ -996,22 +1006,30 @@ __device__ __forceinline__ ElemKey elem_key(const NuArgs &a, uint32_t e) {
     k.end = S.depth + S.key_len;
     k.term = S.term;
     k.virt = true;
+    k.has_win = false;
+    k.win = 0;
     k.uidx = src & 0x7fffffffu;
   } else {
-    const uint64_t r = a.kref[a.u_idx[src]];
+    const uint64_t r = a.u_kref[src];
     k.off = a.kbase + (r & kKrefOff);
     k.start = 0;
     k.origin = 0;
     k.end = (uint32_t)(r >> 48);
     k.term = 1;
     k.virt = false;
+    k.has_win = a.u_win_ok != 0;
+    k.win = k.has_win ? a.u_win[src] : 0ull;
     k.uidx = src;
   }
   return k;
 }
 
-__device__ __forceinline__ uint32_t elem_nib(const Pools &P, const ElemKey &k, uint32_t j) {
+// Nibble j of the element's path; W is the batch's common prefix (the sort
+// window's first nibble): nibbles W .. W + 15 of a new key come from its
+// window when the window is at hand.
+__device__ __forceinline__ uint32_t elem_nib(const Pools &P, const ElemKey &k, uint32_t j, uint32_t W) {
   if (k.term && j + 1 == k.end) return kNibbleT;
+  if (k.has_win && j >= W && j - W < 16) return (uint32_t)(k.win >> (60 - 4 * (j - W))) & 15u;
   return packed_nibble(P.keys + k.off, k.start + (j - k.origin));
 }
 
@@ -1060,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
     uint32_t p = La >= Lb ? (uint32_t)lo : hi;
     uint32_t Lp = max(La, Lb) - 1;
     parent_bid = a.base_branch + a.bid[a.repof[p]];
-    slot = elem_nib(a.P, ek, Lp);
+    slot = elem_nib(a.P, ek, Lp, a.scal[SC_W]);
     lo_depth = Lp + 1;
   }
   uint32_t rank = a.bid[j];
@@ -1116,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
     uint32_t p = La >= Lb ? e - 1 : e;
     uint32_t Lp = max(La, Lb) - 1;
     parent_bid = a.base_branch + a.bid[a.repof[p]];
-    slot = elem_nib(a.P, ek, Lp);
+    slot = elem_nib(a.P, ek, Lp, a.scal[SC_W]);
     start = Lp + 1;
   }
   ref_t child;
@@ -1382,11 +1400,11 @@ hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, 
   return hipGetLastError();
 }
 
-hipError_t nu_unique(const NuArgs &a, const uint32_t *sv, void *temp, hipStream_t s) {
+hipError_t nu_unique(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, void *temp, hipStream_t s) {
   // run[] holds run-start flags; exclusive scan gives each start its run id
   hipError_t e = scan_exclusive_u32(a.run, a.run, a.n, nullptr, temp, s);
   if (e != hipSuccess) return e;
-  k_unique<<<nblk(a.n), kB, 0, s>>>(a, sv, a.run);
+  k_unique<<<nblk(a.n), kB, 0, s>>>(a, sk, sv, a.run);
   return hipGetLastError();
 }
 
@@ -1424,6 +1442,8 @@ hipError_t nu_regroup(NuArgs &a, uint32_t m, uint32_t nruns, void *temp, hipStre
   GMPT_LAUNCH_CHECK();
   uint32_t *t;
   t = a.u_idx; a.u_idx = a.u_idx2; a.u_idx2 = t;
+  uint64_t *t64 = a.u_kref; a.u_kref = a.u_kref2; a.u_kref2 = t64;
+  t64 = a.u_win; a.u_win = a.u_win2; a.u_win2 = t64;
   t = a.u_trie; a.u_trie = a.u_trie2; a.u_trie2 = t;
   uint8_t *l = a.u_lcp; a.u_lcp = a.u_lcp2; a.u_lcp2 = l;
   Stop *st = a.stop; a.stop = a.stop2; a.stop2 = st;
