@@ -896,6 +896,7 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
     void *temp = nullptr;
     if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
     a.seq_base = lock_seq_;
+    a.skref = nullptr;  // the latched insert reads keys through kref only
     uint32_t trie_bits = 0;
     if (has_tid) {
       while ((1ull << trie_bits) < ntries_) trie_bits++;

@@ -77,7 +77,7 @@ struct NuArgs {
   // each unique key's kref and sort window, gathered in unique order so the
   // build reads them in sequence (u_win holds the 16 nibbles from W only when
   // u_win_ok: the last sort's keys were the plain windows)
-  uint64_t *skref;       // per sorted position: kref of that key
+  uint64_t *skref;       // per sorted position: kref of that key (null: not gathered)
   uint64_t *u_kref, *u_kref2, *u_win, *u_win2;
   uint32_t u_win_ok;
   uint8_t *u_lcp2;

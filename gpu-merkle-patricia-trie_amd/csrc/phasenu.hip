@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
   uint64_t ra = (uint64_t)__shfl_up((long long)rb, 1);
   uint32_t ta = (uint32_t)__shfl_up((int)tb, 1);
   if (!active) return;
-  a.skref[j] = rb;
+  if (a.skref) a.skref[j] = rb;
   if (__lane_id() == 0 && j > 0) {
     const uint32_t ia = sv[j - 1];
     ra = a.kref[ia];
@@ -435,8 +435,10 @@ __global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint64_t *sk, co
   }
   if (last) {
     a.u_idx[r] = sv[j];
-    a.u_kref[r] = a.skref[j];
-    if (a.u_win_ok) a.u_win[r] = sk[j];
+    if (a.skref) {
+      a.u_kref[r] = a.skref[j];
+      if (a.u_win_ok) a.u_win[r] = sk[j];
+    }
   }
   if (j + 1 == a.n) a.scal[SC_M_UNIQ] = r + 1;
 }
