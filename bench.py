@@ -67,17 +67,46 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline_build(batch, label):
-    """The oracle (C restatement of the reference CPU MPT) timed on one core."""
+class _OneCore:
+    """Pins the process to one core for a CPU baseline, restoring the
+    previous affinity afterwards (the serving runs use host threads)."""
+
+    def __enter__(self):
+        self.prev = None
+        try:
+            self.prev = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {sorted(self.prev)[0]})
+        except (AttributeError, OSError):
+            pass
+        return self
+
+    def __exit__(self, *exc):
+        if self.prev:
+            try:
+                os.sched_setaffinity(0, self.prev)
+            except OSError:
+                pass
+        return False
+
+
+def cpu_baseline_build(batch, label, runs=5):
+    """The oracle (C restatement of the reference CPU MPT) timed on one core:
+    one warm-up build, then the median of `runs` timed builds (insert + hash
+    in one timed region, as the reference's CPU baseline times it)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-    except (AttributeError, OSError):
-        pass
-    secs, root = oracle.time_build(batch)
-    return {"value": batch.n / secs, "unit": "puts+root-hash/s", "cores": 1, "kind": "port",
-            "sample": f"{label}: full workload, puts_baseline + hashs_dirty_flag, 1 run, {secs:.2f} s",
+    secs = []
+    with _OneCore():
+        _, root = oracle.time_build(batch)  # warm-up
+        for _ in range(runs):
+            dt, r = oracle.time_build(batch)
+            if r != root:
+                raise SystemExit("bench: oracle roots differ between runs")
+            secs.append(dt)
+    med = float(np.median(secs))
+    return {"value": batch.n / med, "unit": "puts+root-hash/s", "cores": 1, "kind": "port",
+            "sample": f"{label}: puts_baseline + hashs_dirty_flag, 1 warm-up + median of {runs} runs "
+                      f"({med:.2f} s each, {min(secs):.2f}-{max(secs):.2f} s)",
             "cpu_model": cpu_model(), "nproc": os.cpu_count()}, root
 
 
@@ -187,8 +216,8 @@ def bench_phase(pkg, args):
     else:
         out["cpu_baseline"] = None
     out["gen_s"] = gen_s
-    db.free()
     t.close()
+    db.free()
     return out
 
 
@@ -269,21 +298,19 @@ def bench_batched(pkg, args):
         "roots_checked_vs_oracle": len(sample), "gen_s": gen_s,
     }
     if not args.no_cpu_baseline:
-        # sequential oracle build + hash of whole tries, on one core, ~10 s sample
-        try:
-            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-        except (AttributeError, OSError):
-            pass
+        # sequential oracle build + hash of whole tries, on one core, a
+        # bounded sample (cpu_seconds of builds)
         order = rng.permutation(ntries)
         done, keys, secs = 0, 0, 0.0
-        for i in order:
-            sub = b.subset(range(int(starts[i]), int(starts[i] + sizes[i])))
-            dt, _ = oracle.time_build(sub)
-            secs += dt
-            done += 1
-            keys += int(sizes[i])
-            if secs > 10.0:
-                break
+        with _OneCore():
+            for i in order:
+                sub = b.subset(range(int(starts[i]), int(starts[i] + sizes[i])))
+                dt, _ = oracle.time_build(sub)
+                secs += dt
+                done += 1
+                keys += int(sizes[i])
+                if secs > args.cpu_seconds:
+                    break
         out["cpu_baseline"] = {"value": done / secs, "unit": "tries/s", "cores": 1, "kind": "port",
                                "sample": f"{done} random whole tries ({keys} keys), puts + hash each, {secs:.1f} s",
                                "cpu_model": cpu_model(), "nproc": os.cpu_count()}
@@ -341,17 +368,14 @@ def bench_lock(pkg, args):
     }
     if not args.no_cpu_baseline:
         oracle = _oracle()
-        try:
-            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-        except (AttributeError, OSError):
-            pass
         o = oracle.OracleMPT()
-        o.puts(pre)
-        o.hash()
-        w0 = time.perf_counter()
-        o.puts(ops)
-        o.hash()
-        secs = time.perf_counter() - w0
+        with _OneCore():
+            o.puts(pre)
+            o.hash()
+            w0 = time.perf_counter()
+            o.puts(ops)
+            o.hash()
+            secs = time.perf_counter() - w0
         if o.root() != roots[-1]:
             raise SystemExit(f"bench: GPU root {roots[-1].hex()} != oracle root {o.root().hex()}")
         out["root_matches_oracle"] = True
@@ -523,6 +547,42 @@ def bench_hbm_copy(pkg, args):
             "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "cpu_baseline": None}
 
 
+def _sub_args(args, **kw):
+    d = dict(vars(args))
+    d.update(kw)
+    return argparse.Namespace(**d)
+
+
+def batched_summary(pkg, args):
+    """The batched part of the metric for the default line: 4,096 uneven tries
+    in one multi-trie pass against one plain call per trie."""
+    r = bench_batched(pkg, _sub_args(args, steps=5, warmup=2, cpu_seconds=5.0))
+    return {"workload": r["config"]["workload"], "tries": r["config"]["tries"], "keys": r["config"]["keys"],
+            "tries_per_s": r["value"], "ms_per_pass": r["ms_per_step"], "mputs_per_s": r["mputs_per_s"],
+            "plain_tries_per_s": r["plain"]["tries_per_s"], "plain_ms_per_pass": r["plain"]["ms_per_pass"],
+            "speedup_vs_plain": r["speedup_vs_plain"], "roots_equal_plain": True,
+            "roots_checked_vs_oracle": r["roots_checked_vs_oracle"], "per_step": r["per_step"],
+            "cpu_baseline": r["cpu_baseline"]}
+
+
+def served_summary(pkg, args):
+    """The served part of the metric for the default line: continuous
+    batching over 1,024 resident tries at 0.7 x its measured saturation, next
+    to one plain insert + hash call per request at 0.7 x theirs."""
+    r = bench_served(pkg, _sub_args(args, duration=args.served_duration, plain_duration=args.served_plain_duration,
+                                    rates=[0.7]))
+    run = r["runs"][0]
+    return {"workload": r["config"]["workload"], "lanes": r["config"]["lanes"],
+            "puts_per_s": run["puts_per_s"], "requests_per_s": run["requests_per_s"],
+            "p50_ms": run["p50_ms"], "p99_ms": run["p99_ms"], "requests": run["requests"],
+            "requests_per_batch": run["requests_per_batch"], "deferred_by_memory": run["deferred_by_memory"],
+            "saturation": r["saturation"],
+            "plain": {k: r["plain"][k] for k in ("saturation_requests_per_s", "offered_rps", "requests_per_s",
+                                                  "p50_ms", "p99_ms", "how")},
+            "speedup_vs_plain_saturation": r["speedup_vs_plain_saturation"],
+            "duration_s": args.served_duration, "plain_duration_s": args.served_plain_duration}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,6 +598,13 @@ def main():
     ap.add_argument("--backlog", type=int, default=400, help="served: requests of the saturation run")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="batched: seconds of the CPU baseline sample")
+    ap.add_argument("--no-batched", action="store_true", help="phase: leave out the batched sub-object")
+    ap.add_argument("--no-served", action="store_true", help="phase: leave out the served sub-object")
+    ap.add_argument("--served-duration", type=float, default=10.0,
+                    help="phase: seconds of the served sub-object's offered-load run")
+    ap.add_argument("--served-plain-duration", type=float, default=5.0,
+                    help="phase: seconds of the served sub-object's plain-call run")
     ap.add_argument("--no-fuse-leaves", action="store_true",
                     help="phase: leaf level as encode kernel + HBM scratch + permutation kernel")
     args = ap.parse_args()
@@ -548,6 +615,14 @@ def main():
     pkg = _import_package()
     out = {"phase": bench_phase, "batched": bench_batched, "lock": bench_lock,
            "served": bench_served, "hbm-copy": bench_hbm_copy}[args.config](pkg, args)
+    if args.config == "phase":
+        # the metric's other two parts: batched vs plain tries/s, served p99
+        if not args.no_batched:
+            print("bench: batched sub-object", file=sys.stderr, flush=True)
+            out["batched"] = batched_summary(pkg, args)
+        if not args.no_served:
+            print("bench: served sub-object", file=sys.stderr, flush=True)
+            out["served"] = served_summary(pkg, args)
     print(json.dumps(out), flush=True)
 
 

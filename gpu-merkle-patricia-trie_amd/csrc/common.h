@@ -100,6 +100,9 @@ enum : uint32_t {
   ERR_POOL = 2,           // a pool ran out (the host sized it too small)
   ERR_INTERNAL = 3,
 };
+// err_arg of the ERR_POOL raised when a batch's packed keys exceed the key
+// arena (the host then grows the arena to the batch's packed total)
+constexpr uint32_t kErrArgKeyArena = 6;
 
 // Device pool views passed to kernels by value.
 struct Pools {

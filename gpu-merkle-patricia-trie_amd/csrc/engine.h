@@ -26,6 +26,7 @@ struct BatchDesc {
   uint32_t n = 0;
   uint64_t keys_bytes = 0;                 // totals (computed from the indexs for host batches)
   uint64_t values_total = 0;
+  uint64_t key_slots = 0;                  // key arena bytes of the packed keys (set by stage_batch)
   bool on_device = false;                  // all pointers above are device pointers
   bool adopt_values = false;               // device batch: nodes point into values_bytes (caller keeps it alive)
 };
@@ -108,6 +109,7 @@ class Forest {
   int grow(DevBuf &b, size_t need, size_t keep);
   int ensure_ws(DevBuf &b, size_t bytes);
   int sync_counters();
+  bool grow_key_arena_after_error();
   int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
   int carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid, void **temp);
   int sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int *cur);

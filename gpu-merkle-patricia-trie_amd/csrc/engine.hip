@@ -239,10 +239,14 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
   if (!b.on_device) {
     // extents of the host buffers: the largest end index of any pair (the
     // kernels reject pairs outside them)
-    uint64_t kb = 0, vb = 0;
+    // and the key arena bytes the packed keys take (pairs may overlap, so
+    // the key buffer's size does not bound it)
+    uint64_t kb = 0, vb = 0, ks = 0;
     for (uint32_t i = 0; i < n; i++) {
-      int64_t e = b.keys_indexs[2 * (size_t)i + 1];
+      const int64_t s = b.keys_indexs[2 * (size_t)i], e = b.keys_indexs[2 * (size_t)i + 1];
       if (e >= 0 && (uint64_t)e + 1 > kb) kb = (uint64_t)e + 1;
+      const int64_t L = (s >= 0 && e >= s && e - s < kMaxKeyNibbles) ? e - s + 1 : 1;
+      ks += (uint64_t)((L / 2 + 8) & ~7ll);
       if (b.values_indexs) {
         int64_t ve = b.values_indexs[2 * (size_t)i + 1];
         if (ve >= 0 && (uint64_t)ve + 1 > vb) vb = (uint64_t)ve + 1;
@@ -250,6 +254,7 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
     }
     d->keys_bytes = kb;
     d->values_total = vb;
+    d->key_slots = ks;
     int r;
     if ((r = ensure_ws(st_keys_, kb + 64))) return r;
     if ((r = ensure_ws(st_kidx_, 8ull * n + 64))) return r;
@@ -286,8 +291,21 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
       HIP_TRY(hipMemcpyAsync(vs, b.values_bytes, b.values_total, hipMemcpyDeviceToDevice, stream_));
     d->values_bytes = vs;
   }
+  // device batches: the packed keys of non-overlapping pairs; the packing
+  // pass reports a larger total (kErrArgKeyArena) and the insert is re-run
+  if (b.on_device) d->key_slots = b.keys_bytes / 2 + 8ull * n;
   *vbase = (uint64_t)(uintptr_t)d->values_bytes;
   return 0;
+}
+
+// After a failed synchronisation: was it the packing pass finding the key
+// arena too small?  Then the arena is grown to the batch's packed total
+// (kept in the scalars) and true is returned, so the caller runs the insert
+// again.  The failed run changed nothing but dirty flags of path nodes.
+bool Forest::grow_key_arena_after_error() {
+  if (cnt_.err != ERR_POOL || cnt_.err_arg != kErrArgKeyArena) return false;
+  const uint64_t ktot = *(const uint64_t *)(cnt_.scal + SC_KTOT);
+  return grow(keys_, key_used_ + ktot + 64, key_used_) == 0;
 }
 
 // Workspace of one PhaseNU-style call: every per-key array of NuArgs carved
@@ -441,28 +459,32 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     if ((r = stage_batch(b, &d, &vbase, true))) return r;
     const uint32_t n = b.n;
     // slots: branches <= 2n+2, shorts <= 2(2n+2), values: 2n+2 for elements + n for updates
-    if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, 3ull * n + 2, d.keys_bytes / 2 + 8ull * n))) return r;
+    if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, 3ull * n + 2, d.key_slots))) return r;
     const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
     NuArgs a;
     void *temp = nullptr;
-    if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
     uint32_t trie_bits = 0;
     if (has_tid) {
       while ((1ull << trie_bits) < ntries_) trie_bits++;
     }
-    HIP_TRY(nu_prepare_keys(a, temp, stream_));
-    int cur = 0;
-    for (int pass = 0; pass >= 0;) {
-      if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) return r;
-      HIP_TRY(nu_descend_and_group(a, temp, stream_));
-      if ((r = sync_counters())) return r;
-      pass = next_pass(pass);
-      if (pass >= 0) {
-        // redo from the sort; marks the discarded descent made are on the
-        // batch's own paths and stay valid
-        HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
-        HIP_TRY(hipMemsetAsync(a.scal + SC_MERGE, 0, 4, stream_));
+    for (int run = 0;; run++) {
+      if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
+      HIP_TRY(nu_prepare_keys(a, temp, stream_));
+      int cur = 0;
+      for (int pass = 0; pass >= 0;) {
+        if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) break;
+        HIP_TRY(nu_descend_and_group(a, temp, stream_));
+        if ((r = sync_counters())) break;
+        pass = next_pass(pass);
+        if (pass >= 0) {
+          // redo from the sort; marks the discarded descent made are on the
+          // batch's own paths and stay valid
+          HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+          HIP_TRY(hipMemsetAsync(a.scal + SC_MERGE, 0, 4, stream_));
+        }
       }
+      if (r == 0) break;
+      if (run > 0 || !grow_key_arena_after_error()) return r;
     }
     if (cnt_.scal[SC_MERGE]) {
       // a split group straddles a subtree: make every group contiguous
@@ -890,30 +912,37 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
     uint64_t vbase = 0;
     if ((r = stage_batch(b, &d, &vbase, true))) return r;
     const uint32_t n = b.n;
-    if ((r = ensure_pools(n, 2ull * n, n, d.keys_bytes / 2 + 8ull * n))) return r;
+    if ((r = ensure_pools(n, 2ull * n, n, d.key_slots))) return r;
     const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
     NuArgs a;
     void *temp = nullptr;
-    if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
-    a.seq_base = lock_seq_;
-    a.skref = nullptr;  // the latched insert reads keys through kref only
     uint32_t trie_bits = 0;
     if (has_tid) {
       while ((1ull << trie_bits) < ntries_) trie_bits++;
     }
-    HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
-    HIP_TRY(nu_prepare_keys(a, temp, stream_));
-    int cur = 0;
-    for (int pass = 0; pass >= 0;) {
-      if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) return r;
-      if ((r = sync_counters())) return r;
-      pass = next_pass(pass);
-      if (pass >= 0) HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+    for (int run = 0;; run++) {
+      if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
+      a.seq_base = lock_seq_;
+      a.skref = nullptr;  // the latched insert reads keys through kref only
+      HIP_TRY(hipMemsetAsync(&d_cnt_->n_restarts, 0, 8, stream_));
+      HIP_TRY(nu_prepare_keys(a, temp, stream_));
+      int cur = 0;
+      for (int pass = 0; pass >= 0;) {
+        if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) break;
+        if ((r = sync_counters())) break;
+        pass = next_pass(pass);
+        if (pass >= 0) HIP_TRY(hipMemsetAsync(&d_cnt_->flag_fallback, 0, 4, stream_));
+      }
+      if (r == 0) break;
+      if (run > 0 || !grow_key_arena_after_error()) return r;
     }
-    uint64_t ktot = *(const uint64_t *)(cnt_.scal + SC_KTOT);
-    HIP_TRY(lock_insert(a, key_used_ + ktot, stream_));
+    const uint64_t ktot = *(const uint64_t *)(cnt_.scal + SC_KTOT);
+    const uint32_t values_before = cnt_.n_value;
+    HIP_TRY(lock_insert(a, key_used_ + ktot, temp, stream_));
     if ((r = sync_counters())) return r;
-    key_used_ = align_up(key_used_ + ktot, 8);
+    // only new keys' nodes refer to the batch's packed keys: a batch that
+    // only overwrites leaves the arena where it was
+    if (cnt_.n_value != values_before) key_used_ = align_up(key_used_ + ktot, 8);
     lock_seq_ += n;
     stats_.restarts = cnt_.n_restarts;
     stats_.lock_fails = cnt_.n_lock_fail;

@@ -4,9 +4,10 @@
 
 namespace gmpt {
 
-// Inserts a.n keys (already packed by nu_prepare_keys) into their tries.
-// Uses a.base_branch / base_short / base_value: per key i the slots
-// base_branch + i, base_short + 2i (+1), base_value + i.
-hipError_t lock_insert(const NuArgs &a, uint64_t key_cap, hipStream_t s);
+// Inserts the batch's unique keys (packed by nu_prepare_keys, sorted and
+// deduplicated) into their tries.  The k-th new key (in sorted order) takes
+// the slots base_branch + k, base_short + 2k (+1), base_value + k; keys that
+// exist are rewritten in place.  temp: scan workspace.
+hipError_t lock_insert(const NuArgs &a, uint64_t key_cap, void *temp, hipStream_t s);
 
 }  // namespace gmpt

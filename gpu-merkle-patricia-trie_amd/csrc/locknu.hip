@@ -18,9 +18,12 @@
 // writer wins, as the sequential CPU insert does): a value is replaced only by
 // a value with a larger order stamp.
 //
-// Node slots are pre-assigned per key (one branch, two short nodes, one value
-// node), so the insert needs no allocator atomics; slots that a key does not
-// use stay unreferenced.
+// Writes are combined first (one lane per distinct key), so whether a key
+// exists is fixed before the insert starts: a read-only descent classifies
+// every key.  An existing key's value node is rewritten in place (only that
+// key refers to it); a new key gets node slots by its rank among the new keys
+// (one branch, two short nodes, one value node), so the insert needs no
+// allocator atomics and a batch of overwrites allocates nothing.
 #include "locknu.h"
 #include "prims.h"
 
@@ -95,6 +98,8 @@ struct Ctx {
   const uint8_t *kp;  // packed nibbles of the key (8-byte aligned)
   uint32_t len;       // hex length, terminator included
   uint32_t i;         // batch position
+  uint32_t slot;      // rank among the batch's new keys (its node slots)
+  bool fresh;         // the key does not exist yet
   uint32_t trie;
   uint64_t seq;
   uint64_t key_off;   // this key's packed bytes in the key arena
@@ -118,6 +123,13 @@ __device__ __forceinline__ void init_value(const NuArgs &a, const Ctx &c, uint32
   st(w + 1, a.hps ? (uint64_t)a.hps[c.i] : 0ull);     // hp
   st(w + 2, (uint64_t)(uint32_t)(ve - vs + 1));       // size, pad
   st(w + 3, c.seq);                                   // seq
+}
+
+// The existing value node `vid` takes this write's value (in place: the
+// key's leaf is its only referrer, and the write is the key's only one in
+// the batch).
+__device__ __forceinline__ void rewrite_value(const NuArgs &a, const Ctx &c, uint32_t vid) {
+  if (ld(&a.P.value[vid].seq) < c.seq) init_value(a, c, vid);
 }
 
 // A fresh short node, written as its eight words (layout of Short).
@@ -160,17 +172,25 @@ __device__ __forceinline__ void publish(ref_t *slot, ref_t r) {
 // New leaf for the key's nibbles [start, len): a short node over a value node,
 // or the value node alone when nothing is left (key ends at a branch slot 16).
 __device__ __forceinline__ ref_t make_leaf(const NuArgs &a, const Ctx &c, uint32_t start, bool *made_short) {
-  uint32_t vid = a.base_value + c.i;
+  uint32_t vid = a.base_value + c.slot;
   init_value(a, c, vid);
   ref_t v = make_ref(T_VALUE, vid);
   *made_short = start < c.len;
   if (!*made_short) return v;
-  uint32_t sid = a.base_short + 2 * c.i;
+  uint32_t sid = a.base_short + 2 * c.slot;
   init_short(a.P.shrt[sid], c.key_off, start, c.len - start, start, 1, v);
   return make_ref(T_SHORT, sid);
 }
 
 enum { ATT_DONE = 0, ATT_RESTART = 1 };
+
+// A key classified as existing reached a point where it would create nodes
+// (or a new key found itself present): the classification and the insert
+// disagree.  Nothing is written; the insert reports an internal error.
+__device__ __forceinline__ int misclassified(const NuArgs &a, const Ctx &c) {
+  raise_error(a.P.cnt, ERR_INTERNAL, 21);
+  return ATT_DONE;
+}
 
 // Dirty marks without atomics.  The keys are sorted and unique, so the lanes
 // whose keys reach a node (every key with the node's nibble prefix) form one
@@ -233,6 +253,7 @@ __device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage
   for (int guard = 0; guard < 2 * kMaxKeyNibbles + 4; guard++) {
     const uint32_t ty = ref_type(node), idx = ref_idx(node);
     if (node == REF_NULL) {
+      if (!c.fresh) return misclassified(a, c);
       if (!ver_upgrade(plock, pver)) return ATT_RESTART;
       bool ms;
       ref_t leaf = make_leaf(a, c, pos, &ms);
@@ -299,25 +320,14 @@ __device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage
       if (p == mreal && p == sreal && term && p == kleft) p++;  // both at the terminator
       if (p == klen) {
         if (ref_type(child) == T_VALUE) {
-          // the key exists: swap the value if this write is later.  A lane
-          // whose write is already superseded finishes without the latch.
+          // the key exists: its value node takes the new value in place,
+          // with no latch (a split of S moves the reference, not the node)
           const uint32_t cv = ref_idx(child);
-          uint64_t cur = 0;
           const uint64_t sv = ld(&S->lock);
-          if (cv < P.cap_value) cur = ld(&P.value[cv].seq);
           load_order();
           if (sv != v) return ATT_RESTART;
-          if (cv < P.cap_value && cur >= c.seq) {
-            mark(P, c, ds, node, sflag, pos);
-            return ATT_DONE;
-          }
-          if (!ver_upgrade(&S->lock, v)) return ATT_RESTART;
-          if (cv >= P.cap_value || ld(&P.value[cv].seq) < c.seq) {
-            const uint32_t vid = a.base_value + c.i;
-            init_value(a, c, vid);
-            publish(&S->child, make_ref(T_VALUE, vid));
-          }
-          ver_unlock(&S->lock);
+          if (c.fresh || cv >= P.cap_value) return misclassified(a, c);
+          rewrite_value(a, c, cv);
           mark(P, c, ds, node, sflag, pos);
           return ATT_DONE;
         }
@@ -341,6 +351,7 @@ __device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage
         continue;
       }
       if (pos + p >= c.len) return ATT_RESTART;
+      if (!c.fresh) return misclassified(a, c);
       // split S at p: a new branch at depth pos + p (the latch upgrades
       // validate everything read above)
       const uint32_t d = pos + p;
@@ -351,10 +362,10 @@ __device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage
         if (p == 0) ver_unlock(plock);
         return ATT_RESTART;
       }
-      const uint32_t bid = a.base_branch + c.i;
+      const uint32_t bid = a.base_branch + c.slot;
       ref_t rest = child;
       if (p + 1 < klen) {
-        const uint32_t rid = a.base_short + 2 * c.i + 1;
+        const uint32_t rid = a.base_short + 2 * c.slot + 1;
         init_short(P.shrt[rid], koff, kstart + p + 1, klen - p - 1, d + 1, term, child);
         rest = make_ref(T_SHORT, rid);
         ds.push(P, rest);
@@ -382,16 +393,13 @@ __device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage
       }
       return ATT_DONE;
     }
-    // a value node in a branch's slot 16: the key exists
+    // a value node in a branch's slot 16: the key exists and takes the new
+    // value in place (the reference to it was read under the branch's
+    // version, validated with the node's own version check above)
     if (parent == REF_NULL || ref_type(parent) != T_BRANCH) return ATT_RESTART;
-    if (!ver_upgrade(plock, pver)) return ATT_RESTART;
     const uint32_t vi = ref_idx(node);
-    if (vi >= P.cap_value || ld(&P.value[vi].seq) < c.seq) {
-      const uint32_t vid = a.base_value + c.i;
-      init_value(a, c, vid);
-      publish(pslot, make_ref(T_VALUE, vid));
-    }
-    ver_unlock(plock);
+    if (c.fresh || vi >= P.cap_value) return misclassified(a, c);
+    rewrite_value(a, c, vi);
     return ATT_DONE;
   }
   return ATT_RESTART;
@@ -401,7 +409,63 @@ __device__ __forceinline__ int attempt(const NuArgs &a, const Ctx &c, DirtyStage
 // write of each distinct key, scal[SC_M_UNIQ] of them).
 constexpr uint32_t kDirtyStage = 4096;  // staged dirty-list entries per block
 
-__global__ __launch_bounds__(256, 5) void k_lock_insert(NuArgs a, uint64_t key_cap, uint32_t max_trips) {
+// Existing keys.  The batch's unique keys are distinct, so a key exists
+// during the insert exactly when it exists before it: a plain descent (no
+// writer runs yet) sets fresh[j] = 1 for each key the insert must create.
+__global__ __launch_bounds__(256) void k_lock_classify(NuArgs a, uint32_t *fresh) {
+  const uint32_t j = grid_tid();
+  if (j >= a.n) return;
+  if (j >= a.scal[SC_M_UNIQ]) {
+    fresh[j] = 0;
+    return;
+  }
+  const Pools &P = a.P;
+  const uint32_t i = a.u_idx[j];
+  const uint64_t kr = a.kref[i];
+  const uint8_t *kp = P.keys + a.kbase + (kr & ((1ull << 48) - 1));
+  const uint32_t len = (uint32_t)(kr >> 48);
+  const uint32_t t = a.tid ? (uint32_t)a.tid[i] : 0u;
+  bool found = false;
+  ref_t node = t < a.ntries ? P.tries[t].root : REF_NULL;
+  uint32_t pos = 0;
+  for (int guard = 0; guard < 2 * kMaxKeyNibbles + 4 && node != REF_NULL; guard++) {
+    const uint32_t ty = ref_type(node), idx = ref_idx(node);
+    if (ty == T_VALUE) {
+      found = pos == len;
+      break;
+    }
+    if (ty == T_BRANCH) {
+      if (pos >= len || idx >= P.cap_branch) break;
+      node = P.branch[idx].child[pos + 1 == len ? kNibbleT : packed_nibble(kp, pos)];
+      pos++;
+      continue;
+    }
+    if (idx >= P.cap_short) break;
+    const Short &S = P.shrt[idx];
+    const uint32_t klen = S.key_len, term = S.term;
+    if (pos + klen > len) break;
+    const uint32_t sreal = klen - term;
+    // term: the node's key ends where the key does (a leaf of this key)
+    if (term && pos + klen != len) break;
+    if (!term && pos + klen == len) break;
+    const uint8_t *sk = P.keys + (S.key_off & ~7ull);
+    uint32_t p = 0;
+    while (p < sreal) {
+      uint64_t x = nib_word(sk, S.key_start + p, S.key_start + sreal) ^ nib_word(kp, pos + p, len - 1);
+      const uint32_t cnt = min(16u, sreal - p);
+      if (cnt < 16) x &= ~0ull << (64u - 4u * cnt);
+      if (x) break;
+      p += cnt;
+    }
+    if (p < sreal) break;
+    pos += klen;
+    node = S.child;
+  }
+  fresh[j] = found ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(256, 5) void k_lock_insert(NuArgs a, const uint32_t *fresh, const uint32_t *rank,
+                                                        uint64_t key_cap, uint32_t max_trips) {
   __shared__ ref_t dbuf[kDirtyStage];
   __shared__ uint32_t dcnt;
   if (threadIdx.x == 0) dcnt = 0;
@@ -412,6 +476,8 @@ __global__ __launch_bounds__(256, 5) void k_lock_insert(NuArgs a, uint64_t key_c
   const uint32_t i = active ? a.u_idx[j] : 0u;
   Ctx c;
   c.i = i;
+  c.slot = active ? rank[j] : 0u;
+  c.fresh = active && fresh[j] != 0;
   c.key_cap = key_cap;
   c.kp = a.P.keys;
   c.len = 1;
@@ -453,21 +519,31 @@ __global__ __launch_bounds__(256, 5) void k_lock_insert(NuArgs a, uint64_t key_c
   }
 }
 
-__global__ void k_bump(Counters *cnt, uint32_t nb, uint32_t ns, uint32_t nv) {
+// The slots the new keys took: per new key one branch, two short nodes and
+// one value node.
+__global__ void k_bump(Counters *cnt, const uint32_t *n_fresh) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    cnt->n_branch += nb;
-    cnt->n_short += ns;
-    cnt->n_value += nv;
+    const uint32_t k = *n_fresh;
+    cnt->n_branch += k;
+    cnt->n_short += 2 * k;
+    cnt->n_value += k;
   }
 }
 
 }  // namespace
 
-hipError_t lock_insert(const NuArgs &a, uint64_t key_cap, hipStream_t s) {
+hipError_t lock_insert(const NuArgs &a, uint64_t key_cap, void *temp, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
-  k_lock_insert<<<nblk(a.n), kB, 0, s>>>(a, key_cap, 1u << 20);
+  // ecount / grp are free once the keys are sorted and deduplicated
+  uint32_t *fresh = a.grp, *rank = a.ecount;
+  uint32_t *n_fresh = a.scal + SC_NFRESH;
+  k_lock_classify<<<nblk(a.n), kB, 0, s>>>(a, fresh);
   GMPT_LAUNCH_CHECK();
-  k_bump<<<1, 64, 0, s>>>(a.P.cnt, a.n, 2 * a.n, a.n);
+  hipError_t e = scan_exclusive_u32(fresh, rank, a.n, n_fresh, temp, s);
+  if (e != hipSuccess) return e;
+  k_lock_insert<<<nblk(a.n), kB, 0, s>>>(a, fresh, rank, key_cap, 1u << 20);
+  GMPT_LAUNCH_CHECK();
+  k_bump<<<1, 64, 0, s>>>(a.P.cnt, n_fresh);
   return hipGetLastError();
 }
 

@@ -119,7 +119,8 @@ enum {
   SC_W = 0, SC_MAXLEN = 1, SC_M_UNIQ = 2, SC_NGROUP = 3, SC_NE = 4, SC_KTOT = 6 /* u64 */, SC_MERGE = 8,
   SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11,
   SC_WOR = 12 /* u64: OR of the windows */, SC_WAND = 14 /* u64: AND of the windows */,
-  SC_UNIFORM = 16 /* 1: every key has key 0's length (packed by the validation pass) */, SC_COUNT = 20
+  SC_UNIFORM = 16 /* 1: every key has key 0's length (packed by the validation pass) */,
+  SC_NFRESH = 17 /* LockNU: keys the insert creates */, SC_COUNT = 20
 };
 
 // Zeroes the call's device scalars (SC_W starts at all ones for its minimum).

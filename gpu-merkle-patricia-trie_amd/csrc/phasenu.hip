@@ -200,7 +200,9 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
     a.klen[i] = len;
     const uint32_t slot = slot_bytes(len);
     a.kpoff[i] = slot;
-    odd = len != L0;
+    // a rejected key turns the equal-slot layout off: the general packing
+    // writes its kref (the equal-slot pass only writes those of keys it packs)
+    odd = !ok || len != L0;
     if (ok && !odd && a.kbase + (uint64_t)(i + 1) * slot <= a.key_cap) {
       const uint64_t koff = (uint64_t)i * slot;
       pack_key((uint64_t *)(a.P.keys + a.kbase + koff), stg.at(a, a.kidx[2 * i]), len);
@@ -282,6 +284,21 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
   const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
   if (i >= a.n) return;
+  // The packed keys must fit the arena.  The host reserves it from the key
+  // buffer's size, which index pairs that overlap or repeat a key range can
+  // exceed: then no key is packed, every key reads as the empty key (its
+  // terminator only, so every later read stays at the batch's first arena
+  // bytes) and the call fails with an arena error; the host grows the arena
+  // to the total and runs the insert again.
+  if (a.kbase + *(const uint64_t *)(a.scal + SC_KTOT) > a.key_cap) {
+    a.kref[i] = 1ull << 48;
+    if (a.sk0) {
+      a.sk0[i] = 0;
+      a.sv0[i] = i;
+    }
+    if (i == 0) raise_error(a.P.cnt, ERR_POOL, kErrArgKeyArena);
+    return;
+  }
   uint32_t L = a.klen[i];
   const uint8_t *k = key_range_ok(a, i) ? stg.at(a, a.kidx[2 * i]) : a.kh;
   const uint64_t koff = a.kpoff[i];

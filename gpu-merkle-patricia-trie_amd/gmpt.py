@@ -213,6 +213,7 @@ class DeviceBatch:
         s.adopt_values = 1 if adopt_values else 0
         self.struct = s
         self.n = b.n
+        self.holders = 0  # tries (MPT objects) that point into the value buffer
 
     def view(self, start, count):
         """Keys [start, start + count) as a single-trie device batch sharing
@@ -223,10 +224,16 @@ class DeviceBatch:
         s.values_indexs = self.struct.values_indexs + 16 * start
         s.trie_ids = None
         s.n = count
-        v.bufs, v.tid, v.struct, v.n = [], None, s, count
+        v.bufs, v.tid, v.struct, v.n, v.holders = [], None, s, count, 0
+        v._parent = self  # keeps the shared buffers alive
         return v
 
     def free(self):
+        """Releases the device buffers; refused while a trie still points into
+        the value buffer (adopt_values) -- clear() or close() that trie first."""
+        if self.holders:
+            raise GmptError("DeviceBatch.free: a trie still refers to this batch's values "
+                            "(clear() or close() it first)")
         for x in self.bufs:
             x.free()
         if self.tid:
@@ -240,11 +247,16 @@ class MPT:
         self._f = ctypes.c_void_p()
         _check(lib().gmpt_forest_create(device, num_tries, ctypes.byref(self._f)))
         self.num_tries = num_tries
+        # device batches whose value buffers the tries point into
+        # (adopt_values): kept alive until the tries stop referring to them
+        self._adopted = []
 
     def close(self):
         if self._f and self._f.value:
             lib().gmpt_forest_destroy(self._f)
             self._f = None
+        if getattr(self, "_adopted", None):
+            self._release_adopted()
 
     def __del__(self):
         try:
@@ -253,11 +265,18 @@ class MPT:
             pass
 
     # --- inserts (gpu_mpt.cuh:69-71, 95-97) -----------------------------------
+    def _adopt(self, batch):
+        if isinstance(batch, DeviceBatch) and batch.struct.adopt_values and batch.n:
+            owner = getattr(batch, "_parent", batch)  # a view's buffers belong to its parent
+            owner.holders += 1
+            self._adopted.append(owner)
+
     def _puts(self, fn, batch, hps=None):
         s = batch.struct if isinstance(batch, DeviceBatch) else host_batch(batch, hps)
         t = ctypes.c_void_p()
         n = ctypes.c_int()
         _check(fn(self._f, ctypes.byref(s), ctypes.byref(t), ctypes.byref(n)))
+        self._adopt(batch)
         return t.value, n.value
 
     def puts_2phase_with_valuehp(self, batch, values_hps=None):
@@ -283,6 +302,7 @@ class MPT:
         nt = ctypes.c_int()
         _check(lib().gmpt_puts_with_read(self._f, ctypes.byref(s), fptr, mode, ctypes.byref(rn), _p(hps), _p(sizes),
                                          ctypes.byref(t), ctypes.byref(nt)))
+        self._adopt(batch)
         reads = [(int(hps[i]), int(sizes[i])) for i in range(rn.value)]
         return t, nt.value, reads
 
@@ -355,8 +375,14 @@ class MPT:
     def reset_trie(self, trie):
         _check(lib().gmpt_reset_trie(self._f, trie))
 
+    def _release_adopted(self):
+        for b in self._adopted:
+            b.holders -= 1
+        self._adopted = []
+
     def clear(self):
         _check(lib().gmpt_forest_clear(self._f))
+        self._release_adopted()
 
     def synchronize(self):
         _check(lib().gmpt_synchronize(self._f))

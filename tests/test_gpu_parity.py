@@ -576,3 +576,160 @@ def test_equal_length_key_packing_edges(gmpt, workloads, oracle, mode):
     b = workloads.batch_from_hex(hk, vals(len(hk)))
     t = _build(gmpt, b, mode)
     assert t.get_root_hash() == oracle.root_of(b)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_short_root_and_embedded_children(gmpt, workloads, oracle, mode):
+    """The reference's encoding quirks on the GPU: a root whose encoding is
+    under 32 bytes is returned raw (cpu_mpt.cuh:413-418, 915-926), and child
+    encodings under 32 bytes are string-wrapped in their parent
+    (node.cuh:75, 248, 254); the exported nodeset matches the oracle's."""
+    b = workloads.batch_from_bytes([b"\x10", b"\x20"], [b"a", b"bb"])
+    t = _build(gmpt, b, mode)
+    root = t.get_root_hash()
+    assert len(root) < 32 and root == oracle.root_of(b)
+    o = oracle.OracleMPT()
+    o.puts(b)
+    o.hash()
+    want = [(h + bytes(32 - len(h)), e) for h, e in o.export()]
+    assert sorted(t.flush_dirty_nodes(b)) == sorted(want)
+    assert t.get_values(b) == [b"a", b"bb"]
+    # embedded children under a hashed root, then a second batch on top
+    keys = [bytes([16 * k]) for k in range(1, 9)]
+    b2 = workloads.batch_from_bytes(keys, [bytes([k]) * (k % 3) for k in range(8)])
+    _put(t, b2, mode)
+    o.puts(b2)
+    o.hash()
+    assert len(t.get_root_hash()) == 32 and t.get_root_hash() == o.root()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_skewed_1000_byte_values_on_preloaded_trie(gmpt, workloads, oracle, mode):
+    """The skewed-update shape with its full 1,000-byte values (leaves of
+    eight Keccak blocks) at reduced size: a preloaded trie of 'user' +
+    19-digit keys, then Zipf(0.99) overwrites (rewritten in place) and
+    inserts, checked against the oracle after each batch."""
+    pre, ops = workloads.config3_lock(seed=9, n_preload=60_000, n_ops=40_000, universe=200_000, value_len=1000)
+    t = gmpt.MPT(device=0)
+    _put(t, pre, "2phase")
+    o = oracle.OracleMPT()
+    o.puts(pre)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    _put(t, ops, mode)
+    o.puts(ops)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    sample = ops.subset(range(0, ops.n, 53))
+    assert t.get_values(sample) == o.gets(sample)
+
+
+def _batched_shape(workloads, scale):
+    """The batched workload's 4,096 lognormal trie sizes (median 256, sigma
+    2.046, clipped to 1..200,000) divided by `scale` (at least one key each;
+    the largest trie keeps its full size), 32-byte keys and values, the keys of
+    all tries shuffled together."""
+    sizes, rng = workloads.config4_sizes()
+    big = int(np.argmax(sizes))
+    sz = np.maximum(1, np.round(sizes / scale)).astype(np.int64)
+    sz[big] = sizes[big]
+    total = int(sz.sum())
+    keys = workloads.unique_random_keys(rng, total, 32)
+    vals = rng.integers(0, 256, size=(total, 32), dtype=np.uint8)
+    tid = np.repeat(np.arange(len(sz), dtype=np.int32), sz)
+    starts = np.concatenate([[0], np.cumsum(sz)[:-1]])
+    perm = rng.permutation(total)
+    allb = workloads.batch_from_fixed(keys[perm], vals[perm].reshape(-1), np.full(total, 32))
+    allb.trie_ids = tid[perm]
+    per = [workloads.batch_from_fixed(keys[s:s + c], vals[s:s + c].reshape(-1), np.full(int(c), 32))
+           for s, c in zip(starts, sz)]
+    return sz, big, allb, per
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batched_4096_uneven_tries(gmpt, workloads, oracle, mode):
+    """4,096 tries of uneven size (1-key tries up to one of 200,000 keys) in
+    one multi-trie insert and one hash pass: every trie's root equals its
+    plain single-trie build, and a sample including the largest trie equals
+    the oracle's."""
+    sz, big, allb, per = _batched_shape(workloads, 16)
+    assert (sz == 1).sum() > 100 and sz[big] == 200_000
+    t = gmpt.MPT(device=0, num_tries=len(sz))
+    _put(t, allb, mode)
+    roots = t.get_root_hashes()
+    p = gmpt.MPT(device=0)
+    for i, b in enumerate(per):
+        p.clear()
+        _put(p, b, "2phase")
+        assert roots[i] == p.get_root_hash(), i
+    rng = np.random.default_rng(4)
+    for i in sorted(set(rng.choice(len(sz), 12, replace=False).tolist()) | {big, int(np.argmin(sz))}):
+        assert roots[i] == oracle.root_of(per[i]), i
+    t.close()
+    p.close()
+
+
+def test_latching_overwrites_allocate_nothing(gmpt, workloads, oracle):
+    """LockNU update-only batches rewrite the keys' value nodes in place and
+    keep no packed keys: across five rounds of updates the pools grow by the
+    staged value bytes only, and every root and value matches the oracle."""
+    rng = np.random.default_rng(63)
+    base = _rand_batch(workloads, 63, 4000, 30, 90)
+    t = gmpt.MPT(device=0)
+    o = oracle.OracleMPT()
+    _put(t, base, "latching")
+    o.puts(base)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    used0 = t.stats()["pool_bytes_used"]
+    staged = 0
+    for r in range(5):
+        flat, lens = workloads.random_values(rng, base.n, 20, 140)
+        upd = workloads.batch_from_hex([base.key(i) for i in range(base.n)],
+                                       [flat[int(lens[:i].sum()):int(lens[:i + 1].sum())].tobytes()
+                                        for i in range(base.n)])
+        upd = upd.subset(rng.permutation(upd.n))
+        _put(t, upd, "latching")
+        o.puts(upd)
+        o.hash()
+        assert t.get_root_hash() == o.root(), r
+        staged += int(lens.sum()) + 8 + 256
+    grown = t.stats()["pool_bytes_used"] - used0
+    assert grown <= staged, (grown, staged)
+    sample = base.subset(range(0, base.n, 13))
+    assert t.get_values(sample) == o.gets(sample)
+    # a batch of new keys and overwrites together still takes slots for the new ones only
+    fresh = _rand_batch(workloads, 64, 3000, 30, 90)
+    mix = workloads.concat_batches([fresh, base.subset(range(0, 2000))])
+    _put(t, mix, "latching")
+    o.puts(mix)
+    o.hash()
+    assert t.get_root_hash() == o.root()
+    assert t.get_values(fresh) == o.gets(fresh)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_index_pairs_sharing_one_key(gmpt, workloads, oracle, mode, where):
+    """10,000 index pairs that all point at one 65-nibble key (the packed keys
+    need far more arena than the key buffer's size suggests): the arena is
+    sized from the pairs (host batches) or grown after the packing pass
+    reports the overflow (device batches), and the root is the oracle's."""
+    key = bytes(range(32))
+    hk = workloads.key_bytes_to_hex(key)
+    n = 10_000
+    rng = np.random.default_rng(5)
+    flat, lens = workloads.random_values(rng, n, 30, 60)
+    b = workloads.Batch(np.frombuffer(hk, np.uint8).copy(), np.tile(np.array([0, len(hk) - 1], np.int32), n),
+                        flat.astype(np.uint8), workloads._index_pairs(lens, np.int64), n)
+    t = gmpt.MPT(device=0)
+    if where == "host":
+        _put(t, b, mode)
+    else:
+        db = gmpt.DeviceBatch(b)
+        fn = t.puts_2phase_with_valuehp if mode == "2phase" else t.puts_latching_with_valuehp_v2
+        targets, nt = fn(db)
+        t.hash_onepass_v2(targets, nt)
+        db.free()
+    assert t.get_root_hash() == oracle.root_of(b)
+    assert t.get_values(b.subset([0])) == [b.value(n - 1)]

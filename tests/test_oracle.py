@@ -295,3 +295,138 @@ def test_census_config1(oracle, workloads):
     assert c["n_short_leaf"] == b.n
     assert c["n_hashed"] == c["n_full"] + c["n_short_leaf"] + c["n_short_ext"]
     assert c["n_embedded"] == 0
+
+
+# ------------------------------------------------------------ node export ----
+
+def _rlp_decode(b, i=0):
+    """One RLP item at b[i:] -> (bytes or list, next index); strict lengths."""
+    p = b[i]
+    if p < 0x80:
+        return bytes([p]), i + 1
+    if p < 0xC0:
+        if p <= 0xB7:
+            n, s = p - 0x80, i + 1
+        else:
+            ll = p - 0xB7
+            n, s = int.from_bytes(b[i + 1:i + 1 + ll], "big"), i + 1 + ll
+        assert s + n <= len(b)
+        return bytes(b[s:s + n]), s + n
+    if p <= 0xF7:
+        n, s = p - 0xC0, i + 1
+    else:
+        ll = p - 0xF7
+        n, s = int.from_bytes(b[i + 1:i + 1 + ll], "big"), i + 1 + ll
+    end, items = s + n, []
+    assert end <= len(b)
+    while s < end:
+        it, s = _rlp_decode(b, s)
+        items.append(it)
+    assert s == end
+    return items, end
+
+
+def _rlp_encode(x):
+    if isinstance(x, list):
+        return _rlp_list(b"".join(_rlp_encode(y) for y in x))
+    return _rlp_str(x)
+
+
+def _hp_decode(c):
+    """Hex-prefix (compact) bytes -> (nibbles, leaf flag)."""
+    flag = c[0] >> 4
+    assert flag < 4
+    nib = [c[0] & 15] if flag & 1 else []
+    if not flag & 1:
+        assert c[0] & 15 == 0
+    for x in c[1:]:
+        nib += [x >> 4, x & 15]
+    return nib, bool(flag & 2)
+
+
+def _nodeset_contents(nodes, root, oracle):
+    """Walks an exported nodeset from the root: every (h, e) must satisfy
+    h == keccak256(e) (e >= 32 B) or h == e, every e must be an RLP list of 2
+    or 17 items that re-encodes to itself, and the walk must visit each
+    exported entry exactly once (identical subtrees, such as equal embedded
+    leaves under different branches, are exported once per occurrence).
+    Returns {key bytes: value bytes} read off the leaves."""
+    by_hash, listed = {}, {}
+    for h, e in nodes:
+        assert h == (oracle.keccak256(e) if len(e) >= 32 else e)
+        item, end = _rlp_decode(e)
+        assert end == len(e) and isinstance(item, list) and len(item) in (2, 17)
+        assert _rlp_encode(item) == e
+        by_hash[h] = item
+        listed[h] = listed.get(h, 0) + 1
+    assert root in by_hash
+    raw, visits = {}, {}
+
+    def visit(ref, path):
+        assert ref in by_hash
+        visits[ref] = visits.get(ref, 0) + 1
+        item = by_hash[ref]
+        if len(item) == 17:
+            for s in range(16):
+                if item[s] != b"":
+                    visit(item[s], path + (s,))
+            if item[16] != b"":
+                raw[path] = item[16]
+            return
+        nib, leaf = _hp_decode(item[0])
+        if leaf:
+            raw[path + tuple(nib)] = item[1]
+        else:
+            assert nib
+            visit(item[1], path + tuple(nib))
+
+    visit(root, ())
+    assert visits == listed  # every exported node is in the trie, once per occurrence
+    out = {}
+    for p, v in raw.items():
+        assert len(p) % 2 == 0 and p not in out
+        out[bytes(p[k] << 4 | p[k + 1] for k in range(0, len(p), 2))] = v
+    return out
+
+
+def _export_cases(workloads):
+    rng = np.random.default_rng(21)
+    keys = workloads.unique_random_keys(rng, 2500, 32)
+    flat, lens = workloads.random_values(rng, 2500, 1, 140)
+    yield "random", workloads.batch_from_fixed(keys, flat, lens)
+    r = random.Random(22)
+    ks = set()
+    while len(ks) < 600:  # prefixes of one another: values in branch slot 16
+        ks.add(bytes(r.choice([0, 1, 0x10, 0xfe]) for _ in range(r.randint(1, 5))))
+    ks = sorted(ks)
+    r.shuffle(ks)
+    yield "prefix-keys", workloads.batch_from_bytes(ks, [bytes([r.getrandbits(8)]) * r.randint(1, 50) for _ in ks])
+    ks = list({bytes(r.getrandbits(8) for _ in range(r.randint(1, 2))) for _ in range(300)})
+    yield "embedded", workloads.batch_from_bytes(ks, [bytes(r.getrandbits(8) for _ in range(r.randint(0, 3)))
+                                                      for _ in ks])
+    yield "short-root", workloads.batch_from_bytes([b"\x10", b"\x20"], [b"a", b"bb"])
+
+
+def test_export_nodeset_is_the_trie(oracle, workloads):
+    """orc_export (the oracle's flush_dirty_nodes / get_all_nodes nodeset):
+    decoding the exported nodes from the root gives back exactly the batch's
+    last-written key -> value map, each hash slot is Keccak-256 of its
+    encoding (or the encoding itself under 32 bytes), every encoding is a
+    well-formed RLP list of 2 or 17 items, and the node count equals the
+    census' branch + short count."""
+    for name, b in _export_cases(workloads):
+        t = oracle.OracleMPT()
+        t.puts(b)
+        t.hash()
+        c = t.census()
+        nodes = t.export()
+        assert len(nodes) == c["n_full"] + c["n_short_leaf"] + c["n_short_ext"], name
+        want = {}
+        for i in range(b.n):
+            hk = b.key(i)
+            assert hk[-1] == 16 and len(hk) % 2 == 1
+            want[bytes(hk[k] << 4 | hk[k + 1] for k in range(0, len(hk) - 1, 2))] = b.value(i)
+        # an empty value in a branch's slot 16 (a key that is a prefix of
+        # another) encodes as 0x80, the same as an empty slot
+        want = {k: v for k, v in want.items() if v or not any(o != k and o.startswith(k) for o in want)}
+        assert _nodeset_contents(nodes, t.root(), oracle) == want, name

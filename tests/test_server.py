@@ -100,3 +100,24 @@ def test_malformed_request_refused_others_unaffected(gmpt, workloads):
     got = [srv.wait(i)[0] for i in ids]
     srv.close()
     assert got == _alone(gmpt, reqs, 0)
+
+
+def test_latching_server_updates_stay_within_budget(gmpt, workloads, oracle):
+    """A LockNU server lane rewriting the same keys over and over: each
+    update keeps only its staged value bytes, so thirty rounds of updates to
+    a 3,000-key trie stay within an 8 MB budget (a node and a packed key per
+    update would need about 30 MB), and the final root is the oracle's."""
+    rng = np.random.default_rng(41)
+    keys = workloads.unique_random_keys(rng, 3000, 32)
+    srv = gmpt.Server(device=0, num_tries=1, lanes=1, mem_budget=8 << 20, mode=1)
+    o = oracle.OracleMPT()
+    for r in range(31):
+        flat, lens = workloads.random_values(rng, len(keys), 20, 40)
+        b = workloads.batch_from_fixed(keys, flat, lens)
+        root, _ = srv.wait(srv.submit(0, b))
+        o.puts(b)
+    o.hash()
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 31 and st["failed"] == 0
+    assert root == o.root()
