@@ -6,4 +6,4 @@ include/gmpt_amd.h and include/libgmpt.h); `gmpt` is its ctypes binding and
 """
 from . import workloads  # noqa: F401
 from . import gmpt  # noqa: F401
-from .gmpt import MPT, DeviceBatch, GmptError, Server, keccak256_batch  # noqa: F401
+from .gmpt import MPT, DeviceBatch, GmptError, Server, build_root_noalloc, keccak256_batch  # noqa: F401

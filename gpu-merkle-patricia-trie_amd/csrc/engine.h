@@ -36,6 +36,29 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// A fixed device region a Forest can take all its device memory from, with
+// no allocation call: buffers are carved in order and never returned (the
+// forest lives for one call).  take() gives nullptr once the region is spent.
+struct Arena {
+  uint8_t *base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  void *take(size_t bytes) {
+    size_t off = (used + 255) & ~(size_t)255;
+    if (off > cap || bytes > cap - off) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
+
+// Host memory a Forest over an Arena copies its counters and hashing bins
+// into (pageable; the caller owns it, so the forest allocates none).
+struct HostMirror {
+  Counters cnt;
+  uint32_t scal[16];
+  HcMeta meta;
+};
+
 struct OpStats {
   double last_hash_ms = 0;    // device time of the hash kernels of the last hash()
   uint32_t hash_launches = 0;
@@ -56,6 +79,10 @@ struct OpStats {
 class Forest {
  public:
   Forest(int device, uint32_t ntries);
+  // A forest that allocates nothing: its device memory comes from `arena`,
+  // its host mirrors from `host`, and it runs on `stream` (not owned).  It
+  // keeps no timing events, so its stats carry no kernel times.
+  Forest(int device, uint32_t ntries, Arena *arena, HostMirror *host, hipStream_t stream);
   ~Forest();
   bool ok() const { return init_ok_; }
 
@@ -108,6 +135,7 @@ class Forest {
   int ensure_pools(uint64_t nb, uint64_t ns, uint64_t nv, uint64_t key_bytes);
   int grow(DevBuf &b, size_t need, size_t keep);
   int ensure_ws(DevBuf &b, size_t bytes);
+  int arena_exhausted();
   int sync_counters();
   bool grow_key_arena_after_error();
   int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
@@ -119,6 +147,7 @@ class Forest {
   int device_ = 0;
   uint32_t ntries_ = 0;
   bool init_ok_ = false;
+  Arena *arena_ = nullptr;     // set: every device buffer is carved from it
   hipStream_t stream_ = nullptr;
   DevBuf branch_, shrt_, value_, keys_, tries_, dirty_;
   Counters *d_cnt_ = nullptr;

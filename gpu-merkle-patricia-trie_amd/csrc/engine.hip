@@ -82,9 +82,30 @@ Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
   init_ok_ = true;
 }
 
+Forest::Forest(int device, uint32_t ntries, Arena *arena, HostMirror *host, hipStream_t stream)
+    : device_(device), ntries_(ntries), arena_(arena), stream_(stream) {
+  if (hipSetDevice(device) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    return;
+  }
+  d_cnt_ = (Counters *)arena->take(sizeof(Counters));
+  if (!d_cnt_) {
+    set_error("device scratch region too small for the counters");
+    return;
+  }
+  h_cnt_ = &host->cnt;
+  h_scal_ = host->scal;
+  h_meta_ = &host->meta;
+  if (hipMemsetAsync(d_cnt_, 0, sizeof(Counters), stream_) != hipSuccess) return;
+  if (grow(tries_, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1), 0) != 0) return;
+  hc_spread_max_ = kHcSpreadMaxDefault;
+  init_ok_ = true;
+}
+
 Forest::~Forest() {
   (void)hipSetDevice(device_);
-  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (stream_ || arena_) (void)hipStreamSynchronize(stream_);
+  if (arena_) return;  // nothing of its own to release
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
                     &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_,
                     &hc_lmeta_, &hc_lmeta_tmp_, &rw_};
@@ -141,27 +162,45 @@ uint64_t Forest::pool_bytes_reserved() const {
 // Regrow a device buffer to at least `need` bytes, keeping the first `keep`.
 int Forest::grow(DevBuf &b, size_t need, size_t keep) {
   if (b.cap >= need) return 0;
-  size_t cap = std::max(need, b.cap + b.cap / 2);
-  cap = align_up(cap, 4096);
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, cap);
-  if (e != hipSuccess) {
-    set_error(std::string("device pool allocation failed: ") + hipGetErrorString(e));
-    return -2;
+  size_t cap;
+  if (arena_) {
+    // exactly what is asked: the region holds one call's forest
+    cap = align_up(need, 256);
+    if (!(p = arena_->take(cap))) return arena_exhausted();
+  } else {
+    cap = align_up(std::max(need, b.cap + b.cap / 2), 4096);
+    hipError_t e = hipMalloc(&p, cap);
+    if (e != hipSuccess) {
+      set_error(std::string("device pool allocation failed: ") + hipGetErrorString(e));
+      return -2;
+    }
   }
   if (b.p && keep) HIP_TRY(hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, stream_));
   // fresh slots read as zero (empty children, unlocked, clean) even before a
   // writer publishes them, which keeps optimistic LockNU readers in bounds
   HIP_TRY(hipMemsetAsync((uint8_t *)p + keep, 0, cap - keep, stream_));
   HIP_TRY(hipStreamSynchronize(stream_));
-  if (b.p) HIP_TRY(hipFree(b.p));
+  if (b.p && !arena_) HIP_TRY(hipFree(b.p));
   b.p = p;
   b.cap = cap;
   return 0;
 }
 
+int Forest::arena_exhausted() {
+  set_error("the batch needs more device scratch than the fixed region holds (" + std::to_string(arena_->cap) +
+            " bytes)");
+  return -2;
+}
+
 int Forest::ensure_ws(DevBuf &b, size_t bytes) {
   if (b.cap >= bytes) return 0;
+  if (arena_) {
+    size_t cap = align_up(bytes, 256);
+    if (!(b.p = arena_->take(cap))) return arena_exhausted();
+    b.cap = cap;
+    return 0;
+  }
   if (b.p) {
     HIP_TRY(hipStreamSynchronize(stream_));
     HIP_TRY(hipFree(b.p));
@@ -216,6 +255,11 @@ int Forest::sync_counters() {
 
 uint8_t *Forest::value_space(uint64_t bytes) {
   bytes = align_up(bytes + 8, 256);
+  if (arena_) {
+    void *p = arena_->take(bytes);
+    if (!p) arena_exhausted();
+    return (uint8_t *)p;
+  }
   if (vchunks_.empty() || vchunk_used_ + bytes > vchunks_.back().cap) {
     DevBuf c;
     c.cap = align_up(std::max<uint64_t>(bytes, 64ull << 20), 1 << 20);
@@ -549,16 +593,19 @@ int Forest::hash(void *targets, int n_targets) {
     HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
                         encl, ws2_.p, stream_, lmeta_tmp, lmeta, rec_));
     HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipEventRecord(ev_meta_, stream_));
-    HIP_TRY(hipEventRecord(ev0_, stream_));
+    if (ev_meta_) HIP_TRY(hipEventRecord(ev_meta_, stream_));
+    if (ev0_) HIP_TRY(hipEventRecord(ev0_, stream_));
     if (lmeta) {
       // the leaves of level 0 (its bins): launched before the bins reach the
       // host, so the GPU hashes them while the host plans the other levels
-      HIP_TRY(hipEventRecord(ev2_, stream_));
+      if (ev2_) HIP_TRY(hipEventRecord(ev2_, stream_));
       HIP_TRY(hc_hash_leaves_direct(P, sorted, lmeta, (const HcMeta *)hc_meta_.p, n, encl, stream_));
-      HIP_TRY(hipEventRecord(ev3_, stream_));
+      if (ev3_) HIP_TRY(hipEventRecord(ev3_, stream_));
     }
-    HIP_TRY(hipEventSynchronize(ev_meta_));
+    if (ev_meta_)
+      HIP_TRY(hipEventSynchronize(ev_meta_));
+    else
+      HIP_TRY(hipStreamSynchronize(stream_));
     const HcMeta &M = *(const HcMeta *)h_meta_;
     if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
     uint64_t *scratch = (uint64_t *)hc_scratch_.p;
@@ -572,7 +619,7 @@ int Forest::hash(void *targets, int n_targets) {
       if (cnt) {
         const bool leaves = lv <= 1;  // level 1: leaves the direct kernel does not take
         const uint32_t skip = lv == 0 ? n_direct : 0u;  // already queued
-        const bool timed = lv == 0 && !lmeta;
+        const bool timed = lv == 0 && !lmeta && ev2_;
         const uint32_t n_ext = leaves ? 0u : M.hist[lv * kHcClasses + kHcClasses - 1];
         if (cnt > skip)
           HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
@@ -585,7 +632,7 @@ int Forest::hash(void *targets, int n_targets) {
       }
       start += cnt;
     }
-    HIP_TRY(hipEventRecord(ev1_, stream_));
+    if (ev1_) HIP_TRY(hipEventRecord(ev1_, stream_));
     parts = (unsigned long long *)((uint8_t *)hc_meta_.p + sizeof(HcMeta));
     HIP_TRY(hc_count(encl, (const HcMeta *)hc_meta_.p, n, n_leaf, parts, stream_));
     stats_.leaf_nodes = n_leaf;
@@ -599,7 +646,7 @@ int Forest::hash(void *targets, int n_targets) {
   stats_.leaf_perms = n ? cnt_.leaf_perms : 0;
   stats_.leaf_bytes = n ? cnt_.leaf_bytes : 0;
   stats_.leaf_perm_ms = 0;
-  if (n > 0) {
+  if (n > 0 && ev0_) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.last_hash_ms = ms;
@@ -666,7 +713,7 @@ int Forest::clear() {
   memset(&cnt_, 0, sizeof(cnt_));
   key_used_ = 0;
   // keep the first value chunk, release the others
-  for (size_t i = 1; i < vchunks_.size(); i++) HIP_TRY(hipFree(vchunks_[i].p));
+  for (size_t i = 1; i < vchunks_.size() && !arena_; i++) HIP_TRY(hipFree(vchunks_[i].p));
   if (vchunks_.size() > 1) vchunks_.resize(1);
   vchunk_used_ = 0;
   return 0;

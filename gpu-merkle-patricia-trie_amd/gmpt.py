@@ -48,7 +48,7 @@ GMPT_SYMBOLS = ["gmpt_forest_create", "gmpt_forest_destroy", "gmpt_last_error", 
                 "gmpt_set_option", "gmpt_get_option", "gmpt_autotune", "gmpt_get_stats",
                 "gmpt_device_count", "gmpt_device_malloc", "gmpt_device_free", "gmpt_memcpy_htod",
                 "gmpt_memcpy_dtoh", "gmpt_device_copy", "gmpt_timer_start", "gmpt_timer_stop", "gmpt_keccak256_batch",
-                "gmpt_keccak256_batch_mode", "gmpt_server_create", "gmpt_server_destroy", "gmpt_server_submit",
+                "gmpt_keccak256_batch_mode", "gmpt_build_root_noalloc", "gmpt_server_create", "gmpt_server_destroy", "gmpt_server_submit",
                 "gmpt_server_poll", "gmpt_server_wait", "gmpt_server_drain", "gmpt_server_get_stats",
                 "gmpt_server_latencies"]
 LIBGMPT_SYMBOLS = ["build_mpt_2phase", "build_mpt_olc", "preprocess", "get_all_nodes", "init_mpt",
@@ -112,6 +112,7 @@ def lib():
         "gmpt_timer_stop": ([vp, ctypes.POINTER(ctypes.c_float)], i32),
         "gmpt_keccak256_batch": ([vp, vp, vp, u32, vp], i32),
         "gmpt_keccak256_batch_mode": ([vp, vp, vp, u32, vp, i32], i32),
+        "gmpt_build_root_noalloc": ([ctypes.POINTER(_Batch), i32, vp, ctypes.POINTER(i32)], i32),
         "gmpt_server_create": ([i32, u32, u32, ctypes.c_uint64, u32, i32, ctypes.POINTER(vp)], i32),
         "gmpt_server_destroy": ([vp], None),
         "gmpt_server_submit": ([vp, u32, ctypes.POINTER(_Batch), ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -238,6 +239,20 @@ class DeviceBatch:
             x.free()
         if self.tid:
             self.tid.free()
+
+
+def build_root_noalloc(batch, mode=0):
+    """Root hash of the trie `batch` builds from empty, computed by the
+    library's no-allocation entry point (a fixed device region per GPU, the
+    current device, at most NOALLOC_MAX_KEYS keys).  mode 0 PhaseNU, 1 LockNU."""
+    s = batch.struct if isinstance(batch, DeviceBatch) else host_batch(batch)
+    out = np.zeros(32, np.uint8)
+    sz = ctypes.c_int()
+    _check(lib().gmpt_build_root_noalloc(ctypes.byref(s), mode, _p(out), ctypes.byref(sz)))
+    return out[:sz.value].tobytes()
+
+
+NOALLOC_MAX_KEYS = 32768
 
 
 class MPT:

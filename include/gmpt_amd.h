@@ -164,6 +164,28 @@ int gmpt_keccak256_batch(const uint8_t *data, const uint64_t *offsets, const uin
 int gmpt_keccak256_batch_mode(const uint8_t *data, const uint64_t *offsets, const uint32_t *lengths, uint32_t n,
                               uint8_t *out, int mode);
 
+/* Root hash of the trie that batch b builds from empty (mode 0: PhaseNU,
+ * 1: LockNU), computed without any allocation call: no hipMalloc, hipFree,
+ * pinned-memory or stream/event creation.  No counterpart in the reference;
+ * it is for callers that cannot allocate (code that must not call hipMalloc:
+ * tight serving loops, harnesses that forbid it), e.g. the per-block
+ * transaction and receipt tries that build_mpt_2phase / build_mpt_olc build
+ * on a fresh trie.  Every device buffer of the call comes from one fixed
+ * device array of GMPT_NOALLOC_SCRATCH_MIB MiB per GPU, held until the process
+ * ends; batches above GMPT_NOALLOC_MAX_KEYS keys, or needing more scratch
+ * than the array holds, fail with an error (-3 / -2) and gmpt_last_error().
+ * Host batches are staged into the array; device batches' values are read
+ * in place.  Runs on the calling thread's current device and its null stream
+ * and returns when the root is on the host: *size = 32, the raw root
+ * encoding's length when it is under 32 bytes, or 0 for an empty batch.
+ * Calls on one GPU share its array: calls from threads of one process take a
+ * per-device lock and run one at a time; callers in other processes have
+ * their own array.  The array keeps the last call's contents; each call
+ * clears what it needs zeroed. */
+#define GMPT_NOALLOC_MAX_KEYS 32768
+#define GMPT_NOALLOC_SCRATCH_MIB 64
+int gmpt_build_root_noalloc(const gmpt_batch *b, int mode, uint8_t out[32], int *size);
+
 /* ---- Serving: continuous batching over HBM-resident tries ----------------
  * No reference counterpart (the reference builds one trie per call).  A
  * server holds num_tries resident tries split over `lanes` forests (one HIP
