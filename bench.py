@@ -119,7 +119,11 @@ def bench_phase(pkg, args):
     t = pkg.MPT(device=0)
     t.set_option("hc_fuse_leaves", 0 if args.no_fuse_leaves else 1)
     fused = not args.no_fuse_leaves
-    spread_max, tune_ms = t.autotune(db)  # picks the Keccak layout threshold on this batch
+    if args.spread_max is None:
+        spread_max, tune_ms = t.autotune(db)  # picks the Keccak layout threshold on this batch
+    else:  # fixed threshold: no autotuner passes (clean kernel traces)
+        t.set_option("hc_spread_max", args.spread_max)
+        spread_max, tune_ms = args.spread_max, None
     roots = []
 
     def step():
@@ -605,6 +609,8 @@ def main():
                     help="phase: seconds of the served sub-object's offered-load run")
     ap.add_argument("--served-plain-duration", type=float, default=5.0,
                     help="phase: seconds of the served sub-object's plain-call run")
+    ap.add_argument("--spread-max", type=int, default=None,
+                    help="phase: fixed hc_spread_max instead of the autotuner's choice")
     ap.add_argument("--no-fuse-leaves", action="store_true",
                     help="phase: leaf level as encode kernel + HBM scratch + permutation kernel")
     args = ap.parse_args()

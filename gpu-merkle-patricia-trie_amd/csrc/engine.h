@@ -16,6 +16,8 @@
 
 namespace gmpt {
 
+class HostStager;
+
 struct BatchDesc {
   const uint8_t *keys_hexs = nullptr;
   const int32_t *keys_indexs = nullptr;    // inclusive [start, end] pairs
@@ -148,6 +150,7 @@ class Forest {
   uint32_t ntries_ = 0;
   bool init_ok_ = false;
   Arena *arena_ = nullptr;     // set: every device buffer is carved from it
+  bool all_empty_ = true;      // no insert since creation or clear(): every trie is empty
   hipStream_t stream_ = nullptr;
   DevBuf branch_, shrt_, value_, keys_, tries_, dirty_;
   Counters *d_cnt_ = nullptr;
@@ -160,6 +163,7 @@ class Forest {
   size_t vchunk_used_ = 0;
   DevBuf st_keys_, st_kidx_, st_vidx_, st_hps_, st_tid_;
   DevBuf ws_, ws2_, rw_;
+  HostStager *stager_ = nullptr; // pinned chunked staging of large host batches (made on first use)
   DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_, hc_lmeta_, hc_lmeta_tmp_;
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation

@@ -6,12 +6,14 @@
 
 #include <algorithm>
 #include <mutex>
+#include <new>
 
 #include "get.h"
 #include "locknu.h"
 #include "phasehc.h"
 #include "phasenu.h"
 #include "prims.h"
+#include "staging.h"
 
 namespace gmpt {
 
@@ -106,6 +108,7 @@ Forest::~Forest() {
   (void)hipSetDevice(device_);
   if (stream_ || arena_) (void)hipStreamSynchronize(stream_);
   if (arena_) return;  // nothing of its own to release
+  delete stager_;
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
                     &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_,
                     &hc_lmeta_, &hc_lmeta_tmp_, &rw_};
@@ -286,14 +289,25 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
     // and the key arena bytes the packed keys take (pairs may overlap, so
     // the key buffer's size does not bound it)
     uint64_t kb = 0, vb = 0, ks = 0;
-    for (uint32_t i = 0; i < n; i++) {
-      const int64_t s = b.keys_indexs[2 * (size_t)i], e = b.keys_indexs[2 * (size_t)i + 1];
-      if (e >= 0 && (uint64_t)e + 1 > kb) kb = (uint64_t)e + 1;
-      const int64_t L = (s >= 0 && e >= s && e - s < kMaxKeyNibbles) ? e - s + 1 : 1;
-      ks += (uint64_t)((L / 2 + 8) & ~7ll);
-      if (b.values_indexs) {
-        int64_t ve = b.values_indexs[2 * (size_t)i + 1];
-        if (ve >= 0 && (uint64_t)ve + 1 > vb) vb = (uint64_t)ve + 1;
+    // large batches go through the pinned chunked stager (never in a forest
+    // that may not allocate)
+    const bool staged = !arena_ && 24ull * n >= HostStager::kMinBytes;
+    if (staged && !stager_ && !(stager_ = new (std::nothrow) HostStager)) {
+      set_error("out of host memory");
+      return -2;
+    }
+    if (staged) {
+      stager_->extents(b.keys_indexs, b.values_indexs, n, &kb, &vb, &ks, kMaxKeyNibbles);
+    } else {
+      for (uint32_t i = 0; i < n; i++) {
+        const int64_t s = b.keys_indexs[2 * (size_t)i], e = b.keys_indexs[2 * (size_t)i + 1];
+        if (e >= 0 && (uint64_t)e + 1 > kb) kb = (uint64_t)e + 1;
+        const int64_t L = (s >= 0 && e >= s && e - s < kMaxKeyNibbles) ? e - s + 1 : 1;
+        ks += (uint64_t)((L / 2 + 8) & ~7ll);
+        if (b.values_indexs) {
+          int64_t ve = b.values_indexs[2 * (size_t)i + 1];
+          if (ve >= 0 && (uint64_t)ve + 1 > vb) vb = (uint64_t)ve + 1;
+        }
       }
     }
     d->keys_bytes = kb;
@@ -303,30 +317,38 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
     if ((r = ensure_ws(st_keys_, kb + 64))) return r;
     if ((r = ensure_ws(st_kidx_, 8ull * n + 64))) return r;
     if ((r = ensure_ws(st_vidx_, 16ull * n + 64))) return r;
-    HIP_TRY(hipMemcpyAsync(st_keys_.p, b.keys_hexs, kb, hipMemcpyHostToDevice, stream_));
-    HIP_TRY(hipMemcpyAsync(st_kidx_.p, b.keys_indexs, 8ull * n, hipMemcpyHostToDevice, stream_));
+    StageSeg seg[6];
+    int ns = 0;
+    seg[ns++] = {st_keys_.p, b.keys_hexs, kb};
+    seg[ns++] = {st_kidx_.p, b.keys_indexs, 8ull * n};
     d->keys_hexs = (const uint8_t *)st_keys_.p;
     d->keys_indexs = (const int32_t *)st_kidx_.p;
     if (b.values_indexs) {
-      HIP_TRY(hipMemcpyAsync(st_vidx_.p, b.values_indexs, 16ull * n, hipMemcpyHostToDevice, stream_));
+      seg[ns++] = {st_vidx_.p, b.values_indexs, 16ull * n};
       d->values_indexs = (const int64_t *)st_vidx_.p;
     }
     if (b.values_hps) {
       if ((r = ensure_ws(st_hps_, 8ull * n + 64))) return r;
-      HIP_TRY(hipMemcpyAsync(st_hps_.p, b.values_hps, 8ull * n, hipMemcpyHostToDevice, stream_));
+      seg[ns++] = {st_hps_.p, b.values_hps, 8ull * n};
       d->values_hps = (const uint64_t *)st_hps_.p;
     }
     if (b.trie_ids) {
       if ((r = ensure_ws(st_tid_, 4ull * n + 64))) return r;
-      HIP_TRY(hipMemcpyAsync(st_tid_.p, b.trie_ids, 4ull * n, hipMemcpyHostToDevice, stream_));
+      seg[ns++] = {st_tid_.p, b.trie_ids, 4ull * n};
       d->trie_ids = (const int32_t *)st_tid_.p;
     }
     d->on_device = true;
     if (copy_values) {
       uint8_t *vs = value_space(vb);
       if (!vs) return -2;
-      if (vb) HIP_TRY(hipMemcpyAsync(vs, b.values_bytes, vb, hipMemcpyHostToDevice, stream_));
+      seg[ns++] = {vs, b.values_bytes, vb};
       d->values_bytes = vs;
+    }
+    if (staged) {
+      HIP_TRY(stager_->copy(stream_, seg, ns));
+    } else {
+      for (int q = 0; q < ns; q++)
+        if (seg[q].bytes) HIP_TRY(hipMemcpyAsync(seg[q].dst, seg[q].src, seg[q].bytes, hipMemcpyHostToDevice, stream_));
     }
   } else if (copy_values && !b.adopt_values) {
     uint8_t *vs = value_space(b.values_total);
@@ -505,6 +527,9 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     // slots: branches <= 2n+2, shorts <= 2(2n+2), values: 2n+2 for elements + n for updates
     if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, 3ull * n + 2, d.key_slots))) return r;
     const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
+    // a bulk build into a forest of empty tries skips the descent and grouping
+    const bool empty = all_empty_;
+    all_empty_ = false;
     NuArgs a;
     void *temp = nullptr;
     uint32_t trie_bits = 0;
@@ -517,7 +542,7 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
       int cur = 0;
       for (int pass = 0; pass >= 0;) {
         if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) break;
-        HIP_TRY(nu_descend_and_group(a, temp, stream_));
+        HIP_TRY(nu_descend_and_group(a, temp, stream_, empty));
         if ((r = sync_counters())) break;
         pass = next_pass(pass);
         if (pass >= 0) {
@@ -536,8 +561,8 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
       HIP_TRY(nu_regroup(a, m, m, temp, stream_));
       stats_.regroups++;
     }
-    HIP_TRY(nu_emit(a, temp, stream_));
-    HIP_TRY(nu_build(a, temp, stream_));
+    HIP_TRY(nu_emit(a, temp, stream_, empty));
+    HIP_TRY(nu_build(a, temp, stream_, empty));
     if ((r = sync_counters())) return r;
     // the batch's packed keys stay only if a new node refers to them: a batch
     // of updates alone (no elements) leaves them unused, so the arena keeps
@@ -711,6 +736,7 @@ int Forest::clear() {
   rec_ = LeafRecords();
   HIP_TRY(hc_clear(d_cnt_, (TrieHdr *)tries_.p, ntries_, stream_));
   memset(&cnt_, 0, sizeof(cnt_));
+  all_empty_ = true;
   key_used_ = 0;
   // keep the first value chunk, release the others
   for (size_t i = 1; i < vchunks_.size() && !arena_; i++) HIP_TRY(hipFree(vchunks_[i].p));
@@ -959,6 +985,7 @@ int Forest::puts_latching(const BatchDesc &b, void **targets, int *n_targets) {
     uint64_t vbase = 0;
     if ((r = stage_batch(b, &d, &vbase, true))) return r;
     const uint32_t n = b.n;
+    all_empty_ = false;
     if ((r = ensure_pools(n, 2ull * n, n, d.key_slots))) return r;
     const bool has_tid = d.trie_ids != nullptr && ntries_ > 1;
     NuArgs a;

@@ -146,14 +146,17 @@ hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, 
 hipError_t nu_unique(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, void *temp, hipStream_t s);
 // Descends every unique key and groups them by stop point; sets SC_MERGE when
 // a split group is interrupted by keys below the split node (nu_regroup).
-hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s);
+// empty: every trie of the forest is empty (the host knows it), so the keys
+// need no descent, the groups are the tries and the elements are emitted
+// here too (nu_emit then does nothing, nu_build skips the update kernel).
+hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s, bool empty = false);
 // Makes every group contiguous (stable sort of the unique keys by group),
 // swapping the unique-key arrays of `a`, and regroups.
 hipError_t nu_regroup(NuArgs &a, uint32_t m, uint32_t nruns, void *temp, hipStream_t s);
 // Emits the element sequence E of every group.
-hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s);
+hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s, bool empty = false);
 // Builds every group's subtree; node and dirty-list slots are fixed by
 // position from base_branch / base_short / base_value (see phasenu.hip).
-hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s);
+hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty = false);
 
 }  // namespace gmpt

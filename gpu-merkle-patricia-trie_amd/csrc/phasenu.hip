@@ -693,6 +693,60 @@ __global__ __launch_bounds__(256) void k_group_merge(NuArgs a) {
   }
 }
 
+// Every trie of the forest empty (a bulk build): each unique key stops at its
+// trie's empty root, the groups are the tries, no key is an update and none
+// splits a node, so each unique key is exactly one element.  One pass writes
+// what k_descend, the group kernels and the element emission would: stops,
+// 1-based group ids, group descriptors, merge targets, the element sequence
+// and level 0 of the minimum tree.  Multi-trie batches give it the exclusive
+// scan of the trie-start flags in grp (k_empty_flags); a single trie has one
+// group.
+__global__ __launch_bounds__(256) void k_empty_flags(NuArgs a) {
+  const uint32_t i = grid_tid();
+  const uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= a.n) return;
+  a.ecount[i] = i < m && (i == 0 || a.u_trie[i] != a.u_trie[i - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_empty_groups(NuArgs a, int multi) {
+  const uint32_t i = grid_tid();
+  const uint32_t m = a.scal[SC_M_UNIQ];
+  if (i >= m) return;
+  const uint32_t t = a.u_trie[i];
+  if (t >= a.ntries) raise_error(a.P.cnt, ERR_BAD_KEY, a.u_idx[i]);
+  const bool start = i == 0 || (multi && a.u_trie[i - 1] != t);
+  const uint32_t g = multi ? a.grp[i] + (start ? 1u : 0u) : 1u;  // 1-based
+  a.grp[i] = g;
+  Stop st;
+  st.stop = REF_NULL;
+  st.attach = REF_NULL;
+  st.base = 0;
+  st.div = 0;
+  st.kind = STOP_ROOT_EMPTY;
+  st.slot = 0;
+  st.less = 0;
+  st.pad = 0;
+  a.stop[i] = st;
+  if (start) {
+    GroupInfo gi;
+    gi.attach = REF_NULL;
+    gi.split = REF_NULL;
+    gi.trie = t;
+    gi.base = 0;
+    gi.slot = 0;
+    gi.has_split = 0;
+    a.ginfo[g - 1] = gi;
+    a.merge[g - 1] = g - 1;
+  }
+  a.e_src[i] = i;
+  a.e_grp[i] = g - 1;
+  if (i > 0) a.st[i - 1] = start ? 0 : (uint8_t)(a.u_lcp[i] + 1);
+  if (i + 1 == m) {
+    a.scal[SC_NGROUP] = g;
+    a.scal[SC_NE] = m;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_gkey(NuArgs a, uint32_t m) {
   uint32_t i = grid_tid();
   if (i >= m) return;
@@ -1440,7 +1494,18 @@ static hipError_t nu_groups(const NuArgs &a, void *temp, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s) {
+hipError_t nu_descend_and_group(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
+  if (empty) {  // bulk build into empty tries: groups and elements in one pass
+    const int multi = a.tid != nullptr;
+    if (multi) {
+      k_empty_flags<<<nblk(a.n), kB, 0, s>>>(a);
+      GMPT_LAUNCH_CHECK();
+      hipError_t e = scan_exclusive_u32(a.ecount, a.grp, a.n, nullptr, temp, s);
+      if (e != hipSuccess) return e;
+    }
+    k_empty_groups<<<nblk(a.n), kB, 0, s>>>(a, multi);
+    return hipGetLastError();
+  }
   k_descend<<<nblk(a.n), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   hipError_t e = nu_groups(a, temp, s);
@@ -1469,7 +1534,8 @@ hipError_t nu_regroup(NuArgs &a, uint32_t m, uint32_t nruns, void *temp, hipStre
   return nu_groups(a, temp, s);
 }
 
-hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s) {
+hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
+  if (empty) return hipSuccess;  // k_empty_groups emitted the elements
   k_emit_count<<<nblk(a.n), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(a.ecount, a.ecount, a.n, nullptr, temp, s);
@@ -1478,7 +1544,7 @@ hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
+hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
   uint32_t ne_max = 2 * a.n + 2;
   size_t cnt = ne_max;  // bound on the entries of each level
   for (int k = 1; k < a.st_levels; k++) {
@@ -1486,8 +1552,10 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s) {
     k_mt_level<<<nblk(cnt), kB, 0, s>>>(a, k);
     GMPT_LAUNCH_CHECK();
   }
-  k_updates<<<nblk(a.n), kB, 0, s>>>(a);
-  GMPT_LAUNCH_CHECK();
+  if (!empty) {  // empty tries hold no key to update
+    k_updates<<<nblk(a.n), kB, 0, s>>>(a);
+    GMPT_LAUNCH_CHECK();
+  }
   k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a, ne_max);
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(a.bid, a.bid, ne_max, a.scal + SC_NREP_TOT, temp, s);

@@ -299,6 +299,18 @@ def test_keccak_layouts_give_same_root(gmpt, workloads, oracle, spread_max):
     assert t.get_root_hash() == oracle.root_of(b)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_large_host_batch_through_pinned_staging(gmpt, workloads, oracle, mode):
+    # a host batch large enough (index pairs >= 8 MB) for the chunked pinned
+    # stager: every array crosses several 8 MB chunks of the staging ring
+    b = _rand_batch(workloads, 97, 400000, 70, 110)
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash() == oracle.root_of(b)
+    sub = b.subset(range(0, b.n, 4099))
+    assert t.get_values(sub) == [sub.value(i) for i in range(sub.n)]
+    t.close()
+
+
 def _leaf_case(workloads, case):
     if case == "c2":
         return _rand_batch(workloads, 93, 5000, 70, 110)
