@@ -118,6 +118,8 @@ def bench_phase(pkg, args):
     db = pkg.DeviceBatch(b, adopt_values=True)
     t = pkg.MPT(device=0)
     t.set_option("hc_fuse_leaves", 0 if args.no_fuse_leaves else 1)
+    if args.pool_host:  # the trie state in pinned host memory (tries beyond HBM)
+        t.set_option("pool_memory", 1)
     fused = not args.no_fuse_leaves
     if args.spread_max is None:
         spread_max, tune_ms = t.autotune(db)  # picks the Keccak layout threshold on this batch
@@ -172,7 +174,8 @@ def bench_phase(pkg, args):
         "data": "synthetic (seeded uniform 32-byte keys, 70-110-byte random values)",
         "config": {"workload": "PhaseNU+PhaseHC, 1,000,000 unique 32-byte keys, 70-110-byte values, "
                                "one batch into an empty trie, inputs resident in HBM",
-                   "keys": b.n, "value_bytes": b.value_bytes},
+                   "keys": b.n, "value_bytes": b.value_bytes,
+                   "trie_state": "pinned host memory" if args.pool_host else "HBM"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PERM_PEAK_G, "unit": "Gperm/s",
                      "frac": achieved / PERM_PEAK_G,
                      "traffic": LEAF_TRAFFIC_BYTES[fused] if b.n == 1_000_000 else None,
@@ -611,6 +614,8 @@ def main():
                     help="phase: seconds of the served sub-object's plain-call run")
     ap.add_argument("--spread-max", type=int, default=None,
                     help="phase: fixed hc_spread_max instead of the autotuner's choice")
+    ap.add_argument("--pool-host", action="store_true",
+                    help="phase: node pools, key and value arenas in pinned host memory (tries beyond HBM)")
     ap.add_argument("--no-fuse-leaves", action="store_true",
                     help="phase: leaf level as encode kernel + HBM scratch + permutation kernel")
     args = ap.parse_args()

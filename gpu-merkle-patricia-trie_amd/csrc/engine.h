@@ -36,6 +36,7 @@ struct BatchDesc {
 struct DevBuf {
   void *p = nullptr;
   size_t cap = 0;
+  bool host = false;  // pinned host memory mapped into the device's address space
 };
 
 // A fixed device region a Forest can take all its device memory from, with
@@ -138,6 +139,8 @@ class Forest {
   int grow(DevBuf &b, size_t need, size_t keep);
   int ensure_ws(DevBuf &b, size_t bytes);
   int arena_exhausted();
+  bool is_pool(const DevBuf &b) const;
+  void release(DevBuf &b);
   int sync_counters();
   bool grow_key_arena_after_error();
   int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
@@ -168,6 +171,10 @@ class Forest {
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
   bool hc_fuse_leaves_ = true;   // level-0 leaves: one encode+hash kernel (k_hc_leaf_direct)
+  // "pool_memory" 1: the trie state (node pools, key and value arenas, dirty
+  // list, trie headers) grows in pinned host memory the kernels reach over
+  // the host link, so a forest can hold more than the GPU's memory
+  bool pool_host_ = false;
   LeafRecords rec_;              // bins the last build recorded for its new leaves (in ws_)
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel

@@ -112,10 +112,8 @@ Forest::~Forest() {
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
                     &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_,
                     &hc_lmeta_, &hc_lmeta_tmp_, &rw_};
-  for (DevBuf *b : bufs)
-    if (b->p) (void)hipFree(b->p);
-  for (DevBuf &b : vchunks_)
-    if (b.p) (void)hipFree(b.p);
+  for (DevBuf *b : bufs) release(*b);
+  for (DevBuf &b : vchunks_) release(b);
   if (d_cnt_) (void)hipFree(d_cnt_);
   if (h_cnt_) (void)hipHostFree(h_cnt_);
   if (h_scal_) (void)hipHostFree(h_scal_);
@@ -162,31 +160,51 @@ uint64_t Forest::pool_bytes_reserved() const {
   return branch_.cap + shrt_.cap + value_.cap + keys_.cap + dirty_.cap + v;
 }
 
+bool Forest::is_pool(const DevBuf &b) const {
+  return &b == &branch_ || &b == &shrt_ || &b == &value_ || &b == &keys_ || &b == &dirty_ || &b == &tries_;
+}
+
+void Forest::release(DevBuf &b) {
+  if (b.p && !arena_) (void)(b.host ? hipHostFree(b.p) : hipFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  b.host = false;
+}
+
 // Regrow a device buffer to at least `need` bytes, keeping the first `keep`.
+// With pool_memory = host, the trie's pools grow in pinned host memory.
 int Forest::grow(DevBuf &b, size_t need, size_t keep) {
   if (b.cap >= need) return 0;
   void *p = nullptr;
   size_t cap;
+  const bool host = !arena_ && pool_host_ && is_pool(b);
   if (arena_) {
     // exactly what is asked: the region holds one call's forest
     cap = align_up(need, 256);
     if (!(p = arena_->take(cap))) return arena_exhausted();
   } else {
     cap = align_up(std::max(need, b.cap + b.cap / 2), 4096);
-    hipError_t e = hipMalloc(&p, cap);
+    hipError_t e = host ? hipHostMalloc(&p, cap, hipHostMallocDefault) : hipMalloc(&p, cap);
     if (e != hipSuccess) {
-      set_error(std::string("device pool allocation failed: ") + hipGetErrorString(e));
+      set_error(std::string(host ? "host pool allocation failed: " : "device pool allocation failed: ") +
+                hipGetErrorString(e));
       return -2;
     }
   }
-  if (b.p && keep) HIP_TRY(hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, stream_));
+  if (b.p && keep) HIP_TRY(hipMemcpyAsync(p, b.p, keep, hipMemcpyDefault, stream_));
   // fresh slots read as zero (empty children, unlocked, clean) even before a
   // writer publishes them, which keeps optimistic LockNU readers in bounds
-  HIP_TRY(hipMemsetAsync((uint8_t *)p + keep, 0, cap - keep, stream_));
+  if (host) {
+    HIP_TRY(hipStreamSynchronize(stream_));
+    memset((uint8_t *)p + keep, 0, cap - keep);
+  } else {
+    HIP_TRY(hipMemsetAsync((uint8_t *)p + keep, 0, cap - keep, stream_));
+  }
   HIP_TRY(hipStreamSynchronize(stream_));
-  if (b.p && !arena_) HIP_TRY(hipFree(b.p));
+  release(b);
   b.p = p;
   b.cap = cap;
+  b.host = host;
   return 0;
 }
 
@@ -206,9 +224,7 @@ int Forest::ensure_ws(DevBuf &b, size_t bytes) {
   }
   if (b.p) {
     HIP_TRY(hipStreamSynchronize(stream_));
-    HIP_TRY(hipFree(b.p));
-    b.p = nullptr;
-    b.cap = 0;
+    release(b);
   }
   size_t cap = align_up(bytes + bytes / 4, 1 << 20);
   hipError_t e = hipMalloc(&b.p, cap);
@@ -266,7 +282,8 @@ uint8_t *Forest::value_space(uint64_t bytes) {
   if (vchunks_.empty() || vchunk_used_ + bytes > vchunks_.back().cap) {
     DevBuf c;
     c.cap = align_up(std::max<uint64_t>(bytes, 64ull << 20), 1 << 20);
-    if (hipMalloc(&c.p, c.cap) != hipSuccess) {
+    c.host = pool_host_;
+    if ((c.host ? hipHostMalloc(&c.p, c.cap, hipHostMallocDefault) : hipMalloc(&c.p, c.cap)) != hipSuccess) {
       set_error("value arena allocation failed");
       return nullptr;
     }
@@ -348,13 +365,13 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
       HIP_TRY(stager_->copy(stream_, seg, ns));
     } else {
       for (int q = 0; q < ns; q++)
-        if (seg[q].bytes) HIP_TRY(hipMemcpyAsync(seg[q].dst, seg[q].src, seg[q].bytes, hipMemcpyHostToDevice, stream_));
+        if (seg[q].bytes) HIP_TRY(hipMemcpyAsync(seg[q].dst, seg[q].src, seg[q].bytes, hipMemcpyDefault, stream_));
     }
   } else if (copy_values && !b.adopt_values) {
     uint8_t *vs = value_space(b.values_total);
     if (!vs) return -2;
     if (b.values_total)
-      HIP_TRY(hipMemcpyAsync(vs, b.values_bytes, b.values_total, hipMemcpyDeviceToDevice, stream_));
+      HIP_TRY(hipMemcpyAsync(vs, b.values_bytes, b.values_total, hipMemcpyDefault, stream_));
     d->values_bytes = vs;
   }
   // device batches: the packed keys of non-overlapping pairs; the packing
@@ -691,7 +708,7 @@ int Forest::root_hash(uint32_t trie, uint8_t *out, int *size) {
     return -3;
   }
   TrieHdr h;
-  HIP_TRY(hipMemcpyAsync(&h, (TrieHdr *)tries_.p + trie, sizeof(TrieHdr), hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(&h, (TrieHdr *)tries_.p + trie, sizeof(TrieHdr), hipMemcpyDefault, stream_));
   HIP_TRY(hipStreamSynchronize(stream_));
   *size = (int)h.root_hash_size;
   if (out) memcpy(out, h.root_hash, h.root_hash_size);
@@ -707,7 +724,7 @@ int Forest::root_hashes(uint32_t first, uint32_t count, uint8_t *out, int32_t *s
   }
   if (count == 0) return 0;
   std::vector<TrieHdr> h(count);
-  HIP_TRY(hipMemcpyAsync(h.data(), (TrieHdr *)tries_.p + first, sizeof(TrieHdr) * count, hipMemcpyDeviceToHost,
+  HIP_TRY(hipMemcpyAsync(h.data(), (TrieHdr *)tries_.p + first, sizeof(TrieHdr) * count, hipMemcpyDefault,
                          stream_));
   HIP_TRY(hipStreamSynchronize(stream_));
   for (uint32_t i = 0; i < count; i++) {
@@ -739,7 +756,7 @@ int Forest::clear() {
   all_empty_ = true;
   key_used_ = 0;
   // keep the first value chunk, release the others
-  for (size_t i = 1; i < vchunks_.size() && !arena_; i++) HIP_TRY(hipFree(vchunks_[i].p));
+  for (size_t i = 1; i < vchunks_.size(); i++) release(vchunks_[i]);
   if (vchunks_.size() > 1) vchunks_.resize(1);
   vchunk_used_ = 0;
   return 0;
@@ -752,6 +769,18 @@ int Forest::set_option(const std::string &name, int64_t value) {
       return -3;
     }
     hc_spread_max_ = (uint32_t)value;
+    return 0;
+  }
+  if (name == "pool_memory") {
+    if (value != 0 && value != 1) {
+      set_error("pool_memory must be 0 (device) or 1 (host)");
+      return -3;
+    }
+    if (arena_ && value) {
+      set_error("a forest over a fixed device region keeps its pools there");
+      return -3;
+    }
+    pool_host_ = value != 0;
     return 0;
   }
   if (name == "hc_fuse_leaves") {
@@ -773,6 +802,10 @@ int Forest::get_option(const std::string &name, int64_t *value) const {
   }
   if (name == "hc_fuse_leaves") {
     *value = hc_fuse_leaves_ ? 1 : 0;
+    return 0;
+  }
+  if (name == "pool_memory") {
+    *value = pool_host_ ? 1 : 0;
     return 0;
   }
   set_error("unknown option: " + name);

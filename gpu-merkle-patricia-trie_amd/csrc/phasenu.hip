@@ -994,6 +994,9 @@ __device__ __forceinline__ ST st_of(const NuArgs &a) {
 __global__ __launch_bounds__(256) void k_split_reps(NuArgs a, uint32_t ne_max) {
   uint32_t j = grid_tid();
   if (j >= ne_max) return;
+  // the build's dirty-list base: the list does not change until the build's
+  // last kernel advances it
+  if (j == 0) a.scal[SC_DBASE] = a.P.cnt->n_dirty;
   ST s = st_of(a);
   bool rep = false;
   if (j < s.nsp) {
@@ -1023,7 +1026,7 @@ struct Slots {
 __device__ __forceinline__ Slots slots_of(const NuArgs &a) {
   Slots q;
   q.ne = a.scal[SC_NE];
-  q.nrep = a.scal[SC_NREP];
+  q.nrep = a.scal[SC_NREP_TOT];  // the branch count, from the representatives' scan
   q.dbase = a.scal[SC_DBASE];
   return q;
 }
@@ -1193,6 +1196,14 @@ __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
   uint32_t e = grid_tid();
   ST s = st_of(a);
   Slots q = slots_of(a);
+  if (e == 0) {
+    // the build's node and dirty-list slots are all assigned by position:
+    // advance the pool counters past them (no kernel of the build reads them)
+    a.P.cnt->n_branch = a.base_branch + q.nrep;
+    a.P.cnt->n_short = a.base_short + q.ne + q.nrep;
+    a.P.cnt->n_value = a.base_value + q.ne;
+    a.P.cnt->n_dirty = q.dbase + q.ne + 2 * q.nrep;
+  }
   if (e >= q.ne) return;
   ElemKey ek = elem_key(a, e);
   uint32_t La = e > 0 ? s.at(0, e - 1) : 0u;
@@ -1269,21 +1280,6 @@ __global__ __launch_bounds__(256) void k_updates(NuArgs a) {
   init_value(a, ref_idx(vr), a.u_idx[i]);
 }
 
-// Records this build's slot bases before it starts and advances the pool and
-// dirty-list counters after it.
-__global__ void k_build_begin(NuArgs a, const uint32_t *nrep_total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  a.scal[SC_NREP] = *nrep_total;
-  a.scal[SC_DBASE] = a.P.cnt->n_dirty;
-}
-__global__ void k_build_end(NuArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t ne = a.scal[SC_NE], nrep = a.scal[SC_NREP];
-  a.P.cnt->n_branch = a.base_branch + nrep;
-  a.P.cnt->n_short = a.base_short + ne + nrep;
-  a.P.cnt->n_value = a.base_value + ne;
-  a.P.cnt->n_dirty = a.scal[SC_DBASE] + ne + 2 * nrep;
-}
 
 }  // namespace
 
@@ -1560,15 +1556,11 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(a.bid, a.bid, ne_max, a.scal + SC_NREP_TOT, temp, s);
   if (e != hipSuccess) return e;
-  k_build_begin<<<1, 64, 0, s>>>(a, a.scal + SC_NREP_TOT);
-  GMPT_LAUNCH_CHECK();
   k_init_branches<<<nblk(ne_max), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   k_link_branches<<<nblk(ne_max), kB, 0, s>>>(a);
   GMPT_LAUNCH_CHECK();
   k_link_elems<<<nblk(ne_max), kB, 0, s>>>(a);
-  GMPT_LAUNCH_CHECK();
-  k_build_end<<<1, 64, 0, s>>>(a);
   return hipGetLastError();
 }
 

@@ -105,7 +105,7 @@ hipError_t HostStager::copy(hipStream_t s, const StageSeg *segs, int nseg) {
         const size_t a = std::min(len, (size_t)p * per), b = std::min(len, a + per);
         if (b > a) memcpy(pin + a, src + off + a, b - a);
       });
-      if ((e = hipMemcpyAsync(dst + off, pin, len, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(dst + off, pin, len, hipMemcpyDefault, s)) != hipSuccess) return e;
       if ((e = hipEventRecord(ev_[k], s)) != hipSuccess) return e;
       pending_[k] = true;
       k = (k + 1) % kBufs;

@@ -159,6 +159,43 @@ def test_incremental_batches(gmpt, workloads, oracle, modes):
     assert t.get_values(sample) == o.gets(sample)
 
 
+@pytest.mark.parametrize("modes", [("2phase", "2phase"), ("latching", "latching"), ("2phase", "latching")])
+def test_trie_state_in_host_memory(gmpt, workloads, oracle, modes):
+    """Out-of-HBM tries (the reference's UMMPT, um_mpt.cuh:13-237): with
+    pool_memory = 1 the node pools, key and value arenas and dirty list live
+    in pinned host memory the kernels reach over the host link.  A build, a
+    second batch of overwrites and new keys (descent marks, splits, latches
+    on host memory), gets and the node export all match the oracle and a
+    device-pool forest."""
+    rng = np.random.default_rng(71)
+    base = _rand_batch(workloads, 71, 20000, 30, 120)
+    upd = base.subset(rng.choice(base.n, 3000, replace=False))
+    flat, lens = workloads.random_values(rng, 3000, 40, 80)
+    upd = workloads.batch_from_hex([upd.key(i) for i in range(upd.n)],
+                                   [flat[int(lens[:i].sum()):int(lens[:i + 1].sum())].tobytes() for i in range(3000)])
+    second = workloads.concat_batches([upd, _rand_batch(workloads, 72, 5000, 1, 200)])
+    second = second.subset(rng.permutation(second.n))
+    t = gmpt.MPT(device=0)
+    t.set_option("pool_memory", 1)
+    assert t.get_option("pool_memory") == 1
+    d = gmpt.MPT(device=0)
+    o = oracle.OracleMPT()
+    for b, mode in ((base, modes[0]), (second, modes[1])):
+        _put(t, b, mode)
+        _put(d, b, mode)
+        o.puts(b)
+        o.hash()
+        assert t.get_root_hash() == o.root()
+    sample = second.subset(range(0, second.n, 41))
+    assert t.get_values(sample) == o.gets(sample)
+    # LockNU numbers the nodes its latched splits create in arrival order, so
+    # the two forests list the same nodes in different orders
+    assert sorted(t.flush_dirty_nodes(sample)) == sorted(d.flush_dirty_nodes(sample))
+    assert t.stats()["pool_bytes_reserved"] > 0
+    t.close()
+    d.close()
+
+
 def test_repeated_updates_rewrite_values_in_place(gmpt, workloads, oracle):
     """PhaseNU update-only batches rewrite the keys' value nodes in place and
     keep no packed keys: across five rounds of updates the pools grow by the
