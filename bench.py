@@ -118,6 +118,7 @@ def bench_phase(pkg, args):
     db = pkg.DeviceBatch(b, adopt_values=True)
     t = pkg.MPT(device=0)
     t.set_option("hc_fuse_leaves", 0 if args.no_fuse_leaves else 1)
+    t.set_option("hc_fuse_inner", 0 if args.no_fuse_inner else 1)
     if args.pool_host:  # the trie state in pinned host memory (tries beyond HBM)
         t.set_option("pool_memory", 1)
     fused = not args.no_fuse_leaves
@@ -616,6 +617,8 @@ def main():
                     help="phase: fixed hc_spread_max instead of the autotuner's choice")
     ap.add_argument("--pool-host", action="store_true",
                     help="phase: node pools, key and value arenas in pinned host memory (tries beyond HBM)")
+    ap.add_argument("--no-fuse-inner", action="store_true",
+                    help="phase: inner levels as 16-lane encode kernel + HBM scratch + permutation kernel")
     ap.add_argument("--no-fuse-leaves", action="store_true",
                     help="phase: leaf level as encode kernel + HBM scratch + permutation kernel")
     args = ap.parse_args()

@@ -171,6 +171,7 @@ class Forest {
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
   bool hc_fuse_leaves_ = true;   // level-0 leaves: one encode+hash kernel (k_hc_leaf_direct)
+  bool hc_fuse_inner_ = true;    // large inner levels: encode + hash with the messages in LDS (k_hc_inner_wave)
   // "pool_memory" 1: the trie state (node pools, key and value arenas, dirty
   // list, trie headers) grows in pinned host memory the kernels reach over
   // the host link, so a forest can hold more than the GPU's memory
@@ -179,6 +180,7 @@ class Forest {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   hipEvent_t ev_meta_ = nullptr;               // the classification bins have reached the host
+  SideStream side_{};                          // second stream for work beside a hash level's main kernel
   OpStats stats_;
 };
 

@@ -76,6 +76,10 @@ Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
   if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return;
   if (hipEventCreate(&ev2_) != hipSuccess || hipEventCreate(&ev3_) != hipSuccess) return;
   if (hipEventCreateWithFlags(&ev_meta_, hipEventDisableTiming) != hipSuccess) return;
+  // default priority, like the main stream
+  if (hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking) != hipSuccess) return;
+  if (hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming) != hipSuccess) return;
+  if (hipEventCreateWithFlags(&side_.join, hipEventDisableTiming) != hipSuccess) return;
   if (grow(tries_, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1), 0) != 0) return;
   if (hipMemset(tries_.p, 0, sizeof(TrieHdr) * std::max<uint32_t>(ntries, 1)) != hipSuccess) return;
   if (hipHostMalloc(&h_meta_, sizeof(HcMeta)) != hipSuccess) return;
@@ -123,6 +127,12 @@ Forest::~Forest() {
   if (ev2_) (void)hipEventDestroy(ev2_);
   if (ev3_) (void)hipEventDestroy(ev3_);
   if (ev_meta_) (void)hipEventDestroy(ev_meta_);
+  if (side_.fork) (void)hipEventDestroy(side_.fork);
+  if (side_.join) (void)hipEventDestroy(side_.join);
+  if (side_.stream) {
+    (void)hipStreamSynchronize(side_.stream);
+    (void)hipStreamDestroy(side_.stream);
+  }
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -666,7 +676,9 @@ int Forest::hash(void *targets, int n_targets) {
         if (cnt > skip)
           HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
                                 encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
-                                timed ? ev3_ : nullptr, n_ext));
+                                timed ? ev3_ : nullptr, n_ext,
+                                hc_fuse_inner_ ? &M.hist[lv * kHcClasses] : nullptr,
+                                side_.stream ? &side_ : nullptr));
         stats_.hash_launches++;
         stats_.hashed_nodes += cnt;
         if (!leaves && n_ext >= kExtDirectMin && cnt > hc_spread_max_) stats_.ext_direct_nodes += n_ext;
@@ -783,12 +795,12 @@ int Forest::set_option(const std::string &name, int64_t value) {
     pool_host_ = value != 0;
     return 0;
   }
-  if (name == "hc_fuse_leaves") {
+  if (name == "hc_fuse_leaves" || name == "hc_fuse_inner") {
     if (value != 0 && value != 1) {
-      set_error("hc_fuse_leaves must be 0 or 1");
+      set_error(name + " must be 0 or 1");
       return -3;
     }
-    hc_fuse_leaves_ = value != 0;
+    (name == "hc_fuse_leaves" ? hc_fuse_leaves_ : hc_fuse_inner_) = value != 0;
     return 0;
   }
   set_error("unknown option: " + name);
@@ -800,8 +812,8 @@ int Forest::get_option(const std::string &name, int64_t *value) const {
     *value = hc_spread_max_;
     return 0;
   }
-  if (name == "hc_fuse_leaves") {
-    *value = hc_fuse_leaves_ ? 1 : 0;
+  if (name == "hc_fuse_leaves" || name == "hc_fuse_inner") {
+    *value = (name == "hc_fuse_leaves" ? hc_fuse_leaves_ : hc_fuse_inner_) ? 1 : 0;
     return 0;
   }
   if (name == "pool_memory") {
@@ -821,6 +833,7 @@ int Forest::autotune(const BatchDesc &sample, int mode, int64_t *chosen, double 
   for (uint32_t c : kCand) {
     t.hc_spread_max_ = c;
     t.hc_fuse_leaves_ = hc_fuse_leaves_;
+    t.hc_fuse_inner_ = hc_fuse_inner_;
     double ms = 1e30;
     for (int rep = 0; rep < 4; rep++) {  // the first run warms the pools; the best of the other three counts
       int r;

@@ -41,15 +41,29 @@ struct LeafRecords {
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp = nullptr,
                        uint64_t *lmeta = nullptr, LeafRecords rec = LeafRecords());
+// A second stream (default priority) and two events for work that runs
+// beside the main stream's kernels within one hash pass.
+struct SideStream {
+  hipStream_t stream;
+  hipEvent_t fork, join;
+};
 // Encodes and hashes list[0..n) (one level: no node depends on another);
 // slot and enc_len are the level's slices of the classification arrays;
 // leaves = the leaf levels (0 and 1: only leaf short nodes).  Of an inner
 // level, the last n_ext entries are its extension class, hashed by the leaf
 // word builder when the level is above spread_max.  Events, when given, are
-// recorded around the permutation kernel.
+// recorded around the permutation kernel.  With cls (the level's node counts
+// per class, from the classification's bins), an inner level above
+// spread_max of 1- and 2-block branches and extensions (all but a sixteenth
+// of its nodes) is encoded and hashed with its messages in LDS in one launch
+// (k_hc_inner_wave); other levels, or all without cls, by a 16-lane encoding
+// kernel, an HBM scratch message and a permutation kernel.  With side, such a
+// level's few nodes of 3 or more blocks are hashed on the side stream beside
+// the LDS kernel (the main stream waits for them before this returns).
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s,
-                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr, uint32_t n_ext = 0);
+                         hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr, uint32_t n_ext = 0,
+                         const uint32_t *cls = nullptr, const SideStream *side = nullptr);
 // Encodes and hashes the classification's level-0 entries (leaves with
 // metadata, see hc_classify; their count is the sum of meta->hist[0 .. 8))
 // in one kernel; the count is read on the device, n_upper (>= it) sizes the

@@ -820,6 +820,178 @@ __device__ __forceinline__ void keccak_lane(const uint64_t *m, uint32_t nb, uint
   }
 }
 
+// Inner levels above the spread threshold whose nodes are mostly 1-block
+// branches and extensions, encoded and hashed in one launch with the
+// messages in LDS: one wave per block, one node per lane for the
+// permutation, but the encoding's loads spread over the whole wave.
+//  1. Each lane reads its branch's 16 child references and writes the bytes
+//     that depend on the branch alone into its LDS row: list header, 0x80
+//     for each empty slot, 0x80 for slot 16, Keccak padding.  An extension
+//     (a short node over a branch) is written whole by its lane: [compact
+//     key, 0xa0 + the child's digest].
+//  2. The wave's present children (a prefix sum of the lanes' child counts
+//     numbers them) are dealt out to the 64 lanes: each loads its children's
+//     digests -- every load of the wave in flight at once -- and writes
+//     0xa0 + digest at the child's offset in its parent's row.
+//  3. Each lane absorbs its row and permutes.
+// A node whose message would be taller than the rows (NB blocks), a branch
+// with a value in slot 16 or an embedded child, and any other short node
+// take the generic encoder and their scratch slot.  Blocks run in reverse
+// order: the level's nodes are sorted by block count with the extensions
+// last, so the few multi-block nodes start first and finish under the bulk.
+// Entries [skip_lo, skip_hi) are hashed elsewhere (by kernels on a second
+// stream) and left alone here.
+template <int NB>
+__global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__restrict__ list,
+                                                      const uint64_t *__restrict__ slot, uint32_t n,
+                                                      uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len,
+                                                      uint32_t skip_lo, uint32_t skip_hi) {
+  constexpr uint32_t RS = 17 * NB + 1;  // row stride in words (odd: lane-strided reads spread over the banks)
+  __shared__ uint64_t rows[64 * RS];
+  __shared__ ref_t nref[64];            // the lanes' nodes
+  __shared__ uint32_t nbase[64];        // first child number of each node
+  __shared__ uint32_t ninfo[64];        // child mask | header length << 16
+  __shared__ uint32_t bad[64];          // 1: the node takes the generic encoder
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = (gridDim.x - 1 - blockIdx.x) * 64 + lane;
+  const bool active = i < n && (i < skip_lo || i >= skip_hi);
+  const ref_t r = active ? list[i] : REF_NULL;
+  const bool branch = active && ref_type(r) == T_BRANCH;
+  bool fast = branch, ext = false;
+  uint32_t mask = 0, enc = 0, nb = 0, hn = 0;
+  uint64_t hdr = 0;
+  if (branch) {
+    const Branch &B = P.branch[ref_idx(r)];
+    const uint4 *cw = (const uint4 *)B.child;  // child[0 .. 15]: 64 bytes, 16-byte aligned
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 v = cw[q];
+      mask |= (v.x != REF_NULL ? 1u : 0u) << (4 * q);
+      mask |= (v.y != REF_NULL ? 2u : 0u) << (4 * q);
+      mask |= (v.z != REF_NULL ? 4u : 0u) << (4 * q);
+      mask |= (v.w != REF_NULL ? 8u : 0u) << (4 * q);
+    }
+    const uint32_t k = __popc(mask);
+    const uint32_t payload = 33 * k + (16 - k) + 1;
+    hdr = rlp_hdr(payload, 0xC0, 0xF7, &hn);
+    enc = hn + payload;
+    nb = keccak_blocks(enc);
+    fast = B.child[16] == REF_NULL && nb <= (uint32_t)NB && enc >= 32;
+  } else if (active && ref_type(r) == T_SHORT) {
+    ext = is_direct_ext(P.shrt[ref_idx(r)]);
+  }
+  const uint32_t k = fast ? (uint32_t)__popc(mask) : 0u;
+  const uint32_t incl = wave_inclusive_scan(k);
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+  nref[lane] = r;
+  nbase[lane] = incl - k;
+  ninfo[lane] = mask | (hn << 16);
+  bad[lane] = fast ? 0u : 1u;
+  uint64_t *row = rows + lane * RS;
+  if (fast)
+    for (uint32_t w = 0; w < 17 * nb; w++) row[w] = 0;
+  __syncthreads();  // rows zeroed; nodes, bases, masks visible to the wave
+  if (fast) {
+    uint8_t *rb = (uint8_t *)row;
+    for (uint32_t q = 0; q < hn; q++) rb[q] = (uint8_t)(hdr >> (8 * q));
+    uint32_t rank = 0;  // present children below slot s
+#pragma unroll
+    for (uint32_t s = 0; s < 16; s++) {
+      const bool has = (mask >> s) & 1u;
+      if (!has) rb[hn + 33 * rank + (s - rank)] = 0x80;
+      rank += has ? 1u : 0u;
+    }
+    rb[enc - 1] = 0x80;  // slot 16: empty
+    rb[enc] = 0x01;      // Keccak padding; enc < 136 * nb, so this byte is the message's own
+    rb[136 * nb - 1] |= 0x80;
+  } else if (ext) {
+    // [compact key, string of the child's digest]: the whole row from this lane
+    const Short &S = P.shrt[ref_idx(r)];
+    const Branch &C = P.branch[ref_idx(S.child)];
+    const uint4 d0 = ((const uint4 *)C.hash)[0], d1 = ((const uint4 *)C.hash)[1];
+    const uint32_t cnt = (uint32_t)S.key_len - S.term;
+    const uint32_t payload = rlp_item_len(cnt / 2 + 1, 0) + 33;
+    uint32_t ehn;
+    const uint64_t ehdr = rlp_hdr(payload, 0xC0, 0xF7, &ehn);
+    const uint32_t eenc = ehn + payload;
+    if (C.hash_size == 32 && keccak_blocks(eenc) <= (uint32_t)NB) {
+      WordWriter w(row);
+      w.inline_bytes(ehdr, ehn);
+      w.rlp_compact(P.keys + S.key_off, S.key_start, cnt, S.term);
+      w.inline_bytes(0xa0, 1);
+      w.push(((uint64_t)d0.y << 32) | d0.x, 8);
+      w.push(((uint64_t)d0.w << 32) | d0.z, 8);
+      w.push(((uint64_t)d1.y << 32) | d1.x, 8);
+      w.push(((uint64_t)d1.w << 32) | d1.z, 8);
+      nb = keccak_blocks(eenc);
+      w.finish_keccak(17 * nb);
+      enc = eenc;
+      bad[lane] = 0;
+      fast = true;
+    }
+  }
+  // the wave's children, numbered node by node, dealt out round robin
+  for (uint32_t t = lane; t < total; t += 64) {
+    uint32_t lo = 0, hi = 63;  // the node: the last with nbase <= t (it has children)
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (nbase[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const uint32_t j = lo, info = ninfo[j];
+    const uint32_t o = t - nbase[j];  // the child's rank among the node's present children
+    uint32_t s = 0;                   // its slot: the o-th set bit of the mask
+    uint32_t mm = info & 0xffffu;
+    for (uint32_t q = 0; q < o; q++) mm &= mm - 1;
+    s = __ffs(mm) - 1;
+    const ref_t c = P.branch[ref_idx(nref[j])].child[s];
+    const bool sh = ref_type(c) == T_SHORT;
+    const uint8_t *h = sh ? P.shrt[ref_idx(c)].hash : P.branch[ref_idx(c)].hash;
+    const uint4 d0 = ((const uint4 *)h)[0], d1 = ((const uint4 *)h)[1];
+    const uint32_t hs = sh ? P.shrt[ref_idx(c)].hash_size : P.branch[ref_idx(c)].hash_size;
+    if (hs != 32) {
+      bad[j] = 1;  // an embedded child: the node's own lane encodes it generically
+      continue;
+    }
+    uint8_t *rb = (uint8_t *)(rows + j * RS) + (info >> 16) + 33 * o + (s - o);
+    rb[0] = 0xa0;
+    const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+    for (int q = 0; q < 32; q++) rb[1 + q] = (uint8_t)(dw[q >> 2] >> (8 * (q & 3)));
+  }
+  __syncthreads();  // every row complete
+  if (!active) return;
+  const bool use_row = fast && bad[lane] == 0;
+  uint64_t *out = scratch + slot[i] / 8;
+  if (!use_row) {
+    encode_generic(P, r, out, &enc);
+    nb = enc >= 32 ? keccak_blocks(enc) : 0u;
+  }
+  enc_len[i] = enc;
+  if (enc < 32) return;  // the raw encoding is the node's "hash", stored by the encoder
+  uint64_t st[25];
+#pragma unroll
+  for (int q = 0; q < 25; q++) st[q] = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    if (use_row) {
+#pragma unroll
+      for (int q = 0; q < 17; q++) st[q] ^= row[17 * b + q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 17; q++) st[q] ^= out[17 * b + q];
+    }
+    keccak_f1600(st);
+  }
+  uint64_t *hp = (uint64_t *)node_hash(P, r);
+  hp[0] = st[0];
+  hp[1] = st[1];
+  hp[2] = st[2];
+  hp[3] = st[3];
+  node_done(P, r, 32);
+}
+
 __global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list, const uint64_t *slot,
                                                       const uint32_t *enc_len, uint32_t n,
                                                       const uint64_t *scratch) {
@@ -1317,7 +1489,7 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
-                         hipEvent_t perm_end, uint32_t n_ext) {
+                         hipEvent_t perm_end, uint32_t n_ext, const uint32_t *cls, const SideStream *side) {
   if (n == 0) return hipSuccess;
   if (!leaves && n <= spread_max) {  // small inner level: encode and hash in one launch
     if (perm_begin) {
@@ -1327,6 +1499,37 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     k_hc_small<<<(n + 3) / 4, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
     GMPT_LAUNCH_CHECK();
     if (perm_end) return hipEventRecord(perm_end, s);
+    return hipSuccess;
+  }
+  // a level of 1- and 2-block branches and extensions (all but a sixteenth
+  // of its nodes): one launch with the messages in LDS, the rest generic
+  // inside it; rows of one block when no node needs two
+  if (!leaves && cls && 16ull * ((uint64_t)cls[0] + cls[1] + cls[kExtClass]) >= 15ull * n) {
+    // the few nodes of 3 or more blocks (classes 2 .. 6, between the 2-block
+    // class and the extensions) would take the generic encoder and hold the
+    // launch open long after the bulk: with a second stream they are encoded
+    // by 16-lane groups and permuted there, concurrently with the bulk
+    uint32_t lo = 0, hi = 0;
+    const uint32_t multi = n - std::min(n, cls[0] + cls[1] + cls[kExtClass]);
+    if (side && multi) {
+      lo = std::min(n, cls[0] + cls[1]);
+      hi = lo + multi;
+      hipError_t e;
+      if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
+      k_hc_encode_inner<<<(multi + kInnerNodes - 1) / kInnerNodes, kB, 0, side->stream>>>(P, list + lo, slot + lo, multi,
+                                                                                         scratch, enc_len + lo);
+      GMPT_LAUNCH_CHECK();
+      k_hc_perm_lane<<<nblk(multi), kB, 0, side->stream>>>(P, list + lo, slot + lo, enc_len + lo, multi, scratch);
+      GMPT_LAUNCH_CHECK();
+      if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
+    }
+    if (cls[1])
+      k_hc_inner_wave<2><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
+    else
+      k_hc_inner_wave<1><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
+    GMPT_LAUNCH_CHECK();
+    if (side && multi) return hipStreamWaitEvent(s, side->join, 0);
     return hipSuccess;
   }
   // the level's extension class, sorted last: its own launch when it is large

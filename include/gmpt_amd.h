@@ -133,7 +133,13 @@ int gmpt_synchronize(gmpt_forest *f);
  * default 1): leaves whose key remainder has at most 64 nibbles and whose
  * value is under 16,383 bytes are encoded and hashed by one kernel, the
  * message built in registers, instead of an encode kernel, an HBM scratch
- * buffer and a permutation kernel.  No reference counterpart. */
+ * buffer and a permutation kernel.  "hc_fuse_inner" (0 or 1, default 1):
+ * inner levels of mostly 1-block branches and extensions are encoded and
+ * hashed in one kernel with the messages in LDS.  No reference counterpart.
+ * "pool_memory" (0 device, 1 host): 1 makes the trie state (node pools, key
+ * and value arenas, dirty list) grow in pinned host memory the kernels reach
+ * over the host link, so a forest can hold tries larger than the GPU's
+ * memory; replaces the unified-memory UMMPT (um_mpt.cuh:13-237). */
 int gmpt_set_option(gmpt_forest *f, const char *name, int64_t value);
 int gmpt_get_option(gmpt_forest *f, const char *name, int64_t *value);
 /* Autotuner: inserts `sample` (mode 0 PhaseNU, 1 LockNU) into a scratch forest

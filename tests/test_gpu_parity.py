@@ -348,6 +348,28 @@ def test_large_host_batch_through_pinned_staging(gmpt, workloads, oracle, mode):
     t.close()
 
 
+@pytest.mark.parametrize("fuse_inner", [0, 1], ids=["encode+permute", "lds-rows"])
+@pytest.mark.parametrize("mode", MODES)
+def test_inner_level_kernels_give_same_root(gmpt, workloads, oracle, fuse_inner, mode):
+    # every inner level above the spread threshold (0: all of them) by the
+    # LDS-row kernels or the 16-lane encoder + scratch + permutation; tiny
+    # values make embedded children and short nodes over them (the generic
+    # encoder inside the row kernels), prefix keys put values in slot 16
+    cases = [_rand_batch(workloads, 92, 30000, 20, 300), _rand_batch(workloads, 96, 4000, 0, 3)]
+    rng = np.random.default_rng(98)
+    ks = [bytes(rng.integers(0, 256, int(rng.integers(1, 6)), dtype=np.uint8)) for _ in range(3000)]
+    ks = list(dict.fromkeys(ks))
+    cases.append(workloads.batch_from_bytes(ks, [bytes(rng.integers(0, 256, 40, dtype=np.uint8)) for _ in ks]))
+    for b in cases:
+        t = gmpt.MPT(device=0)
+        t.set_option("hc_spread_max", 0)
+        t.set_option("hc_fuse_inner", fuse_inner)
+        assert t.get_option("hc_fuse_inner") == fuse_inner
+        _put(t, b, mode)
+        assert t.get_root_hash() == oracle.root_of(b)
+        t.close()
+
+
 def _leaf_case(workloads, case):
     if case == "c2":
         return _rand_batch(workloads, 93, 5000, 70, 110)
