@@ -278,6 +278,18 @@ def bench_batched(pkg, args):
 
     bt, broots = timed(batched_step, args.steps, args.warmup)
     st = fb.stats()
+    # the same pass as a replayed HIP graph (the first call captures)
+    fg = pkg.MPT(device=0, num_tries=ntries)
+    replays = []
+
+    def graph_step():
+        replays.append(fg.rebuild_graph(db))
+        return fg.get_root_hashes()
+
+    fg.rebuild_graph(db)  # capture
+    gt, groots = timed(graph_step, args.steps, args.warmup)
+    if groots != broots or not all(replays):
+        raise SystemExit("bench: graph replay roots differ from the eager pass, or a replay did not stand")
     pt, proots = timed(plain_step, max(1, args.steps // 5), 1)
     if broots != proots:
         bad = [i for i in range(ntries) if broots[i] != proots[i]][:5]
@@ -299,6 +311,8 @@ def bench_batched(pkg, args):
                    "tries": ntries, "keys": int(b.n)},
         "plain": {"tries_per_s": ntries / pt, "ms_per_pass": pt * 1e3,
                   "how": "one insert + hash + root call per trie on a reused single-trie forest"},
+        "graph": {"tries_per_s": ntries / gt, "ms_per_pass": gt * 1e3, "replays_stood": len(replays),
+                  "how": "the same multi-trie pass replayed as one captured HIP graph (gmpt_rebuild_graph)"},
         "speedup_vs_plain": pt / bt,
         "mputs_per_s": b.n / bt / 1e6,
         "per_step": {"permutations": st["perms"], "nodes_hashed": st["hashed_nodes"],
@@ -326,6 +340,7 @@ def bench_batched(pkg, args):
         out["cpu_baseline"] = None
     fb.close()
     fp.close()
+    fg.close()
     db.free()
     return out
 
@@ -568,6 +583,7 @@ def batched_summary(pkg, args):
     return {"workload": r["config"]["workload"], "tries": r["config"]["tries"], "keys": r["config"]["keys"],
             "tries_per_s": r["value"], "ms_per_pass": r["ms_per_step"], "mputs_per_s": r["mputs_per_s"],
             "plain_tries_per_s": r["plain"]["tries_per_s"], "plain_ms_per_pass": r["plain"]["ms_per_pass"],
+            "graph_tries_per_s": r["graph"]["tries_per_s"], "graph_ms_per_pass": r["graph"]["ms_per_pass"],
             "speedup_vs_plain": r["speedup_vs_plain"], "roots_equal_plain": True,
             "roots_checked_vs_oracle": r["roots_checked_vs_oracle"], "per_step": r["per_step"],
             "cpu_baseline": r["cpu_baseline"]}

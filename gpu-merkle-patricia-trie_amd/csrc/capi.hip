@@ -97,6 +97,17 @@ int gmpt_puts_2phase(gmpt_forest *f, const gmpt_batch *b, void **targets, int *n
   return f->f->puts_2phase(d, targets, n_targets);
 }
 
+int gmpt_rebuild_graph(gmpt_forest *f, const gmpt_batch *b, int *replayed) {
+  if (!f || !replayed) return fail("null argument");
+  BatchDesc d;
+  int r;
+  if ((r = to_desc(b, &d)) || (r = check_put_batch(b))) return r;
+  bool rep = false;
+  r = f->f->rebuild_graph(d, &rep);
+  *replayed = rep ? 1 : 0;
+  return r;
+}
+
 int gmpt_puts_latching(gmpt_forest *f, const gmpt_batch *b, void **targets, int *n_targets) {
   if (!f) return fail("null forest");
   BatchDesc d;

@@ -86,7 +86,7 @@ struct Counters {
   uint32_t flag_fallback;  // PhaseNU: a sort window tie between distinct keys
   uint32_t n_restarts;   // LockNU: optimistic restarts
   uint32_t n_lock_fail;  // LockNU: keys that restarted at least once
-  uint32_t pad;
+  uint32_t graph_bad;    // a replayed pass found its batch unlike the captured one: hash kernels skip
   unsigned long long n_perms;    // PhaseHC: Keccak-f permutations run
   unsigned long long enc_bytes;  // PhaseHC: bytes of node encodings produced
   unsigned long long leaf_perms; // PhaseHC: permutations of the leaf level

@@ -125,6 +125,13 @@ class Forest {
   // Times PhaseHC on `sample` (inserted into a scratch forest on the same
   // device) for each candidate spread threshold and keeps the fastest.
   int autotune(const BatchDesc &sample, int mode, int64_t *chosen, double *best_ms);
+  // Bulk rebuild of a device batch (values adopted) into the emptied forest
+  // -- clear, PhaseNU, PhaseHC -- as one replay of a captured HIP graph.  The
+  // first call for a batch (its buffers and sizes) runs eagerly and captures
+  // the pass; later calls replay it, and a replay whose batch turns out unlike
+  // the captured one (sort fallback, other hash levels) is redone eagerly and
+  // captured again.  *replayed tells which ran.
+  int rebuild_graph(const BatchDesc &b, bool *replayed);
   uint32_t ntries() const { return ntries_; }
   uint64_t pool_bytes_used() const;
   uint64_t pool_bytes_reserved() const;
@@ -139,6 +146,13 @@ class Forest {
   int grow(DevBuf &b, size_t need, size_t keep);
   int ensure_ws(DevBuf &b, size_t bytes);
   int arena_exhausted();
+  int hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves, bool captured, uint32_t *n_leaf_out);
+  // rebuild_graph's parts (graph.hip)
+  int rebuild_eager(const BatchDesc &b);
+  int capture_rebuild(const BatchDesc &b);
+  int replay_graph();
+  bool graph_ok();
+  void drop_graph();
   bool is_pool(const DevBuf &b) const;
   void release(DevBuf &b);
   int sync_counters();
@@ -181,6 +195,18 @@ class Forest {
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   hipEvent_t ev_meta_ = nullptr;               // the classification bins have reached the host
   SideStream side_{};                          // second stream for work beside a hash level's main kernel
+  // the captured rebuild pass and the batch it was captured for
+  struct GraphPlan {
+    bool valid = false;
+    BatchDesc batch;               // buffers and sizes the graph's launches use
+    uint32_t n_dirty = 0, ne = 0;  // dirty nodes and elements of the captured pass
+    uint64_t key_used = 0;         // key arena bytes after it
+    uint32_t hist[kHcBins];        // its hash bins
+  };
+  GraphPlan plan_;
+  DevBuf plan_hist_;                 // plan_.hist on the device, for the replay check
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
   OpStats stats_;
 };
 

@@ -112,6 +112,8 @@ Forest::~Forest() {
   (void)hipSetDevice(device_);
   if (stream_ || arena_) (void)hipStreamSynchronize(stream_);
   if (arena_) return;  // nothing of its own to release
+  drop_graph();
+  release(plan_hist_);
   delete stager_;
   DevBuf *bufs[] = {&branch_, &shrt_, &value_, &keys_, &tries_, &dirty_, &st_keys_, &st_kidx_, &st_vidx_,
                     &st_hps_, &st_tid_, &ws_, &ws2_, &hc_sorted_, &hc_meta_, &hc_cursor_, &hc_slot_, &hc_enc_, &hc_scratch_,
@@ -660,32 +662,8 @@ int Forest::hash(void *targets, int n_targets) {
       HIP_TRY(hipStreamSynchronize(stream_));
     const HcMeta &M = *(const HcMeta *)h_meta_;
     if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
-    uint64_t *scratch = (uint64_t *)hc_scratch_.p;
-    uint32_t n_direct = 0;  // the leaves of level 0, all hashed by the direct kernel
-    if (lmeta)
-      for (uint32_t c = 0; c < kHcClasses; c++) n_direct += M.hist[c];
-    uint32_t start = 0, n_leaf = 0;
-    for (uint32_t lv = 0; lv < kHcLevels; lv++) {
-      uint32_t cnt = 0;
-      for (uint32_t c = 0; c < kHcClasses; c++) cnt += M.hist[lv * kHcClasses + c];
-      if (cnt) {
-        const bool leaves = lv <= 1;  // level 1: leaves the direct kernel does not take
-        const uint32_t skip = lv == 0 ? n_direct : 0u;  // already queued
-        const bool timed = lv == 0 && !lmeta && ev2_;
-        const uint32_t n_ext = leaves ? 0u : M.hist[lv * kHcClasses + kHcClasses - 1];
-        if (cnt > skip)
-          HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
-                                encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
-                                timed ? ev3_ : nullptr, n_ext,
-                                hc_fuse_inner_ ? &M.hist[lv * kHcClasses] : nullptr,
-                                side_.stream ? &side_ : nullptr));
-        stats_.hash_launches++;
-        stats_.hashed_nodes += cnt;
-        if (!leaves && n_ext >= kExtDirectMin && cnt > hc_spread_max_) stats_.ext_direct_nodes += n_ext;
-        if (leaves) n_leaf += cnt;
-      }
-      start += cnt;
-    }
+    uint32_t n_leaf = 0;
+    if ((r = hash_levels(P, M.hist, lmeta != nullptr, false, &n_leaf))) return r;
     if (ev1_) HIP_TRY(hipEventRecord(ev1_, stream_));
     parts = (unsigned long long *)((uint8_t *)hc_meta_.p + sizeof(HcMeta));
     HIP_TRY(hc_count(encl, (const HcMeta *)hc_meta_.p, n, n_leaf, parts, stream_));
@@ -709,6 +687,43 @@ int Forest::hash(void *targets, int n_targets) {
       stats_.leaf_perm_ms = ms;
     }
   }
+  return 0;
+}
+
+// The hash levels after the leaves the direct kernel takes, from the bins
+// (hist, kHcBins counts), one range per non-empty level; captured = true for
+// a pass being captured into a graph (no timing events, no second stream).
+int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves, bool captured,
+                        uint32_t *n_leaf_out) {
+  ref_t *sorted = (ref_t *)hc_sorted_.p;
+  uint64_t *slot = (uint64_t *)hc_slot_.p;
+  uint32_t *encl = (uint32_t *)hc_enc_.p;
+  uint64_t *scratch = (uint64_t *)hc_scratch_.p;
+  uint32_t n_direct = 0;  // the leaves of level 0, all hashed by the direct kernel
+  if (direct_leaves)
+    for (uint32_t c = 0; c < kHcClasses; c++) n_direct += hist[c];
+  uint32_t start = 0, n_leaf = 0;
+  for (uint32_t lv = 0; lv < kHcLevels; lv++) {
+    uint32_t cnt = 0;
+    for (uint32_t c = 0; c < kHcClasses; c++) cnt += hist[lv * kHcClasses + c];
+    if (cnt) {
+      const bool leaves = lv <= 1;  // level 1: leaves the direct kernel does not take
+      const uint32_t skip = lv == 0 ? n_direct : 0u;  // already queued
+      const bool timed = lv == 0 && !direct_leaves && ev2_ && !captured;
+      const uint32_t n_ext = leaves ? 0u : hist[lv * kHcClasses + kHcClasses - 1];
+      if (cnt > skip)
+        HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
+                              encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
+                              timed ? ev3_ : nullptr, n_ext, hc_fuse_inner_ ? &hist[lv * kHcClasses] : nullptr,
+                              side_.stream && !captured ? &side_ : nullptr));
+      stats_.hash_launches++;
+      stats_.hashed_nodes += cnt;
+      if (!leaves && n_ext >= kExtDirectMin && cnt > hc_spread_max_) stats_.ext_direct_nodes += n_ext;
+      if (leaves) n_leaf += cnt;
+    }
+    start += cnt;
+  }
+  *n_leaf_out = n_leaf;
   return 0;
 }
 

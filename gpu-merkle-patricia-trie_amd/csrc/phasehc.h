@@ -77,6 +77,14 @@ hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64
 constexpr uint32_t kHcCountParts = 256;
 hipError_t hc_count(const uint32_t *enc_len, const HcMeta *meta, uint32_t n, uint32_t n_leaf,
                     unsigned long long *parts, hipStream_t s);
+// Index of the element count among an insert's device scalars (PhaseNU's SC_NE).
+constexpr uint32_t SC_NE_INDEX = 4;
+// Replay check of a captured insert + hash pass: marks the pass bad
+// (Counters::graph_bad, which the hash kernels test) unless the batch sorted
+// without fallback, raised no error and gave n_dirty dirty nodes, ne elements
+// (scal[SC_NE_INDEX]) and the bins plan_hist (kHcBins counts, device).
+hipError_t hc_graph_check(const Pools &P, const HcMeta *meta, const uint32_t *plan_hist, uint32_t n_dirty, uint32_t ne,
+                          const uint32_t *scal, hipStream_t s);
 // Zeroes the forest's counters and every trie header (empty tries).
 hipError_t hc_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries, hipStream_t s);
 // Copies every trie's root hash into its header and empties the dirty list;

@@ -612,6 +612,7 @@ __global__ __launch_bounds__(256) void k_hc_encode_leaf(Pools P, const ref_t *__
                                                         const uint64_t *__restrict__ slot, uint32_t n,
                                                         uint64_t *__restrict__ scratch,
                                                         uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __shared__ uint64_t head[kLeafNodes][kLeafHeadWords];
   const uint32_t g = threadIdx.x / kLeafGroup, s = threadIdx.x % kLeafGroup;
   const uint32_t i = blockIdx.x * kLeafNodes + g;
@@ -798,6 +799,7 @@ __global__ __launch_bounds__(256) void k_hc_encode_inner(Pools P, const ref_t *_
                                                          const uint64_t *__restrict__ slot, uint32_t n,
                                                          uint64_t *__restrict__ scratch,
                                                          uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __shared__ uint64_t lds[kInnerNodes][kInnerWords];
   const uint32_t g = threadIdx.x / kInnerGroup, s = threadIdx.x % kInnerGroup;
   const uint32_t i = blockIdx.x * kInnerNodes + g;
@@ -846,6 +848,7 @@ __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__re
                                                       const uint64_t *__restrict__ slot, uint32_t n,
                                                       uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len,
                                                       uint32_t skip_lo, uint32_t skip_hi) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   constexpr uint32_t RS = 17 * NB + 1;  // row stride in words (odd: lane-strided reads spread over the banks)
   __shared__ uint64_t rows[64 * RS];
   __shared__ ref_t nref[64];            // the lanes' nodes
@@ -995,6 +998,7 @@ __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__re
 __global__ __launch_bounds__(256) void k_hc_perm_lane(Pools P, const ref_t *list, const uint64_t *slot,
                                                       const uint32_t *enc_len, uint32_t n,
                                                       const uint64_t *scratch) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   uint32_t i = grid_tid();
   if (i >= n) return;
   uint32_t enc = enc_len[i];
@@ -1157,6 +1161,7 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
                                                         const uint64_t *__restrict__ lmeta,
                                                         const uint32_t *__restrict__ hist,
                                                         uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __shared__ uint64_t head[256][kLeafDirectHead];
   const uint32_t i = grid_tid();
   uint32_t n = 0;  // the leaves of level 0, sorted first
@@ -1174,6 +1179,7 @@ __global__ __launch_bounds__(256) void k_hc_ext_direct(Pools P, const ref_t *__r
                                                        const uint64_t *__restrict__ slot, uint32_t n,
                                                        uint64_t *__restrict__ scratch,
                                                        uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __shared__ uint64_t head[256][kLeafDirectHead];
   const uint32_t i = grid_tid();
   if (i >= n) return;  // no block barrier below
@@ -1285,6 +1291,7 @@ __device__ __forceinline__ bool spread_digest_lane(uint32_t lane) { return (lane
 __global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *list, const uint64_t *slot,
                                                         const uint32_t *enc_len, uint32_t n,
                                                         const uint64_t *scratch) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   const uint32_t lane = __lane_id();
   const uint32_t i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   uint32_t enc = i < n ? enc_len[i] : 0;
@@ -1305,6 +1312,7 @@ __global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *li
 __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restrict__ list,
                                                   const uint64_t *__restrict__ slot, uint32_t n,
                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __shared__ uint64_t lds[256 / 64][kInnerWords];
   const uint32_t lane = __lane_id(), g = threadIdx.x / 64;
   const uint32_t i = blockIdx.x * (256 / 64) + g;
@@ -1581,6 +1589,31 @@ __global__ __launch_bounds__(256) void k_clear(Counters *cnt, TrieHdr *tries, ui
     ((uint32_t *)tries)[q] = 0;
 }
 }  // namespace
+
+namespace {
+// A replayed pass is valid when the batch sorted without fallback, raised no
+// error, and produced the captured pass's dirty-list length, element count
+// and hash bins; otherwise it marks the pass bad (the hash kernels, which
+// launch with the captured level ranges, then skip).
+__global__ __launch_bounds__(256) void k_graph_check(Pools P, const uint32_t *__restrict__ hist,
+                                                     const uint32_t *__restrict__ plan, uint32_t nbins,
+                                                     uint32_t n_dirty, uint32_t ne, const uint32_t *scal) {
+  __shared__ uint32_t bad;
+  if (threadIdx.x == 0)
+    bad = P.cnt->err != 0 || P.cnt->flag_fallback != 0 || P.cnt->n_dirty != n_dirty || scal[SC_NE_INDEX] != ne;
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (hist[b] != plan[b]) bad = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && bad) P.cnt->graph_bad = 1;
+}
+}  // namespace
+
+hipError_t hc_graph_check(const Pools &P, const HcMeta *meta, const uint32_t *plan_hist, uint32_t n_dirty, uint32_t ne,
+                          const uint32_t *scal, hipStream_t s) {
+  k_graph_check<<<1, kB, 0, s>>>(P, meta->hist, plan_hist, kHcBins, n_dirty, ne, scal);
+  return hipGetLastError();
+}
 
 hipError_t hc_clear(Counters *cnt, TrieHdr *tries, uint32_t ntries, hipStream_t s) {
   size_t words = std::max<size_t>(sizeof(Counters) / 4, (size_t)ntries * sizeof(TrieHdr) / 4);

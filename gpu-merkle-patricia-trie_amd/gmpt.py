@@ -48,7 +48,7 @@ GMPT_SYMBOLS = ["gmpt_forest_create", "gmpt_forest_destroy", "gmpt_last_error", 
                 "gmpt_set_option", "gmpt_get_option", "gmpt_autotune", "gmpt_get_stats",
                 "gmpt_device_count", "gmpt_device_malloc", "gmpt_device_free", "gmpt_memcpy_htod",
                 "gmpt_memcpy_dtoh", "gmpt_device_copy", "gmpt_timer_start", "gmpt_timer_stop", "gmpt_keccak256_batch",
-                "gmpt_keccak256_batch_mode", "gmpt_build_root_noalloc", "gmpt_server_create", "gmpt_server_destroy", "gmpt_server_submit",
+                "gmpt_keccak256_batch_mode", "gmpt_build_root_noalloc", "gmpt_rebuild_graph", "gmpt_server_create", "gmpt_server_destroy", "gmpt_server_submit",
                 "gmpt_server_poll", "gmpt_server_wait", "gmpt_server_drain", "gmpt_server_get_stats",
                 "gmpt_server_latencies"]
 LIBGMPT_SYMBOLS = ["build_mpt_2phase", "build_mpt_olc", "preprocess", "get_all_nodes", "init_mpt",
@@ -113,6 +113,7 @@ def lib():
         "gmpt_keccak256_batch": ([vp, vp, vp, u32, vp], i32),
         "gmpt_keccak256_batch_mode": ([vp, vp, vp, u32, vp, i32], i32),
         "gmpt_build_root_noalloc": ([ctypes.POINTER(_Batch), i32, vp, ctypes.POINTER(i32)], i32),
+        "gmpt_rebuild_graph": ([vp, ctypes.POINTER(_Batch), ctypes.POINTER(i32)], i32),
         "gmpt_server_create": ([i32, u32, u32, ctypes.c_uint64, u32, i32, ctypes.POINTER(vp)], i32),
         "gmpt_server_destroy": ([vp], None),
         "gmpt_server_submit": ([vp, u32, ctypes.POINTER(_Batch), ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -320,6 +321,19 @@ class MPT:
         self._adopt(batch)
         reads = [(int(hps[i]), int(sizes[i])) for i in range(rn.value)]
         return t, nt.value, reads
+
+    def rebuild_graph(self, batch):
+        """Empties the forest and bulk-builds `batch` (a DeviceBatch made with
+        adopt_values=True) as one replay of a captured HIP graph; the first
+        call for a batch's buffers captures.  Returns True when a replay stood
+        (False: the pass ran eagerly, and was captured for the next call)."""
+        if not isinstance(batch, DeviceBatch) or not batch.struct.adopt_values:
+            raise GmptError("rebuild_graph takes a DeviceBatch with adopt_values=True")
+        rep = ctypes.c_int()
+        self._release_adopted()  # the clear drops the trie's references
+        _check(lib().gmpt_rebuild_graph(self._f, ctypes.byref(batch.struct), ctypes.byref(rep)))
+        self._adopt(batch)
+        return bool(rep.value)
 
     # --- hashing (gpu_mpt.cuh:131) --------------------------------------------
     def hash_onepass_v2(self, targets, n):

@@ -147,6 +147,20 @@ int gmpt_get_option(gmpt_forest *f, const char *name, int64_t *value);
  * the fastest on f.  No reference counterpart. */
 int gmpt_autotune(gmpt_forest *f, const gmpt_batch *sample, int mode, int64_t *chosen, double *best_ms);
 int gmpt_get_stats(gmpt_forest *f, gmpt_stats *out);
+/* Bulk rebuild of device batch b (on_device and adopt_values set; trie_ids
+ * for a multi-trie forest) into the forest after emptying it: clear, PhaseNU
+ * insert, PhaseHC hash, as one replay of a captured HIP graph.  The first call
+ * for a batch -- its device buffers and sizes -- runs the pass eagerly,
+ * captures it and replays it once; later calls with the same buffers (new
+ * contents of the same shape) replay it with one host synchronisation
+ * instead of four.  The device checks each replay against the captured pass
+ * (no fallback sort, the same dirty-node count and hash levels); a batch that
+ * differs is redone eagerly and captured again, so results are always those
+ * of the plain calls.  *replayed = 1 when the replay stood.  Roots are then
+ * read with gmpt_get_root_hash(es).  Replaces the batched form's sequence of
+ * MPT calls per trie (gpu_mpt.cuh:1262-1399, 1162-1191); no reference
+ * counterpart for the graph itself. */
+int gmpt_rebuild_graph(gmpt_forest *f, const gmpt_batch *b, int *replayed);
 
 /* Device helpers so callers can keep batches resident in HBM. */
 int gmpt_device_count(int *count);

@@ -739,6 +739,46 @@ def test_batched_4096_uneven_tries(gmpt, workloads, oracle, mode):
     t.close()
     p.close()
 
+def test_graph_rebuild_replays_and_checks_its_batch(gmpt, workloads, oracle):
+    """The batched pass as a replayed HIP graph (gmpt_rebuild_graph): the
+    first call captures, later calls with the same buffers replay; roots equal
+    the eager multi-trie pass's and the oracle's.  New contents of the same
+    shape (other keys: other hash levels) fail the device's replay check and
+    are redone eagerly with the right roots; the same contents again replay."""
+    sz, big, allb, per = _batched_shape(workloads, 64)
+    db = gmpt.DeviceBatch(allb, adopt_values=True)
+    t = gmpt.MPT(device=0, num_tries=len(sz))
+    e = gmpt.MPT(device=0, num_tries=len(sz))
+    _put(e, allb, "2phase")
+    want = e.get_root_hashes()
+    assert t.rebuild_graph(db) is False  # captured (and replayed once)
+    assert t.get_root_hashes() == want
+    for _ in range(2):
+        assert t.rebuild_graph(db) is True
+        assert t.get_root_hashes() == want
+    rng = np.random.default_rng(8)
+    for i in sorted(set(rng.choice(len(sz), 6, replace=False).tolist()) | {big}):
+        assert want[i] == oracle.root_of(per[i]), i
+    # other keys and values in the same buffers (same lengths, index pairs and tries)
+    keys2 = workloads.unique_random_keys(rng, allb.n, 32)
+    vals2 = rng.integers(0, 256, size=(allb.n, 32), dtype=np.uint8)
+    b2 = workloads.batch_from_fixed(keys2, vals2.reshape(-1), np.full(allb.n, 32))
+    b2.trie_ids = allb.trie_ids
+    assert np.array_equal(b2.keys_indexs, allb.keys_indexs) and np.array_equal(b2.values_indexs, allb.values_indexs)
+    L = gmpt.gmpt.lib()
+    for buf, arr in ((db.bufs[0], b2.keys_hexs), (db.bufs[2], b2.values_bytes)):
+        assert L.gmpt_memcpy_htod(buf.ptr, arr.ctypes.data, arr.nbytes) == 0
+    e.clear()
+    _put(e, b2, "2phase")
+    want2 = e.get_root_hashes()
+    assert t.rebuild_graph(db) is False  # the replay check found other hash levels
+    assert t.get_root_hashes() == want2
+    assert t.rebuild_graph(db) is True
+    assert t.get_root_hashes() == want2
+    t.close()
+    e.close()
+    db.free()
+
 
 def test_latching_overwrites_allocate_nothing(gmpt, workloads, oracle):
     """LockNU update-only batches rewrite the keys' value nodes in place and
