@@ -486,8 +486,10 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
 
 // Sorts the packed keys (by trie, then nibble path) and keeps the last
 // occurrence of each key: u_idx[0 .. scal[SC_M_UNIQ]) in sorted order.  pass 0
-// sorts by the windows' leading bits with a fix-up of equal runs, pass 1 by
-// every window bit that varies across the batch, pass 2 by the whole keys.
+// sorts by the whole windows (top-8-bit buckets sorted in LDS; above
+// kBucketSortMaxN keys, by the windows' leading bits with a fix-up of equal
+// runs), pass 1 by every window bit that varies across the batch, pass 2 by
+// the whole keys.
 // Passes 0 and 2 need no host synchronisation: the caller reads
 // flag_fallback at its next one and, if a flag is set, runs next_pass() and
 // repeats the work that followed.

@@ -1119,9 +1119,11 @@ __device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, ui
   const int32_t t0 = (int32_t)u - (int32_t)vstart;
   const uint32_t sh = (uint32_t)t0 & 7u;
   const int32_t jbase = (t0 - (int32_t)sh) >> 3;
+  // the head lies in the first block, on bytes no value byte or padding
+  // shares: the state starts as it
   uint64_t st[25];
 #pragma unroll
-  for (int q = 0; q < 25; q++) st[q] = 0;
+  for (int q = 0; q < 25; q++) st[q] = q < (int)kLeafHeadWords ? hw[q] : 0ull;
 #pragma unroll 1
   for (uint32_t b = 0; b < nb; b++) {
     {
@@ -1140,7 +1142,6 @@ __device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, ui
         const uint64_t f = funnel_bytes(W[q], W[q + 1], sh);
         const uint64_t mk = hasv && hi_b > lo_b ? (~0ull << (8 * lo_b)) & (~0ull >> (64 - 8 * hi_b)) : 0ull;
         uint64_t v = f & mk;
-        if (q < (int)kLeafHeadWords && b == 0) v |= hw[q];  // the head lies in the first block
         if ((int32_t)(enc >> 3) == w) v |= 0x01ull << (8 * (enc & 7));  // Keccak padding
         if (q == 16 && b + 1 == nb) v |= 0x80ull << 56;
         st[q] ^= v;

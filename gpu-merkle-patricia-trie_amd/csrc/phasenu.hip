@@ -1423,7 +1423,13 @@ hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStrea
   uint32_t *v[2] = {a.sv0, a.sv1};
   int cur = cur_io ? *cur_io : 0, which = 0;
   hipError_t e;
-  if (!full && !win_all_bits) {
+  if (!full && !win_all_bits && a.n <= kBucketSortMaxN) {
+    // whole windows (ties by input position) in one pass plus a sort of each
+    // top-8-bit bucket in LDS; a bucket too large for LDS asks for pass 1
+    e = bucket_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, &which, temp, s, &a.P.cnt->flag_fallback);
+    if (e != hipSuccess) return e;
+    cur ^= which;
+  } else if (!full && !win_all_bits) {
     // top bits, then each run of equal top bits by the whole window
     e = radix_sort_u64(k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], a.n, 64 - kTopBits, 64, &which, temp, s);
     if (e != hipSuccess) return e;
@@ -1554,7 +1560,9 @@ hipError_t nu_emit(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
 }
 
 hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
-  uint32_t ne_max = 2 * a.n + 2;
+  // elements: the unique keys, plus one virtual element per split short node
+  // (none when the tries were empty)
+  uint32_t ne_max = empty ? a.n + 1 : 2 * a.n + 2;
   size_t cnt = ne_max;  // bound on the entries of each level
   for (int k = 1; k < a.st_levels; k++) {
     cnt = (cnt + 15) / 16;

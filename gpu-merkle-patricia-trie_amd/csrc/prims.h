@@ -25,6 +25,20 @@ hipError_t radix_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1
 hipError_t radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int begin_bit,
                           int end_bit, int *which, void *temp, hipStream_t s);
 
+// Sort of (keys, vals) by (key, val) for n <= kBucketSortMaxN: one stable
+// pass on the key's top 8 bits, then each of the 256 buckets sorted in LDS by
+// one workgroup (a counting sort on the next 12 bits, then an insertion sort
+// of each run of equal 20-bit prefixes by (key, val), so equal keys end in
+// value order).  A bucket above kBucketCap entries, or with a run above 64,
+// is left in the first pass's stable order and sets bit 1 of *overflow: the
+// caller then sorts another way, stably (keys of uniform spread fill buckets
+// of ~n/256 and runs of ~1).
+// n <= kBucketCap skips the first pass.  Buffers as radix_sort_u64.
+constexpr size_t kBucketCap = 8192;
+constexpr size_t kBucketSortMaxN = 1600000;  // 6,250 per bucket on average
+hipError_t bucket_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int *which, void *temp,
+                           hipStream_t s, uint32_t *overflow);
+
 // ---------------------------------------------------------------- device --
 
 // Inclusive scan of one value per lane over a wave64.

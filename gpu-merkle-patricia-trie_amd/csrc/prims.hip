@@ -252,7 +252,146 @@ hipError_t radix_impl(K *k0, uint32_t *v0, K *k1, uint32_t *v1, size_t n, int be
   return hipSuccess;
 }
 
+// One bucket of the top-8-bit pass per workgroup: bucket d spans
+// [goff[d * nblocks], goff[(d + 1) * nblocks]) of the pass's output (the
+// exclusive scan of the digit-major per-tile counts), or the whole array
+// without goff.  The bucket is counting-sorted in LDS by the next 12 key bits
+// (4,096 sub-buckets: about one entry each for a bucket of uniform keys),
+// then each sub-bucket's run is insertion-sorted by (key, value), which also
+// fixes the order the LDS atomics placed equal sub-buckets in.  A bucket
+// with a run longer than kSubMaxRun is left as it came and sets bit 1 of
+// *overflow.
+constexpr int kBkThreads = 1024;
+constexpr uint32_t kBkPer = (uint32_t)(kBucketCap / kBkThreads);  // entries per thread
+constexpr uint32_t kSubs = 4096;
+constexpr uint32_t kSubPer = kSubs / kBkThreads;                    // sub-buckets per thread
+constexpr uint32_t kSubMaxRun = 64;
+static_assert(kBucketCap % kBkThreads == 0 && kSubs % kBkThreads == 0, "bucket tiling");
+
+__global__ __launch_bounds__(1024) void k_bucket_sort(uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                      uint32_t n, const uint32_t *__restrict__ goff,
+                                                      uint32_t nblocks, uint32_t *overflow) {
+  __shared__ uint64_t sk[kBucketCap];
+  __shared__ uint32_t sv[kBucketCap];
+  __shared__ uint32_t start[kSubs];
+  __shared__ uint32_t wsum[kBkThreads / 64];
+  const uint32_t d = blockIdx.x;
+  const uint32_t lo = goff ? goff[(size_t)d * nblocks] : 0u;
+  const uint32_t hi = goff && d + 1 < 256 ? goff[(size_t)(d + 1) * nblocks] : n;
+  const uint32_t c = hi - lo;  // uniform over the block
+  if (c <= 1) return;
+  if (c > kBucketCap) {
+    if (threadIdx.x == 0) atomicOr(overflow, 2u);
+    return;
+  }
+  // the sub-bucket bits: below the bucket's 8, or the top 12 of a single bucket
+  const int sub_shift = goff ? 44 : 52;
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (uint32_t j = 0; j < kSubPer; j++) start[t + j * kBkThreads] = 0;
+  __syncthreads();
+  uint64_t k[kBkPer];
+  uint32_t v[kBkPer], sub[kBkPer], pos[kBkPer];
+#pragma unroll
+  for (uint32_t q = 0; q < kBkPer; q++) {
+    const uint32_t e = t + q * kBkThreads;
+    if (e < c) {
+      k[q] = keys[lo + e];
+      v[q] = vals[lo + e];
+      sub[q] = (uint32_t)(k[q] >> sub_shift) & (kSubs - 1);
+      pos[q] = atomicAdd(&start[sub[q]], 1u);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the counts; thread t owns sub-buckets kSubPer t ..
+  uint32_t cnt[kSubPer], sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kSubPer; j++) {
+    cnt[j] = start[kSubPer * t + j];
+    sum += cnt[j];
+  }
+  const uint32_t incl = wave_inclusive_scan(sum);
+  if (__lane_id() == 63) wsum[t >> 6] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t w = 0; w < (t >> 6); w++) base += wsum[w];
+  uint32_t first[kSubPer];
+  bool long_run = false;
+  {
+    uint32_t x = base + incl - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < kSubPer; j++) {
+      first[j] = x;
+      start[kSubPer * t + j] = x;  // only this thread reads or writes these entries here
+      x += cnt[j];
+      long_run |= cnt[j] > kSubMaxRun;
+    }
+  }
+  // a run too long to insertion-sort: the bucket stays as it came (in the
+  // previous pass's stable order, which the caller's other sort relies on)
+  if (__syncthreads_or(long_run)) {
+    if (t == 0) atomicOr(overflow, 2u);
+    return;
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < kBkPer; q++) {
+    const uint32_t e = t + q * kBkThreads;
+    if (e < c) {
+      const uint32_t p = start[sub[q]] + pos[q];
+      sk[p] = k[q];
+      sv[p] = v[q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kSubPer; j++) {
+    if (cnt[j] < 2) continue;
+    const uint32_t f = first[j];
+    for (uint32_t q = 1; q < cnt[j]; q++) {
+      const uint64_t kk = sk[f + q];
+      const uint32_t vv = sv[f + q];
+      uint32_t p = q;
+      while (p > 0 && (sk[f + p - 1] > kk || (sk[f + p - 1] == kk && sv[f + p - 1] > vv))) {
+        sk[f + p] = sk[f + p - 1];
+        sv[f + p] = sv[f + p - 1];
+        p--;
+      }
+      sk[f + p] = kk;
+      sv[f + p] = vv;
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = t; e < c; e += kBkThreads) {
+    keys[lo + e] = sk[e];
+    vals[lo + e] = sv[e];
+  }
+}
+
 }  // namespace
+
+hipError_t bucket_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int *which, void *temp,
+                           hipStream_t s, uint32_t *overflow) {
+  *which = 0;
+  if (n <= 1) return hipSuccess;
+  if (n > kBucketSortMaxN) return hipErrorInvalidValue;
+  if (n <= kBucketCap) {
+    k_bucket_sort<<<1, kBkThreads, 0, s>>>(k0, v0, (uint32_t)n, nullptr, 0, overflow);
+    return hipGetLastError();
+  }
+  const uint32_t nb = (uint32_t)((n + kRsTile - 1) / kRsTile);
+  uint32_t *hist = (uint32_t *)temp;
+  uint32_t *scan_tmp = hist + (size_t)256 * nb + 16;
+  k_rs_hist<uint64_t><<<nb, kRsBlock, 0, s>>>(k0, n, 56, hist, nb);
+  GMPT_LAUNCH_CHECK();
+  hipError_t e = scan_impl<uint32_t>(hist, hist, (size_t)256 * nb, nullptr, scan_tmp, s);
+  if (e != hipSuccess) return e;
+  k_rs_scatter<uint64_t><<<nb, kRsBlock, 0, s>>>(k0, v0, k1, v1, n, 56, hist, nb);
+  GMPT_LAUNCH_CHECK();
+  k_bucket_sort<<<256, kBkThreads, 0, s>>>(k1, v1, (uint32_t)n, hist, nb, overflow);
+  GMPT_LAUNCH_CHECK();
+  *which = 1;
+  return hipSuccess;
+}
 
 size_t scan_temp_bytes(size_t n) { return scan_temp_elems<uint64_t>(n) * sizeof(uint64_t) + 64; }
 

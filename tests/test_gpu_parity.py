@@ -739,6 +739,31 @@ def test_batched_4096_uneven_tries(gmpt, workloads, oracle, mode):
     t.close()
     p.close()
 
+
+def test_batched_4096_uneven_tries_through_graph(gmpt, workloads, oracle):
+    """The same 4,096-trie batch through the replayed-graph pass
+    (gmpt_rebuild_graph): the capturing call and two replays give the eager
+    multi-trie pass's roots, and a sample including the largest and a 1-key
+    trie equals the oracle's."""
+    sz, big, allb, per = _batched_shape(workloads, 16)
+    e = gmpt.MPT(device=0, num_tries=len(sz))
+    _put(e, allb, "2phase")
+    want = e.get_root_hashes()
+    e.close()
+    db = gmpt.DeviceBatch(allb, adopt_values=True)
+    t = gmpt.MPT(device=0, num_tries=len(sz))
+    assert t.rebuild_graph(db) is False
+    assert t.get_root_hashes() == want
+    for _ in range(2):
+        assert t.rebuild_graph(db) is True
+        assert t.get_root_hashes() == want
+    rng = np.random.default_rng(5)
+    for i in sorted(set(rng.choice(len(sz), 8, replace=False).tolist()) | {big, int(np.argmin(sz))}):
+        assert want[i] == oracle.root_of(per[i]), i
+    t.close()
+    db.free()
+
+
 def test_graph_rebuild_replays_and_checks_its_batch(gmpt, workloads, oracle):
     """The batched pass as a replayed HIP graph (gmpt_rebuild_graph): the
     first call captures, later calls with the same buffers replay; roots equal
