@@ -574,6 +574,13 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
       for (int pass = 0; pass >= 0;) {
         if ((r = sort_unique(a, trie_bits, temp, pass, &cur))) break;
         HIP_TRY(nu_descend_and_group(a, temp, stream_, empty));
+        if (empty) {
+          // into empty tries no group needs merging: the build follows at
+          // once (its kernels skip themselves while the sort asks for another
+          // pass or an error is pending), one synchronisation per pass
+          HIP_TRY(nu_emit(a, temp, stream_, true));
+          HIP_TRY(nu_build(a, temp, stream_, true));
+        }
         if ((r = sync_counters())) break;
         pass = next_pass(pass);
         if (pass >= 0) {
@@ -586,15 +593,17 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
       if (r == 0) break;
       if (run > 0 || !grow_key_arena_after_error()) return r;
     }
-    if (cnt_.scal[SC_MERGE]) {
-      // a split group straddles a subtree: make every group contiguous
-      uint32_t m = cnt_.scal[SC_M_UNIQ];
-      HIP_TRY(nu_regroup(a, m, m, temp, stream_));
-      stats_.regroups++;
+    if (!empty) {
+      if (cnt_.scal[SC_MERGE]) {
+        // a split group straddles a subtree: make every group contiguous
+        uint32_t m = cnt_.scal[SC_M_UNIQ];
+        HIP_TRY(nu_regroup(a, m, m, temp, stream_));
+        stats_.regroups++;
+      }
+      HIP_TRY(nu_emit(a, temp, stream_, false));
+      HIP_TRY(nu_build(a, temp, stream_, false));
+      if ((r = sync_counters())) return r;
     }
-    HIP_TRY(nu_emit(a, temp, stream_, empty));
-    HIP_TRY(nu_build(a, temp, stream_, empty));
-    if ((r = sync_counters())) return r;
     // the batch's packed keys stay only if a new node refers to them: a batch
     // of updates alone (no elements) leaves them unused, so the arena keeps
     // its place (keys of repeated updates do not accumulate)

@@ -45,13 +45,18 @@ __device__ __forceinline__ void leaf_meta_words(uint64_t key_off, uint32_t key_s
 }
 
 constexpr uint32_t kLeafPackedNone = 0xffffffffu;  // no recorded bin: derive it from the node
-constexpr uint32_t kPackedNbMax = (1u << 20) - 1;   // blocks field of a packed bin, saturated
+constexpr uint32_t kPackedNbMax = (1u << 19) - 1;   // blocks field of a packed bin, saturated
+constexpr uint32_t kLeafFastNibbles = 64;           // key nibbles a leaf head holds in registers
+constexpr uint32_t kLeafDirectMaxValue = 16382;     // value bytes the metadata holds (14 bits, saturated above)
 
-// Packed bin (bin | blocks << 12) of a leaf: level 0, class by blocks.
+// Packed bin (bin | blocks << 12) of a leaf: class by blocks, level 0 when
+// the leaf-level word builder takes it (key remainder <= 64 nibbles, value
+// <= 16,382 bytes), else level 1 (the generic leaf encoder).
 __device__ __forceinline__ uint32_t leaf_packed(uint32_t cnt, uint32_t vsz) {
   const uint32_t nb = leaf_blocks(cnt, vsz);
   const uint32_t cls = nb > kHcClasses ? kHcClasses - 1 : nb - 1;
-  return cls | (min(nb, kPackedNbMax) << 12);
+  const uint32_t level = cnt > kLeafFastNibbles || vsz > kLeafDirectMaxValue ? 1u : 0u;
+  return (level * kHcClasses + cls) | (min(nb, kPackedNbMax) << 12);
 }
 
 }  // namespace gmpt

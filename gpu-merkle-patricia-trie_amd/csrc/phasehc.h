@@ -16,14 +16,16 @@ constexpr uint32_t kExtDirectMin = 16384;
 // Classification results, read back by the host in one copy.
 struct HcMeta {
   uint32_t hist[kHcBins];   // nodes per (level, class) bin
+  uint32_t slot_lo;         // first sorted node with a scratch slot (past the level-0 leaves of hist[0 .. 8)
+                            // when they are hashed without one); slot_lo, nvalid is the slot scan's range
   uint32_t nvalid;          // binned nodes (the dirty list minus null holes)
-  uint32_t pad;
   uint64_t scratch_bytes;   // sum of the nodes' scratch slot sizes
 };
 
 size_t hc_temp_bytes(uint32_t n);
 // Bins dirty nodes by (level, block class): out = the list ordered by bin,
-// slot[i] = byte offset of sorted node i's message slot in the scratch buffer;
+// slot[i] = byte offset of sorted node i's message slot in the scratch buffer
+// (for slot_lo <= i < nvalid; other entries are not written);
 // packed (n entries) is workspace.  With lmeta (2n words, lmeta_tmp 2n words
 // of workspace) the leaves whose key remainder has at most 64 nibbles and
 // whose value is under 16,383 bytes -- level 0, the first hist[0 .. 8)

@@ -94,7 +94,6 @@ __device__ uint32_t estimate_blocks(const Pools &P, ref_t r, bool *leaf) {
   return keccak_blocks(len);
 }
 
-constexpr uint32_t kLeafFastNibbles = 64;  // key nibbles a leaf head holds in registers
 // Inner nodes have at most 5 blocks, so an inner level's last class is free
 // for the extensions k_hc_ext_direct hashes: a short node over a branch with
 // a key of 1 .. 64 nibbles.
@@ -132,7 +131,6 @@ __device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) 
 // node uses (a node at nibble depth 255 cannot exist), and take the encode +
 // scratch + permutation kernels.
 constexpr uint32_t kLdsLeafBins = kHcClasses;  // the bins of level 0
-constexpr uint32_t kLeafDirectMaxValue = 16382; // value bytes the metadata holds (14 bits, saturated above)
 
 __global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
   uint32_t i = grid_tid();
@@ -155,6 +153,10 @@ __device__ __forceinline__ bool leaf_too_long(uint64_t m0) {
 // are issued together before any is used (the block count is kept low for
 // the global histogram, so each thread's loads must overlap).
 constexpr int kHcPer = (int)(kHcTile / 256);
+// `packed` entries: bin | blocks << 12 (saturated at kPackedNbMax), and
+// kPackedRecorded for a leaf the last build recorded: its metadata is read
+// from the build's records, not from lmeta_tmp.
+constexpr uint32_t kPackedRecorded = 1u << 31;
 
 __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restrict__ list, uint32_t n,
                                                  uint32_t *__restrict__ hist, uint32_t *__restrict__ packed,
@@ -164,20 +166,14 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
   const uint32_t i0 = blockIdx.x * kHcTile + threadIdx.x;
   ref_t r[kHcPer];
   uint32_t rpk[kHcPer];
-  uint4 rm[kHcPer];  // a recorded leaf's metadata
 #pragma unroll
   for (int q = 0; q < kHcPer; q++) {
     const uint32_t i = i0 + 256u * q;
     const uint32_t ri = i - rec.lo;
     r[q] = i < n ? list[i] : REF_NULL;  // builds leave null holes
-    // a leaf the last build created: its bin and metadata were recorded then
+    // a leaf the last build created: its bin (level 1 when the word builder
+    // cannot take it) and metadata were recorded then
     rpk[q] = i < n && rec.pk && ri < rec.n ? rec.pk[ri] : kLeafPackedNone;
-  }
-#pragma unroll
-  for (int q = 0; q < kHcPer; q++) {
-    const uint32_t ri = i0 + 256u * q - rec.lo;
-    rm[q] = lmeta_tmp && rpk[q] != kLeafPackedNone ? *(const uint4 *)(rec.meta + 2 * (size_t)ri)
-                                                   : make_uint4(0, 0, 0, 0);
   }
   __syncthreads();
 #pragma unroll
@@ -186,24 +182,23 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
     if (i >= n) break;
     uint32_t pk = 0xffffffffu;
     if (r[q] != REF_NULL) {
-      uint32_t nb, bin;
+      uint32_t nb, bin, flag = 0;
       if (rpk[q] != kLeafPackedNone) {
         bin = rpk[q] & 0xfff;
-        nb = rpk[q] >> 12;
+        nb = (rpk[q] >> 12) & kPackedNbMax;
+        flag = kPackedRecorded;
+        // without the word builder every leaf is hashed at level 0
+        if (!lmeta_tmp && bin >= kHcClasses) bin -= kHcClasses;
       } else {
         bin = bin_of(P, r[q], &nb);
-      }
-      if (lmeta_tmp && bin < kLdsLeafBins) {
-        uint64_t *m = lmeta_tmp + 2 * (size_t)i;
-        if (rpk[q] != kLeafPackedNone) {
-          *(uint4 *)m = rm[q];
-        } else {
+        if (lmeta_tmp && bin < kLdsLeafBins) {
+          uint64_t *m = lmeta_tmp + 2 * (size_t)i;
           leaf_meta(P, r[q], m);
+          if (leaf_too_long(m[0])) bin += kHcClasses;
         }
-        if (leaf_too_long(m[0])) bin += kHcClasses;
       }
       atomicAdd(&h[bin], 1u);
-      pk = bin | (min(nb, kPackedNbMax) << 12);
+      pk = bin | (min(nb, kPackedNbMax) << 12) | flag;
     }
     packed[i] = pk;
   }
@@ -219,9 +214,16 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *__rest
                                                     uint32_t *__restrict__ cursor, ref_t *__restrict__ out,
                                                     uint64_t *__restrict__ slot,
                                                     const uint64_t *__restrict__ lmeta_tmp,
-                                                    uint64_t *__restrict__ lmeta) {
+                                                    uint64_t *__restrict__ lmeta, LeafRecords rec, HcMeta *meta) {
   __shared__ uint32_t cnt[kHcBins];
   __shared__ uint32_t base[kHcBins];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the slot scan's range starts past the level-0 leaves hashed without a slot
+    uint32_t lo = 0;
+    if (lmeta)
+      for (uint32_t c = 0; c < kLdsLeafBins; c++) lo += meta->hist[c];
+    meta->slot_lo = lo;
+  }
   for (uint32_t b = threadIdx.x; b < kHcBins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
   constexpr int kPer = kHcTile / 256;
@@ -242,24 +244,18 @@ __global__ __launch_bounds__(256) void k_hc_scatter(Pools P, const ref_t *__rest
       uint32_t i = blockIdx.x * kHcTile + q * 256 + threadIdx.x;
       uint32_t pos = base[pk[q] & 0xfff] + rank[q];
       ref_t r = list[i];
-      uint32_t nb = pk[q] >> 12;
+      uint32_t nb = (pk[q] >> 12) & kPackedNbMax;
       if (nb == kPackedNbMax) bin_of(P, r, &nb);  // saturated: recompute exactly
       out[pos] = r;
       if (lmeta && (pk[q] & 0xfff) < kLdsLeafBins) {  // hashed by k_hc_leaf_direct: no scratch slot
-        lmeta[2 * (size_t)pos] = lmeta_tmp[2 * (size_t)i];
-        lmeta[2 * (size_t)pos + 1] = lmeta_tmp[2 * (size_t)i + 1];
+        const uint64_t *m = (pk[q] & kPackedRecorded) ? rec.meta + 2 * (size_t)(i - rec.lo) : lmeta_tmp + 2 * (size_t)i;
+        *(uint4 *)(lmeta + 2 * (size_t)pos) = *(const uint4 *)m;
         nb = 0;
       }
       slot[pos] = 136ull * nb;
     }
 }
 
-// Slot sizes of the positions past the binned nodes (null holes of the dirty
-// list) are zero, so one scan over the whole list gives every slot's offset.
-__global__ __launch_bounds__(256) void k_hc_slot_tail(const uint32_t *nvalid, uint32_t n, uint64_t *slot) {
-  uint32_t i = grid_tid();
-  if (i < n && i >= *nvalid) slot[i] = 0;
-}
 
 // ------------------------------------------------------------------ encode --
 
@@ -1481,7 +1477,7 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
   k_hc_zero<<<1, kB, 0, s>>>((uint32_t *)&P.cnt->n_perms, 8);
   GMPT_LAUNCH_CHECK();
   if (n == 0) {
-    k_hc_zero<<<1, kB, 0, s>>>(&meta->nvalid, 4);  // nvalid, pad, scratch_bytes
+    k_hc_zero<<<1, kB, 0, s>>>(&meta->slot_lo, 4);  // slot_lo, nvalid, scratch_bytes
     return hipGetLastError();
   }
   unsigned tiles = (unsigned)((n + kHcTile - 1) / kHcTile);
@@ -1489,11 +1485,10 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
   GMPT_LAUNCH_CHECK();
   hipError_t e = scan_exclusive_u32(meta->hist, cursor, kHcBins, &meta->nvalid, temp, s);
   if (e != hipSuccess) return e;
-  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, packed, n, cursor, out, slot, lmeta_tmp, lmeta);
+  k_hc_scatter<<<tiles, kB, 0, s>>>(P, list, packed, n, cursor, out, slot, lmeta_tmp, lmeta, rec, meta);
   GMPT_LAUNCH_CHECK();
-  k_hc_slot_tail<<<nblk(n), kB, 0, s>>>(&meta->nvalid, n, slot);
-  GMPT_LAUNCH_CHECK();
-  return scan_exclusive_u64(slot, slot, n, &meta->scratch_bytes, temp, s);
+  // slot offsets of the sorted nodes that have a slot: [slot_lo, nvalid)
+  return scan_exclusive_u64(slot, slot, n, &meta->scratch_bytes, temp, s, nullptr, &meta->slot_lo);
 }
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,

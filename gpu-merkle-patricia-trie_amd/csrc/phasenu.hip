@@ -993,9 +993,9 @@ __device__ __forceinline__ ST st_of(const NuArgs &a) {
 // bid[j] = 1 for representatives; an exclusive scan turns it into the
 // branch's rank.  Entries up to the launch bound are written (0 past nsp).
 __global__ __launch_bounds__(256) void k_split_reps(NuArgs a, uint32_t ne_max) {
-  // the sort needs a fallback pass: the build waits for it (the host only runs
-  // the build after checking, a replayed pass may not have)
-  if (a.P.cnt->flag_fallback) return;
+  // the sort needs a fallback pass, or an error is pending: the build waits
+  // for the host (which may run it before reading either)
+  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
   uint32_t j = grid_tid();
   if (j >= ne_max) return;
   // the build's dirty-list base: the list does not change until the build's
@@ -1042,9 +1042,9 @@ __device__ __forceinline__ void put_dirty(const NuArgs &a, uint32_t slot, ref_t 
 }
 
 __global__ __launch_bounds__(256) void k_init_branches(NuArgs a) {
-  // the sort needs a fallback pass: the build waits for it (the host only runs
-  // the build after checking, a replayed pass may not have)
-  if (a.P.cnt->flag_fallback) return;
+  // the sort needs a fallback pass, or an error is pending: the build waits
+  // for the host (which may run it before reading either)
+  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
   uint32_t j = grid_tid();
   ST s = st_of(a);
   if (j >= s.nsp) return;
@@ -1139,9 +1139,9 @@ __device__ __forceinline__ void install(const NuArgs &a, const GroupInfo &gi, re
 
 // Branches: link each to its parent (through an extension when depths skip).
 __global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
-  // the sort needs a fallback pass: the build waits for it (the host only runs
-  // the build after checking, a replayed pass may not have)
-  if (a.P.cnt->flag_fallback) return;
+  // the sort needs a fallback pass, or an error is pending: the build waits
+  // for the host (which may run it before reading either)
+  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
   uint32_t j = grid_tid();
   ST s = st_of(a);
   if (j >= s.nsp) return;
@@ -1203,9 +1203,9 @@ __device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32
 }
 
 __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
-  // the sort needs a fallback pass: the build waits for it (the host only runs
-  // the build after checking, a replayed pass may not have)
-  if (a.P.cnt->flag_fallback) return;
+  // the sort needs a fallback pass, or an error is pending: the build waits
+  // for the host (which may run it before reading either)
+  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
   uint32_t e = grid_tid();
   ST s = st_of(a);
   Slots q = slots_of(a);

@@ -13,9 +13,11 @@ size_t radix_temp_bytes(size_t n);
 // (device pointer).  `in` and `out` may alias.
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, void *temp,
                               hipStream_t s);
-// With skip (a device word), the scan does nothing when *skip != 0.
+// With skip (a device word), the scan does nothing when *skip != 0.  With
+// range (device words lo, hi), only entries [lo, hi) are summed, read and
+// written; out[i] for i in [lo, hi) is the sum of in[lo .. i).
 hipError_t scan_exclusive_u64(const uint64_t *in, uint64_t *out, size_t n, uint64_t *total, void *temp,
-                              hipStream_t s, const uint32_t *skip = nullptr);
+                              hipStream_t s, const uint32_t *skip = nullptr, const uint32_t *range = nullptr);
 
 // Stable LSD sort of (keys, vals) by key bits [begin_bit, end_bit), 8 bits per
 // pass.  Buffers ping-pong between (k0,v0) and (k1,v1); returns in *which the

@@ -18,17 +18,29 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 constexpr size_t kScanDirectTiles = 2048;  // up to this many tiles: two launches per scan
 
+// With range (a device pair lo, hi), entries outside [lo, hi) count as zero
+// and are neither read nor written (tiles outside the range touch no memory).
+struct ScanRange {
+  size_t lo, hi;
+  __device__ __forceinline__ ScanRange(const uint32_t *range, size_t n)
+      : lo(range ? range[0] : 0), hi(range ? (range[1] < n ? range[1] : n) : n) {}
+  __device__ __forceinline__ bool in(size_t j) const { return j >= lo && j < hi; }
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, size_t n, T *__restrict__ sums,
-                                                     const uint32_t *skip) {
+                                                     const uint32_t *skip, const uint32_t *range) {
   if (skip && *skip) return;  // uniform over the grid
   __shared__ T sh[4];
+  const ScanRange rg(range, n);
   size_t base = (size_t)blockIdx.x * kScanTile;
   T acc = 0;
+  if (base < rg.hi && base + kScanTile > rg.lo) {
 #pragma unroll
-  for (int i = 0; i < kScanItems; i++) {
-    size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
-    if (j < n) acc += in[j];
+    for (int i = 0; i < kScanItems; i++) {
+      size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
+      if (rg.in(j)) acc += in[j];
+    }
   }
   T tot;
   block_exclusive_scan_256<T>(acc, sh, &tot);
@@ -41,15 +53,18 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const T *__restrict__ in, s
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *__restrict__ out, size_t n,
                                                    const T *__restrict__ offsets, const T *__restrict__ tile_sums,
-                                                   T *__restrict__ total, const uint32_t *skip) {
+                                                   T *__restrict__ total, const uint32_t *skip,
+                                                   const uint32_t *range) {
   if (skip && *skip) return;  // uniform over the grid
   __shared__ T tile[kScanTile];
   __shared__ T sh[4];
+  const ScanRange rg(range, n);
   size_t base = (size_t)blockIdx.x * kScanTile;
+  const bool touch = base < rg.hi && base + kScanTile > rg.lo;  // uniform over the block
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
     size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
-    tile[i * kScanBlock + threadIdx.x] = j < n ? in[j] : T(0);
+    tile[i * kScanBlock + threadIdx.x] = touch && rg.in(j) ? in[j] : T(0);
   }
   T off = 0;
   if (tile_sums) {
@@ -79,7 +94,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
     size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
-    if (j < n) out[j] = tile[i * kScanBlock + threadIdx.x];
+    if (touch && rg.in(j)) out[j] = tile[i * kScanBlock + threadIdx.x];
   }
   if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = off + tot;
 }
@@ -96,27 +111,27 @@ size_t scan_temp_elems(size_t n) {
 
 template <typename T>
 hipError_t scan_impl(const T *in, T *out, size_t n, T *total, T *temp, hipStream_t s,
-                     const uint32_t *skip = nullptr) {
+                     const uint32_t *skip = nullptr, const uint32_t *range = nullptr) {
   if (n == 0) {
     if (total) return hipMemsetAsync(total, 0, sizeof(T), s);
     return hipSuccess;
   }
   size_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 1) {
-    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, nullptr, total, skip);
+    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, nullptr, total, skip, range);
     return hipGetLastError();
   }
   T *sums = temp;
   T *rest = temp + nb + 1;
-  k_scan_reduce<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, sums, skip);
+  k_scan_reduce<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, sums, skip, range);
   GMPT_LAUNCH_CHECK();
   if (nb <= kScanDirectTiles) {  // each tile adds up the sums before it
-    k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, nullptr, sums, total, skip);
+    k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, nullptr, sums, total, skip, range);
     return hipGetLastError();
   }
   hipError_t e = scan_impl<T>(sums, sums, nb, nullptr, rest, s, skip);
   if (e != hipSuccess) return e;
-  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, nullptr, total, skip);
+  k_scan_down<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums, nullptr, total, skip, range);
   return hipGetLastError();
 }
 
@@ -406,8 +421,8 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint3
 }
 
 hipError_t scan_exclusive_u64(const uint64_t *in, uint64_t *out, size_t n, uint64_t *total, void *temp,
-                              hipStream_t s, const uint32_t *skip) {
-  return scan_impl<uint64_t>(in, out, n, total, (uint64_t *)temp, s, skip);
+                              hipStream_t s, const uint32_t *skip, const uint32_t *range) {
+  return scan_impl<uint64_t>(in, out, n, total, (uint64_t *)temp, s, skip, range);
 }
 
 hipError_t radix_sort_u64(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int begin_bit,
