@@ -699,7 +699,7 @@ __global__ __launch_bounds__(256) void k_group_merge(NuArgs a) {
 // splits a node, so each unique key is exactly one element.  One pass writes
 // what k_descend, the group kernels and the element emission would: stops,
 // 1-based group ids, group descriptors, merge targets, the element sequence
-// and level 0 of the minimum tree.  Multi-trie batches give it the exclusive
+// and levels 0 .. 2 of the minimum tree.  Multi-trie batches give it the exclusive
 // scan of the trie-start flags in grp (k_empty_flags); a single trie has one
 // group.
 __global__ __launch_bounds__(256) void k_empty_flags(NuArgs a) {
@@ -709,10 +709,29 @@ __global__ __launch_bounds__(256) void k_empty_flags(NuArgs a) {
   a.ecount[i] = i < m && (i == 0 || a.u_trie[i] != a.u_trie[i - 1]) ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_empty_groups(NuArgs a, int multi) {
-  const uint32_t i = grid_tid();
-  const uint32_t m = a.scal[SC_M_UNIQ];
-  if (i >= m) return;
+// Levels 1 and 2 of the split points' minimum tree (k_mt_level) over the
+// block's 256 split points sp (255 past the last, above every valid value):
+// a level-1 entry per 16 lanes and the block's level-2 entry.  Every thread
+// of the block calls it.
+__device__ __forceinline__ void mt_block_levels(const NuArgs &a, uint32_t sp) {
+  __shared__ uint32_t wmin[4];
+  uint32_t v = sp;
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+  const size_t off1 = mt_level_len(a.st_stride, 0);
+  if (a.st_levels > 1 && (threadIdx.x & 15) == 0) a.st[off1 + (size_t)blockIdx.x * 16 + threadIdx.x / 16] = (uint8_t)v;
+  v = min(v, (uint32_t)__shfl_xor((int)v, 16));
+  v = min(v, (uint32_t)__shfl_xor((int)v, 32));
+  if (__lane_id() == 0) wmin[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (a.st_levels > 2 && threadIdx.x == 0)
+    a.st[off1 + mt_level_len(a.st_stride, 1) + blockIdx.x] = (uint8_t)min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+}
+
+// Element i of a build into empty tries: its stop, group and element
+// entries; returns split point i (between elements i and i + 1: 0 where the
+// next element starts another trie), or 255 for the last element.
+__device__ __forceinline__ uint32_t empty_element(const NuArgs &a, uint32_t i, uint32_t m, int multi) {
   const uint32_t t = a.u_trie[i];
   if (t >= a.ntries) raise_error(a.P.cnt, ERR_BAD_KEY, a.u_idx[i]);
   const bool start = i == 0 || (multi && a.u_trie[i - 1] != t);
@@ -741,10 +760,42 @@ __global__ __launch_bounds__(256) void k_empty_groups(NuArgs a, int multi) {
   }
   a.e_src[i] = i;
   a.e_grp[i] = g - 1;
-  if (i > 0) a.st[i - 1] = start ? 0 : (uint8_t)(a.u_lcp[i] + 1);
   if (i + 1 == m) {
     a.scal[SC_NGROUP] = g;
     a.scal[SC_NE] = m;
+    return 255;
+  }
+  const uint32_t sp = multi && a.u_trie[i + 1] != t ? 0u : (uint32_t)a.u_lcp[i + 1] + 1;
+  a.st[i] = (uint8_t)sp;
+  return sp;
+}
+
+__global__ __launch_bounds__(256) void k_empty_groups(NuArgs a, int multi) {
+  const uint32_t i = grid_tid();
+  const uint32_t m = a.scal[SC_M_UNIQ];
+  mt_block_levels(a, i < m ? empty_element(a, i, m, multi) : 255u);
+}
+
+// Levels k0 .. of the minimum tree in one workgroup, each from the one below.
+__global__ __launch_bounds__(256) void k_mt_top(NuArgs a, int k0) {
+  const uint32_t ne = a.scal[SC_NE];
+  uint32_t cnt = ne > 0 ? ne - 1 : 0;  // valid entries of level k - 1
+  size_t off = 0;                      // and its offset
+  for (int q = 0; q < k0 - 1; q++) {
+    off += mt_level_len(a.st_stride, q);
+    cnt = (cnt + 15) / 16;
+  }
+  for (int k = k0; k < a.st_levels; k++) {
+    const size_t len = mt_level_len(a.st_stride, k - 1);
+    const uint32_t nout = (cnt + 15) / 16;
+    for (uint32_t b = threadIdx.x; b < nout; b += blockDim.x) {
+      uint32_t mn = 255;
+      for (uint32_t j = 16 * b; j < 16 * b + 16 && j < cnt; j++) mn = min(mn, (uint32_t)a.st[off + j]);
+      a.st[off + len + b] = (uint8_t)mn;
+    }
+    __syncthreads();  // the level is complete for the next one
+    off += len;
+    cnt = nout;
   }
 }
 
@@ -1563,11 +1614,18 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
   // elements: the unique keys, plus one virtual element per split short node
   // (none when the tries were empty)
   uint32_t ne_max = empty ? a.n + 1 : 2 * a.n + 2;
-  size_t cnt = ne_max;  // bound on the entries of each level
-  for (int k = 1; k < a.st_levels; k++) {
-    cnt = (cnt + 15) / 16;
-    k_mt_level<<<nblk(cnt), kB, 0, s>>>(a, k);
-    GMPT_LAUNCH_CHECK();
+  if (empty) {  // k_empty_groups wrote levels 0 .. 2
+    if (a.st_levels > 3) {
+      k_mt_top<<<1, kB, 0, s>>>(a, 3);
+      GMPT_LAUNCH_CHECK();
+    }
+  } else {
+    size_t cnt = ne_max;  // bound on the entries of each level
+    for (int k = 1; k < a.st_levels; k++) {
+      cnt = (cnt + 15) / 16;
+      k_mt_level<<<nblk(cnt), kB, 0, s>>>(a, k);
+      GMPT_LAUNCH_CHECK();
+    }
   }
   if (!empty) {  // empty tries hold no key to update
     k_updates<<<nblk(a.n), kB, 0, s>>>(a);
