@@ -201,7 +201,7 @@ def bench_phase(pkg, args):
     # the same step from host buffers: the library stages keys, indexes and
     # values over PCIe inside the timed region (reported, never `value`)
     host_s = []
-    for k in range(3):
+    for k in range(6):
         t.clear()
         t.synchronize()
         w1 = time.perf_counter()

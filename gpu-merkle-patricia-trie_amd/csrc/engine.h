@@ -17,6 +17,7 @@
 namespace gmpt {
 
 class HostStager;
+struct StageSeg;
 
 struct BatchDesc {
   const uint8_t *keys_hexs = nullptr;
@@ -157,7 +158,10 @@ class Forest {
   void release(DevBuf &b);
   int sync_counters();
   bool grow_key_arena_after_error();
-  int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values);
+  // With defer, a host batch large enough for the pinned stager leaves its
+  // values out and describes them in *defer (bytes 0 when nothing is left):
+  // the caller stages them later, before anything reads value bytes.
+  int stage_batch(const BatchDesc &b, BatchDesc *dev, uint64_t *vbase, bool copy_values, StageSeg *defer = nullptr);
   int carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid, void **temp);
   int sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int *cur);
   int next_pass(int pass);

@@ -309,9 +309,10 @@ uint8_t *Forest::value_space(uint64_t bytes) {
 
 // Host batches are copied to device staging; values go into the value arena
 // (device batches too, unless the caller lets the trie adopt its buffer).
-int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool copy_values) {
+int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool copy_values, StageSeg *defer) {
   *d = b;
   uint32_t n = b.n;
+  if (defer) *defer = StageSeg{nullptr, nullptr, 0};
   if (!b.on_device) {
     // extents of the host buffers: the largest end index of any pair (the
     // kernels reject pairs outside them)
@@ -370,7 +371,10 @@ int Forest::stage_batch(const BatchDesc &b, BatchDesc *d, uint64_t *vbase, bool 
     if (copy_values) {
       uint8_t *vs = value_space(vb);
       if (!vs) return -2;
-      seg[ns++] = {vs, b.values_bytes, vb};
+      if (staged && defer)
+        *defer = StageSeg{vs, b.values_bytes, vb};
+      else
+        seg[ns++] = {vs, b.values_bytes, vb};
       d->values_bytes = vs;
     }
     if (staged) {
@@ -553,7 +557,14 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
   if (b.n > 0) {
     BatchDesc d;
     uint64_t vbase = 0;
-    if ((r = stage_batch(b, &d, &vbase, true))) return r;
+    StageSeg vals{nullptr, nullptr, 0};  // a large host batch's values, staged while the insert kernels run
+    if ((r = stage_batch(b, &d, &vbase, true, side_.stream ? &vals : nullptr))) return r;
+    // the values' copies go on the second stream, after all work queued so
+    // far on the forest's stream (not after the insert kernels)
+    if (vals.bytes) {
+      HIP_TRY(hipEventRecord(side_.fork, stream_));
+      HIP_TRY(hipStreamWaitEvent(side_.stream, side_.fork, 0));
+    }
     const uint32_t n = b.n;
     // slots: branches <= 2n+2, shorts <= 2(2n+2), values: 2n+2 for elements + n for updates
     if ((r = ensure_pools(2ull * n + 2, 4ull * n + 4, 3ull * n + 2, d.key_slots))) return r;
@@ -580,6 +591,15 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
           // pass or an error is pending), one synchronisation per pass
           HIP_TRY(nu_emit(a, temp, stream_, true));
           HIP_TRY(nu_build(a, temp, stream_, true));
+        }
+        if (vals.bytes) {
+          // no insert kernel reads value bytes (only their index pairs): the
+          // values cross PCIe on the second stream while they run, and the
+          // forest's stream waits for them before anything queued later
+          HIP_TRY(stager_->copy(side_.stream, &vals, 1));
+          HIP_TRY(hipEventRecord(side_.join, side_.stream));
+          HIP_TRY(hipStreamWaitEvent(stream_, side_.join, 0));
+          vals.bytes = 0;
         }
         if ((r = sync_counters())) break;
         pass = next_pass(pass);

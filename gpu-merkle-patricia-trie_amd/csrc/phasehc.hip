@@ -721,9 +721,11 @@ constexpr uint32_t kInnerWords = 80;              // 640 LDS bytes per node >= 4
 // segment streams the message words to out.  Other nodes (extension short
 // nodes, branches with embedded children or a value) take the generic encoder
 // on lane 0.  Every thread of the block calls this (it synchronises the
-// block); lane 0 returns the encoding length.
+// block); lane 0 returns the encoding length.  With in_lds, the LDS copy of
+// a fast node's message stands (*in_lds set on the segment's lanes) and out
+// is not written; the generic encoder still writes out.
 __device__ __forceinline__ uint32_t encode_inner_node(const Pools &P, ref_t r, bool active, bool part, uint32_t s,
-                                                      uint64_t *lds, uint64_t *out) {
+                                                      uint64_t *lds, uint64_t *out, bool *in_lds = nullptr) {
   const bool is_branch = part && active && ref_type(r) == T_BRANCH;
   ref_t c = REF_NULL, c16 = REF_NULL;
   if (is_branch) {
@@ -782,8 +784,10 @@ __device__ __forceinline__ uint32_t encode_inner_node(const Pools &P, ref_t r, b
     }
   }
   __syncthreads();
+  if (in_lds) *in_lds = fast;
   if (fast) {
-    for (uint32_t w = s; w < 17 * nb; w += kInnerGroup) out[w] = lds[w];
+    if (!in_lds)
+      for (uint32_t w = s; w < 17 * nb; w += kInnerGroup) out[w] = lds[w];
   } else if (part && active && s == 0) {
     encode_generic(P, r, out, &enc);
   }
@@ -1304,8 +1308,8 @@ __global__ __launch_bounds__(256) void k_hc_perm_spread(Pools P, const ref_t *li
 
 // Small inner levels in one launch: one node per wave; lanes 0..15 encode it
 // (encode_inner_node), then the wave runs the sponge with the state spread
-// over it.  The message passes through the node's scratch slot, which the
-// block barrier makes visible to the wave (it stays in cache).
+// over it, reading the message from the node's LDS copy (a generic-encoded
+// node's from its scratch slot, which the block barrier makes visible).
 __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restrict__ list,
                                                   const uint64_t *__restrict__ slot, uint32_t n,
                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
@@ -1316,14 +1320,17 @@ __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restri
   const bool active = i < n;
   const ref_t r = active ? list[i] : REF_NULL;
   uint64_t *out = active ? scratch + slot[i] / 8 : scratch;
-  uint32_t enc = encode_inner_node(P, r, active, lane < kInnerGroup, lane & (kInnerGroup - 1), lds[g], out);
+  bool in_lds = false;
+  uint32_t enc = encode_inner_node(P, r, active, lane < kInnerGroup, lane & (kInnerGroup - 1), lds[g], out, &in_lds);
   enc = (uint32_t)__shfl((int)enc, 0, 64);
+  in_lds = __shfl((int)in_lds, 0, 64) != 0;
   if (active && lane == 0) enc_len[i] = enc;
   __syncthreads();  // the node's message words are visible to all its lanes
   const uint32_t nb = active && enc >= 32 ? keccak_blocks(enc) : 0u;  // uniform over the wave
   SpreadLane c(lane);
   uint32_t l, h;
-  sponge_spread(out, nb, lane, c, l, h);
+  // a fast node's message stays in LDS (at most 4 blocks: 68 of the 80 words)
+  sponge_spread(in_lds ? lds[g] : out, nb, lane, c, l, h);
   if (nb == 0) return;
   if (spread_digest_lane(lane)) ((uint64_t *)node_hash(P, r))[lane >> 3] = ((uint64_t)h << 32) | l;
   if (lane == 0) node_done(P, r, 32);
