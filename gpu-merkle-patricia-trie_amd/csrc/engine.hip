@@ -40,7 +40,7 @@ size_t nu_ws_bytes(size_t n, bool tries) {
   size_t ne = 2 * n + 2;
   size_t b = 0;
   b += n * (4 + 8 + 8 + 16 + 8 + (tries ? 8 : 0) + 1 + 1 + 4 + 8 + 2 + sizeof(Stop) + 4 + sizeof(GroupInfo) + 4);
-  b += ne * (4 + 16);  // leaf records
+  b += ne * (4 + 16 + 4);  // leaf and branch records
   b += n * 8 * 5;      // sorted krefs; unique-order krefs and windows (two copies each)
   b += n * (4 * 7 + 1 + sizeof(Stop));
   b += ne * (4 + 4 + 4 + 4) + mt_bytes(align_up(ne, 256));
@@ -474,6 +474,7 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.e_grp = c.take<uint32_t>(ne);
   a.leaf_pk = c.take<uint32_t>(ne);
   a.leaf_meta = c.take<uint64_t>(2 * ne);
+  a.bdesc = c.take<uint32_t>(ne);
   a.st_stride = align_up(ne, 256);
   a.st_levels = mt_levels(a.st_stride);
   a.st = c.take<uint8_t>(mt_bytes(a.st_stride));
@@ -636,6 +637,9 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     rec_.meta = a.leaf_meta;
     rec_.lo = cnt_.scal[SC_DBASE];
     rec_.n = cnt_.scal[SC_NE];
+    rec_.bdesc = a.bdesc;
+    rec_.blo = rec_.lo + rec_.n;
+    rec_.bn = cnt_.scal[SC_NREP_TOT];
   }
   if (targets) *targets = dirty_.p;
   if (n_targets) *n_targets = (int)cnt_.n_dirty;

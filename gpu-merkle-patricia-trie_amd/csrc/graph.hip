@@ -113,6 +113,10 @@ int Forest::capture_rebuild(const BatchDesc &b) {
     rec.meta = a.leaf_meta;
     rec.lo = 0;  // the dirty list starts empty after the clear
     rec.n = plan_.ne;
+    // then the build's branches and their extension slots, as many each
+    rec.bdesc = a.bdesc;
+    rec.blo = plan_.ne;
+    rec.bn = (plan_.n_dirty - std::min(plan_.n_dirty, plan_.ne)) / 2;
     HcMeta *meta = (HcMeta *)hc_meta_.p;
     if ((e = hc_classify(P, (const ref_t *)dirty_.p, n, meta, (uint32_t *)hc_cursor_.p, (ref_t *)hc_sorted_.p,
                          (uint64_t *)hc_slot_.p, (uint32_t *)hc_enc_.p, ws2_.p, stream_, lmeta_tmp, lmeta, rec)) !=

@@ -59,4 +59,22 @@ __device__ __forceinline__ uint32_t leaf_packed(uint32_t cnt, uint32_t vsz) {
   return (level * kHcClasses + cls) | (min(nb, kPackedNbMax) << 12);
 }
 
+// A new branch's record (NuArgs::bdesc): depth | children << 16, and
+// kBranchRecChase when a child sits in slot 16 (its value's size counts).
+constexpr uint32_t kBranchRecChild = 1u << 16;
+constexpr uint32_t kBranchRecChase = 1u << 31;
+
+// Packed bin of a branch from its record, as the classification derives it
+// from the node (every child counted as a 32-byte digest, slot 16 empty), or
+// kLeafPackedNone when the node must be read.
+__device__ __forceinline__ uint32_t branch_packed(uint32_t rec) {
+  if (rec & kBranchRecChase) return kLeafPackedNone;
+  const uint32_t depth = rec & 0xffffu, k = (rec >> 16) & 0x7fffu;
+  const uint32_t pay = 17 + 32 * k;  // 16 - k empty slots, k 33-byte children, empty slot 16
+  const uint32_t nb = keccak_blocks(rlp_hdr_len(pay) + pay);
+  const uint32_t cls = nb > kHcClasses ? kHcClasses - 1 : nb - 1;
+  const uint32_t level = 1 + (kHcMaxDepth - min(depth, kHcMaxDepth));
+  return (level * kHcClasses + cls) | (min(nb, kPackedNbMax) << 12);
+}
+
 }  // namespace gmpt

@@ -39,6 +39,10 @@ struct LeafRecords {
   const uint32_t *pk = nullptr;
   const uint64_t *meta = nullptr;
   uint32_t lo = 0, n = 0;
+  // the build's new branches, dirty-list entries [blo, blo + bn): bdesc[k] as
+  // NuArgs::bdesc (depth, child count, whether to derive the bin from the node)
+  const uint32_t *bdesc = nullptr;
+  uint32_t blo = 0, bn = 0;
 };
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp = nullptr,

@@ -175,6 +175,15 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
     // cannot take it) and metadata were recorded then
     rpk[q] = i < n && rec.pk && ri < rec.n ? rec.pk[ri] : kLeafPackedNone;
   }
+  // a branch the last build created: its bin from the depth and child count
+  // recorded then (or kLeafPackedNone: read the node)
+  uint32_t bpk[kHcPer];
+#pragma unroll
+  for (int q = 0; q < kHcPer; q++) {
+    const uint32_t i = i0 + 256u * q;
+    const uint32_t bi = i - rec.blo;
+    bpk[q] = i < n && rec.bdesc && bi < rec.bn ? branch_packed(rec.bdesc[bi]) : kLeafPackedNone;
+  }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kHcPer; q++) {
@@ -189,6 +198,9 @@ __global__ __launch_bounds__(256) void k_hc_hist(Pools P, const ref_t *__restric
         flag = kPackedRecorded;
         // without the word builder every leaf is hashed at level 0
         if (!lmeta_tmp && bin >= kHcClasses) bin -= kHcClasses;
+      } else if (bpk[q] != kLeafPackedNone) {
+        bin = bpk[q] & 0xfff;
+        nb = (bpk[q] >> 12) & kPackedNbMax;
       } else {
         bin = bin_of(P, r[q], &nb);
         if (lmeta_tmp && bin < kLdsLeafBins) {

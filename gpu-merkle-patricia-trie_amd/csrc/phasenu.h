@@ -55,6 +55,7 @@ struct NuArgs {
   uint64_t *kref;        // kpoff | nibbles << 48: one load gives a key's place and length
   uint64_t key_cap;      // key arena bytes allocated (bounds the speculative packing)
   uint32_t *leaf_pk;     // per element: the new leaf's hashing bin (hc_leaf.h), or kLeafPackedNone
+  uint32_t *bdesc;       // per new branch (by rank): depth | children << 16 | kBranchRecChase (hc_leaf.h)
   uint64_t *leaf_meta;   // per element: the new leaf's metadata words (two)
   // sort
   uint64_t *sk0, *sk1;

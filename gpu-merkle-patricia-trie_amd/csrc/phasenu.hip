@@ -1113,7 +1113,17 @@ __global__ __launch_bounds__(256) void k_init_branches(NuArgs a) {
   for (int k = 0; k < 16; k++) w[k] = 0;
   B.depth = (uint16_t)(L - 1);
   B.dirty = 1;
+  a.bdesc[a.bid[j]] = L - 1;  // the links below count its children
   put_dirty(a, q.dbase + q.ne + a.bid[j], make_ref(T_BRANCH, id));
+}
+
+// One more child linked into the build's branch of rank `rank` (the record
+// the classification bins it by); a child in slot 16 makes it read the node.
+__device__ __forceinline__ void count_child(const NuArgs &a, uint32_t rank, uint32_t slot) {
+  if (slot == kNibbleT)
+    atomicOr(&a.bdesc[rank], kBranchRecChase);
+  else
+    atomicAdd(&a.bdesc[rank], kBranchRecChild);
 }
 
 // Source of an E element's key nibbles.
@@ -1232,10 +1242,12 @@ __global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
     }
   }
   put_dirty(a, q.dbase + q.ne + q.nrep + rank, ext);
-  if (is_root)
+  if (is_root) {
     install(a, gi, me);
-  else
+  } else {
     a.P.branch[parent_bid].child[slot] = me;
+    count_child(a, parent_bid - a.base_branch, slot);
+  }
 }
 
 // Elements: new keys become a leaf short node + value node; a split short
@@ -1319,10 +1331,12 @@ __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
   }
   a.leaf_pk[e] = pk;
   put_dirty(a, q.dbase + e, sh);
-  if (is_root)
+  if (is_root) {
     install(a, gi, me);
-  else
+  } else {
     a.P.branch[parent_bid].child[slot] = me;
+    count_child(a, parent_bid - a.base_branch, slot);
+  }
 }
 
 // Existing keys: swap in a new value node (runs before the links, so a leaf
