@@ -143,6 +143,47 @@ __device__ __forceinline__ void pack_key(uint64_t *dst, const uint8_t *k, uint32
   }
 }
 
+// 8 hex nibble bytes (0x0N each, memory order) -> 4 packed bytes, high nibble
+// first: per 32-bit half, byte pairs merge by shifts, then one byte permute
+// picks bytes 0 and 2 of each half.
+__device__ __forceinline__ uint32_t pack8_perm(uint64_t h) {
+  const uint32_t lo = (uint32_t)h, hi = (uint32_t)(h >> 32);
+  const uint32_t ylo = (lo << 4) | (lo >> 8), yhi = (hi << 4) | (hi >> 8);
+  return __builtin_amdgcn_perm(yhi, ylo, 0x06040200u);
+}
+
+// A key of at most kFastKeyBytes hex bytes held in registers: the aligned
+// words holding its bytes (no other word is read), and chunk j = its bytes
+// [8j, 8j + 8) (zero past the key).
+constexpr uint32_t kFastKeyBytes = 72;
+constexpr int kFastKeyWords = (int)((7 + kFastKeyBytes + 7) / 8);  // aligned words a key can touch
+constexpr int kFastKeyChunks = (int)(kFastKeyBytes / 8);
+struct KeyChunks {
+  uint64_t c[kFastKeyChunks + 1];
+  __device__ __forceinline__ void load(const uint8_t *k, uint32_t L) {
+    const uintptr_t a = (uintptr_t)k;
+    const uint32_t sh = (uint32_t)(a & 7);
+    const uint64_t *q = (const uint64_t *)(a - sh);
+    const uint32_t nw = (sh + L + 7) / 8;
+    uint64_t w[kFastKeyWords + 1];
+#pragma unroll
+    for (int j = 0; j <= kFastKeyWords; j++) w[j] = (uint32_t)j < nw ? q[j] : 0ull;
+#pragma unroll
+    for (int j = 0; j <= kFastKeyChunks; j++) {
+      const uint64_t x = sh ? (w[j] >> (8 * sh)) | (w[j + 1] << (64 - 8 * sh)) : w[j];
+      const int32_t left = (int32_t)L - 8 * j;  // key bytes from this chunk's first
+      c[j] = left >= 8 ? x : left > 0 ? x & ((1ull << (8 * left)) - 1) : 0ull;
+    }
+  }
+  // byte b (b < kFastKeyBytes)
+  __device__ __forceinline__ uint32_t byte(uint32_t b) const {
+    uint64_t v = 0;
+#pragma unroll
+    for (int j = 0; j <= kFastKeyChunks; j++) v = (uint32_t)j == b / 8 ? c[j] : v;
+    return (uint32_t)(v >> (8 * (b & 7))) & 0xffu;
+  }
+};
+
 // Validates every key, and reduces per block the common prefix with key 0
 // and the longest key (no global atomics: one slot per block).  A key as long
 // as key 0 is also packed at once to slot i of a layout with equal slots: when
@@ -158,15 +199,43 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
   if (i < a.n) {
     bool ok = key_range_ok(a, i);
     int32_t L = 1;
+    bool packed = false;  // a short key checked and packed from registers
     if (ok) {
       int32_t s = a.kidx[2 * i], e = a.kidx[2 * i + 1];
       L = e - s + 1;
       const uint8_t *k = stg.at(a, s);
       uint32_t nn = (uint32_t)L - 1;  // nibbles before the terminator
-      ok = k[nn] == kNibbleT;
-      for (uint32_t o = 0; ok && o < nn; o += 8) {
-        uint64_t w = ldn(k + o, min(8u, nn - o));
-        ok = (w & 0xF0F0F0F0F0F0F0F0ull) == 0;
+      if ((uint32_t)L <= kFastKeyBytes) {
+        KeyChunks kc;
+        kc.load(k, (uint32_t)L);
+        uint64_t hi = 0;  // high nibbles of the bytes before the terminator
+#pragma unroll
+        for (int j = 0; j < kFastKeyChunks; j++) {
+          const int32_t left = (int32_t)nn - 8 * j;
+          hi |= left >= 8 ? kc.c[j] : left > 0 ? kc.c[j] & ((1ull << (8 * left)) - 1) : 0ull;
+        }
+        ok = kc.byte(nn) == kNibbleT && (hi & 0xF0F0F0F0F0F0F0F0ull) == 0;
+        // the same slot layout as pack_key: nibble pairs, zero past nn
+        if (ok && (uint32_t)L == L0 && a.kbase + (uint64_t)(i + 1) * slot_bytes((uint32_t)L) <= a.key_cap) {
+          uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + (uint64_t)i * slot_bytes((uint32_t)L));
+          const uint32_t nbw = slot_bytes((uint32_t)L) / 8;
+#pragma unroll
+          for (int b = 0; b < (kFastKeyChunks + 1) / 2; b++) {
+            if ((uint32_t)b >= nbw) break;
+            uint64_t h0 = kc.c[2 * b], h1 = kc.c[2 * b + 1];
+            const int32_t l0 = (int32_t)nn - 16 * b, l1 = l0 - 8;  // real nibbles left in each chunk
+            h0 = l0 >= 8 ? h0 : l0 > 0 ? h0 & ((1ull << (8 * l0)) - 1) : 0ull;
+            h1 = l1 >= 8 ? h1 : l1 > 0 ? h1 & ((1ull << (8 * l1)) - 1) : 0ull;
+            dst[b] = (uint64_t)pack8_perm(h0) | ((uint64_t)pack8_perm(h1) << 32);
+          }
+          packed = true;
+        }
+      } else {
+        ok = k[nn] == kNibbleT;
+        for (uint32_t o = 0; ok && o < nn; o += 8) {
+          uint64_t w = ldn(k + o, min(8u, nn - o));
+          ok = (w & 0xF0F0F0F0F0F0F0F0ull) == 0;
+        }
       }
       // common prefix with key 0 (the terminator is a symbol of its own)
       if (ok && key_range_ok(a, 0)) {
@@ -206,7 +275,7 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
     odd = !ok || len != L0;
     if (ok && !odd && a.kbase + (uint64_t)(i + 1) * slot <= a.key_cap) {
       const uint64_t koff = (uint64_t)i * slot;
-      pack_key((uint64_t *)(a.P.keys + a.kbase + koff), stg.at(a, a.kidx[2 * i]), len);
+      if (!packed) pack_key((uint64_t *)(a.P.keys + a.kbase + koff), stg.at(a, a.kidx[2 * i]), len);
       a.kref[i] = koff | ((uint64_t)len << 48);
     }
   }
