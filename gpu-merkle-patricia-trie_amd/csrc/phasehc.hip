@@ -1176,11 +1176,14 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
                                                         uint32_t *__restrict__ enc_len) {
   if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __shared__ uint64_t head[256][kLeafDirectHead];
-  const uint32_t i = grid_tid();
+  const uint32_t j = grid_tid();
   uint32_t n = 0;  // the leaves of level 0, sorted first
 #pragma unroll
   for (uint32_t c = 0; c < kLdsLeafBins; c++) n += hist[c];
-  if (i >= n) return;  // no block barrier below
+  if (j >= n) return;  // no block barrier below
+  // the leaves from the last: the level is sorted by block count, so the
+  // longest leaves start first and the 1-block ones fill the end of the launch
+  const uint32_t i = n - 1 - j;
   enc_len[i] = hash_leaf_shaped(P, list[i], lmeta[2 * (size_t)i], lmeta[2 * (size_t)i + 1], head[threadIdx.x]);
 }
 
