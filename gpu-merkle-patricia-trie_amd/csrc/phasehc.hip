@@ -1094,11 +1094,27 @@ __device__ __noinline__ uint32_t leaf_raw(const Pools &P, ref_t r) {
 
 constexpr uint32_t kLeafDirectHead = 7;  // LDS words per thread for the head (6 used; odd spreads banks)
 
+// Message words [wlo, whi] that can hold a leaf's string bytes or its 0x01
+// padding byte, from its metadata word m0 alone (a superset: the encoding
+// length is taken as if the string had a header, the string's start one byte
+// earlier than that).
+__device__ __forceinline__ void leaf_value_words(uint64_t m0, uint32_t &wlo, uint32_t &whi) {
+  const uint32_t cnt = (uint32_t)(m0 >> 41) & 255u, vsz = (uint32_t)(m0 >> 50);
+  const uint32_t kc = cnt / 2 + 1;
+  const uint32_t payload = (kc == 1 ? 1u : 1u + kc) + rlp_hdr_len(vsz) + vsz;
+  const uint32_t enc = rlp_hdr_len(payload) + payload;
+  wlo = (enc - vsz - 1) / 8;
+  whi = enc / 8;
+}
+
 // Encodes and hashes one node shaped [compact key, string] from its metadata
 // words m0 (leaf_meta_words) and va (the string's bytes), building each
 // message word in registers; hp is the thread's LDS row for the message head.
-// Returns the encoding length.
-__device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, uint64_t m0, uint64_t va, uint64_t *hp) {
+// Only message words [wlo, whi] (the same on every lane of the wave, and
+// covering each lane's leaf_value_words) are built from the string: the
+// others hold head bytes or zeros.  Returns the encoding length.
+__device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, uint64_t m0, uint64_t va, uint64_t *hp,
+                                                     uint32_t wlo = 0, uint32_t whi = 1u << 30) {
   const uint32_t vsz = (uint32_t)(m0 >> 50);
   const uint32_t vb0 = vsz == 1 ? (uint32_t)(gload(va & ~7ull) >> (8 * (va & 7))) & 0xffu : 0u;
   const LeafLayout l = leaf_layout(m0, vb0);
@@ -1139,14 +1155,22 @@ __device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, ui
 #pragma unroll 1
   for (uint32_t b = 0; b < nb; b++) {
     {
+      // this block's words of the window (wave-uniform bounds: the word
+      // loops below branch over the words outside it)
+      const int32_t qlo = max((int32_t)wlo - (int32_t)(17 * b), 0);
+      const int32_t qhi = min((int32_t)whi - (int32_t)(17 * b), 16);
       uint64_t W[18];
 #pragma unroll
       for (int q = 0; q < 18; q++) {
-        const int32_t j = clamp_i(jbase + (int32_t)(17 * b) + q, 0, J);  // clamped into the value
-        W[q] = hasv ? gload(V0 + 8 * (uint64_t)(uint32_t)j) : 0ull;
+        W[q] = 0;
+        if (q >= qlo && q <= qhi + 1) {
+          const int32_t j = clamp_i(jbase + (int32_t)(17 * b) + q, 0, J);  // clamped into the value
+          W[q] = hasv ? gload(V0 + 8 * (uint64_t)(uint32_t)j) : 0ull;
+        }
       }
 #pragma unroll
       for (int q = 0; q < 17; q++) {
+        if (q < qlo || q > qhi) continue;
         const int32_t w = 17 * (int32_t)b + q;
         // value bytes: message bytes [max(vstart, 8w), min(enc, 8w + 8)) of this word
         const int32_t lo_b = clamp_i((int32_t)vstart - 8 * w, 0, 8);
@@ -1155,9 +1179,9 @@ __device__ __forceinline__ uint32_t hash_leaf_shaped(const Pools &P, ref_t r, ui
         const uint64_t mk = hasv && hi_b > lo_b ? (~0ull << (8 * lo_b)) & (~0ull >> (64 - 8 * hi_b)) : 0ull;
         uint64_t v = f & mk;
         if ((int32_t)(enc >> 3) == w) v |= 0x01ull << (8 * (enc & 7));  // Keccak padding
-        if (q == 16 && b + 1 == nb) v |= 0x80ull << 56;
         st[q] ^= v;
       }
+      if (b + 1 == nb) st[16] ^= 0x80ull << 56;  // the last byte of the last block
     }
     keccak_f1600(st);
   }
@@ -1180,11 +1204,23 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
   uint32_t n = 0;  // the leaves of level 0, sorted first
 #pragma unroll
   for (uint32_t c = 0; c < kLdsLeafBins; c++) n += hist[c];
-  if (j >= n) return;  // no block barrier below
   // the leaves from the last: the level is sorted by block count, so the
   // longest leaves start first and the 1-block ones fill the end of the launch
-  const uint32_t i = n - 1 - j;
-  enc_len[i] = hash_leaf_shaped(P, list[i], lmeta[2 * (size_t)i], lmeta[2 * (size_t)i + 1], head[threadIdx.x]);
+  const bool act = j < n;
+  const uint32_t i = act ? n - 1 - j : 0u;
+  const uint64_t m0 = act ? lmeta[2 * (size_t)i] : 0ull;
+  // the wave's window of string words, reduced while every lane is active
+  uint32_t wlo = 0xffffu, whi = 0;
+  if (act) leaf_value_words(m0, wlo, whi);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    wlo = min(wlo, (uint32_t)__shfl_xor((int)wlo, off));
+    whi = max(whi, (uint32_t)__shfl_xor((int)whi, off));
+  }
+  wlo = __builtin_amdgcn_readfirstlane(wlo);
+  whi = __builtin_amdgcn_readfirstlane(whi);
+  if (!act) return;  // no block barrier below
+  enc_len[i] = hash_leaf_shaped(P, list[i], m0, lmeta[2 * (size_t)i + 1], head[threadIdx.x], wlo, whi);
 }
 
 // The extension class of an inner level (kExtClass): each extension encodes

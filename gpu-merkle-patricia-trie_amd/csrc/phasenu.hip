@@ -448,10 +448,17 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
                                                   uint64_t cmask) {
   const uint32_t j = grid_tid();
   const bool active = j < a.n;
+  // with every key as long as key 0, packed in equal slots, a key's place is
+  // its input index times the slot size: no load
+  const uint32_t Lu = a.scal[SC_UNIFORM] ? a.scal[SC_MAXLEN] : 0u;
+  const uint64_t su = slot_bytes(Lu);
+  const auto kref_of = [&](uint32_t i) -> uint64_t {
+    return Lu ? (uint64_t)i * su | (uint64_t)Lu << 48 : a.kref[i];
+  };
   // key j's place and trie; key j - 1's come from the lane before (one
   // random load per key instead of two), or a load on a wave's first lane
   const uint32_t ib = active ? sv[j] : 0u;
-  const uint64_t rb = active ? a.kref[ib] : 0ull;
+  const uint64_t rb = active ? kref_of(ib) : 0ull;
   const uint32_t tb = active ? trie_of(a, ib) : 0u;
   uint64_t ra = (uint64_t)__shfl_up((long long)rb, 1);
   uint32_t ta = (uint32_t)__shfl_up((int)tb, 1);
@@ -459,7 +466,7 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
   if (a.skref) a.skref[j] = rb;
   if (__lane_id() == 0 && j > 0) {
     const uint32_t ia = sv[j - 1];
-    ra = a.kref[ia];
+    ra = kref_of(ia);
     ta = trie_of(a, ia);
   }
   uint8_t fl = 0, l = 0;
