@@ -43,7 +43,7 @@ size_t nu_ws_bytes(size_t n, bool tries) {
   b += ne * (4 + 16 + 4);  // leaf and branch records
   b += n * 8 * 5;      // sorted krefs; unique-order krefs and windows (two copies each)
   b += n * (4 * 7 + 1 + sizeof(Stop));
-  b += ne * (4 + 4 + 4 + 4) + mt_bytes(align_up(ne, 256));
+  b += ne * (4 + 4 + 4 + 4 + 8) + mt_bytes(align_up(ne, 256)) + 4 * (ne / 256 + 2);
   b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
   b += 64 * 256;  // alignment slack for ~40 carves
   return b;
@@ -480,6 +480,8 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.st = c.take<uint8_t>(mt_bytes(a.st_stride));
   a.repof = c.take<uint32_t>(ne);
   a.bid = c.take<uint32_t>(ne);
+  a.plohi = c.take<uint64_t>(ne);
+  a.bcnt = c.take<uint32_t>(ne / 256 + 2);
   *temp = c.take<uint8_t>(std::max(radix_temp_bytes(n), scan_temp_bytes(ne)));
   if (c.off > ws_.cap) {
     set_error("workspace carve overflow");

@@ -51,7 +51,7 @@ struct NuArgs {
   uint64_t vbase;        // device address of this batch's value bytes
   // per input key
   uint32_t *klen;
-  uint64_t *kpoff;       // relative to kbase after the scan
+  uint64_t *kpoff;       // per block of 256 keys: its slot bytes, then (k_keys_reduce) its first slot's offset
   uint64_t *kref;        // kpoff | nibbles << 48: one load gives a key's place and length
   uint64_t key_cap;      // key arena bytes allocated (bounds the speculative packing)
   uint32_t *leaf_pk;     // per element: the new leaf's hashing bin (hc_leaf.h), or kLeafPackedNone
@@ -90,7 +90,10 @@ struct NuArgs {
   size_t st_stride;      // bytes of level 0 (a multiple of 256)
   int st_levels;
   uint32_t *repof;       // per split point: representative split point (or ~0u)
-  uint32_t *bid;         // per split point: branch slot (reps only)
+  uint32_t *bid;         // per split point: branch rank (reps only)
+  uint32_t *bcnt;        // per block of 256 split points: representatives (k_split_reps)
+  uint64_t *plohi;       // per representative split point: its interval's bounding split points
+                         // (nearest to the left and right with a smaller value) lo | hi << 32, -1 / nsp past the ends
   // LockNU
   uint64_t seq_base;     // batch order stamp of key 0
   uint32_t base_branch, base_short, base_value;  // pre-assigned node slots

@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
   const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
   uint32_t lcp0 = UINT_MAX, len = 0, odd = 0;
+  uint64_t sbytes = 0;  // the key's packed slot size
   const uint32_t L0 = key_range_ok(a, 0) ? (uint32_t)(a.kidx[1] - a.kidx[0] + 1) : 0u;
   if (i < a.n) {
     bool ok = key_range_ok(a, i);
@@ -219,6 +220,7 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
         if (ok && (uint32_t)L == L0 && a.kbase + (uint64_t)(i + 1) * slot_bytes((uint32_t)L) <= a.key_cap) {
           uint64_t *dst = (uint64_t *)(a.P.keys + a.kbase + (uint64_t)i * slot_bytes((uint32_t)L));
           const uint32_t nbw = slot_bytes((uint32_t)L) / 8;
+          uint64_t first = 0;  // the slot's first word: nibbles 0 .. 15
 #pragma unroll
           for (int b = 0; b < (kFastKeyChunks + 1) / 2; b++) {
             if ((uint32_t)b >= nbw) break;
@@ -227,8 +229,15 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
             h0 = l0 >= 8 ? h0 : l0 > 0 ? h0 & ((1ull << (8 * l0)) - 1) : 0ull;
             h1 = l1 >= 8 ? h1 : l1 > 0 ? h1 & ((1ull << (8 * l1)) - 1) : 0ull;
             dst[b] = (uint64_t)pack8_perm(h0) | ((uint64_t)pack8_perm(h1) << 32);
+            if (b == 0) first = dst[b];
           }
           packed = true;
+          // the sort window of a batch with no common prefix (W = 0), as
+          // k_keys_pack builds it from the slot: that pass skips such a batch
+          if (a.sk0) {
+            a.sk0[i] = __builtin_bswap64(first);
+            a.sv0[i] = i;
+          }
         }
       } else {
         ok = k[nn] == kNibbleT;
@@ -269,7 +278,7 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
     len = (uint32_t)L;
     a.klen[i] = len;
     const uint32_t slot = slot_bytes(len);
-    a.kpoff[i] = slot;
+    sbytes = slot;
     // a rejected key turns the equal-slot layout off: the general packing
     // writes its kref (the equal-slot pass only writes those of keys it packs)
     odd = !ok || len != L0;
@@ -283,13 +292,18 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
     lcp0 = min(lcp0, (uint32_t)__shfl_xor(lcp0, off));
     len = max(len, (uint32_t)__shfl_xor(len, off));
     odd |= (uint32_t)__shfl_xor(odd, off);
+    sbytes += (uint64_t)__shfl_xor((unsigned long long)sbytes, off);
   }
+  __shared__ uint64_t sb[4];
   if (__lane_id() == 0) {
     sw[threadIdx.x >> 6] = lcp0;
     sl[threadIdx.x >> 6] = len | (odd << 31);
+    sb[threadIdx.x >> 6] = sbytes;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    // the block's slot bytes: k_keys_reduce turns these into block offsets
+    a.kpoff[blockIdx.x] = sb[0] + sb[1] + sb[2] + sb[3];
     blk_w[blockIdx.x] = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
     const uint32_t m = sl[0] | sl[1] | sl[2] | sl[3];
     const uint32_t l = max(max(sl[0] & 0xFFFFu, sl[1] & 0xFFFFu), max(sl[2] & 0xFFFFu, sl[3] & 0xFFFFu));
@@ -300,11 +314,19 @@ __global__ __launch_bounds__(256) void k_keys_scan(NuArgs a, uint32_t *blk_w, ui
 __global__ __launch_bounds__(256) void k_keys_reduce(NuArgs a, const uint32_t *blk_w, const uint32_t *blk_len,
                                                      uint32_t nb) {
   __shared__ uint32_t sw[4], sl[4], so[4];
+  __shared__ uint64_t s64[4];
   uint32_t w = UINT_MAX, l = 0, odd = 0;
-  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
-    w = min(w, blk_w[b]);
-    l = max(l, blk_len[b] & 0xFFFFu);
-    odd |= blk_len[b] >> 31;
+  for (uint32_t b0 = threadIdx.x; b0 < nb; b0 += 8 * blockDim.x) {
+#pragma unroll
+    for (uint32_t q = 0; q < 8; q++) {  // eight blocks' loads in flight
+      const uint32_t b = b0 + q * blockDim.x;
+      if (b < nb) {
+        const uint32_t bw = blk_w[b], bl = blk_len[b];
+        w = min(w, bw);
+        l = max(l, bl & 0xFFFFu);
+        odd |= bl >> 31;
+      }
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     w = min(w, (uint32_t)__shfl_xor(w, off));
@@ -317,16 +339,31 @@ __global__ __launch_bounds__(256) void k_keys_reduce(NuArgs a, const uint32_t *b
     so[threadIdx.x >> 6] = odd;
   }
   __syncthreads();
+  // every key as long as key 0, and every one of them packed: the offsets
+  // are i * slot and the packing pass only builds the sort windows
+  const uint32_t L0 = key_range_ok(a, 0) ? (uint32_t)(a.kidx[1] - a.kidx[0] + 1) : 0u;
+  const uint64_t ktot = (uint64_t)a.n * slot_bytes(L0);
+  const bool uni = (so[0] | so[1] | so[2] | so[3]) == 0 && a.n > 0 && L0 > 0 && a.kbase + ktot <= a.key_cap;
+  uint64_t kbytes = ktot;
+  if (!uni) {  // uniform over the block
+    // exclusive scan of the blocks' slot bytes in place (each thread a
+    // contiguous run of blocks): the packing pass adds a block-local scan
+    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = min(nb, threadIdx.x * per), b1 = min(nb, b0 + per);
+    uint64_t part = 0;
+    for (uint32_t b = b0; b < b1; b++) part += a.kpoff[b];
+    uint64_t pre = block_exclusive_scan_256<uint64_t>(part, s64, &kbytes);
+    for (uint32_t b = b0; b < b1; b++) {
+      const uint64_t v = a.kpoff[b];
+      a.kpoff[b] = pre;
+      pre += v;
+    }
+  }
   if (threadIdx.x == 0) {
     a.scal[SC_W] = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
     a.scal[SC_MAXLEN] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
-    // every key as long as key 0, and every one of them packed: the offsets
-    // are i * slot, so the offset scan is skipped
-    const uint32_t L0 = key_range_ok(a, 0) ? (uint32_t)(a.kidx[1] - a.kidx[0] + 1) : 0u;
-    const uint64_t ktot = (uint64_t)a.n * slot_bytes(L0);
-    const bool uni = (so[0] | so[1] | so[2] | so[3]) == 0 && a.n > 0 && L0 > 0 && a.kbase + ktot <= a.key_cap;
     a.scal[SC_UNIFORM] = uni;
-    if (uni) *(uint64_t *)(a.scal + SC_KTOT) = ktot;
+    *(uint64_t *)(a.scal + SC_KTOT) = kbytes;
   }
 }
 
@@ -339,11 +376,14 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
     // bytes (nibble W sits in 8-byte word W / 16 of the slot, and the slot
     // holds byte W / 2 since W <= L)
     const uint32_t i = grid_tid();
+    const uint32_t W = a.scal[SC_W];
+    // every key took k_keys_scan's register path (equal lengths of at most
+    // kFastKeyBytes), which wrote the windows for W = 0
+    if (W == 0 && a.scal[SC_MAXLEN] <= kFastKeyBytes) return;
     if (i >= a.n || !a.sk0) return;
     const uint32_t L = a.klen[i];
     const uint32_t slot = slot_bytes(L);
     const uint64_t *src = (const uint64_t *)(a.P.keys + a.kbase + (uint64_t)i * slot);
-    const uint32_t W = a.scal[SC_W];
     const uint32_t q = W / 16, d = W % 16;
     const uint64_t hi = __builtin_bswap64(src[q]);
     const uint64_t lo = (q + 1) * 8 < slot ? __builtin_bswap64(src[q + 1]) : 0ull;
@@ -353,6 +393,12 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
   }
   const KeyStager stg = stage_keys(a, lds);
   uint32_t i = grid_tid();
+  // the key's slot: its block's offset plus the slots before it in the block
+  __shared__ uint64_t s64[4];
+  const uint32_t Li = i < a.n ? a.klen[i] : 0u;
+  uint64_t btot;
+  const uint64_t koff =
+      a.kpoff[blockIdx.x] + block_exclusive_scan_256<uint64_t>(i < a.n ? slot_bytes(Li) : 0u, s64, &btot);
   if (i >= a.n) return;
   // The packed keys must fit the arena.  The host reserves it from the key
   // buffer's size, which index pairs that overlap or repeat a key range can
@@ -369,9 +415,8 @@ __global__ __launch_bounds__(256) void k_keys_pack(NuArgs a) {
     if (i == 0) raise_error(a.P.cnt, ERR_POOL, kErrArgKeyArena);
     return;
   }
-  uint32_t L = a.klen[i];
+  const uint32_t L = Li;
   const uint8_t *k = key_range_ok(a, i) ? stg.at(a, a.kidx[2 * i]) : a.kh;
-  const uint64_t koff = a.kpoff[i];
   a.kref[i] = koff | ((uint64_t)L << 48);
   pack_key((uint64_t *)(a.P.keys + a.kbase + koff), k, L);
   const uint32_t nn = L - 1;  // real nibbles (0 for a rejected key)
@@ -440,6 +485,25 @@ __device__ __forceinline__ uint64_t scatter_bits(uint64_t x, uint64_t m) {
   return r;
 }
 
+// Offset of this block's first entry from per-block counts cnt (one per
+// 256-entry block of the launch): with pre, cnt already holds the exclusive
+// scan of the counts; otherwise the block adds up the counts of the blocks
+// before it (few blocks: kSelfSumBlocks at most).  Every thread of the block
+// calls it (it synchronises the block unless pre).
+constexpr uint32_t kSelfSumBlocks = 4096;
+__device__ __forceinline__ uint32_t block_offset(const uint32_t *cnt, bool pre, uint32_t *sh) {
+  if (pre) return cnt[blockIdx.x];
+  uint32_t part = 0;  // all of the thread's loads in flight at once
+#pragma unroll
+  for (uint32_t q = 0; q < kSelfSumBlocks / 256; q++) {
+    const uint32_t t = threadIdx.x + 256 * q;
+    part += t < blockIdx.x ? cnt[t] : 0u;
+  }
+  uint32_t tot;
+  block_exclusive_scan_256<uint32_t>(part, sh, &tot);
+  return tot;
+}
+
 // Adjacent keys of the sorted order: LCP, duplicate and new-trie flags.
 // With cmask != 0 the keys are windows compressed to the bits of cmask
 // (k_compress_windows): the XOR of two is scattered back to window bits
@@ -462,14 +526,14 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
   const uint32_t tb = active ? trie_of(a, ib) : 0u;
   uint64_t ra = (uint64_t)__shfl_up((long long)rb, 1);
   uint32_t ta = (uint32_t)__shfl_up((int)tb, 1);
-  if (!active) return;
+  uint8_t fl = 0, l = 0;
+  if (active) {
   if (a.skref) a.skref[j] = rb;
   if (__lane_id() == 0 && j > 0) {
     const uint32_t ia = sv[j - 1];
     ra = kref_of(ia);
     ta = trie_of(a, ia);
   }
-  uint8_t fl = 0, l = 0;
   if (j == 0) {
     fl = 2;
   } else {
@@ -507,21 +571,29 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
   }
   a.lcp[j] = l;
   a.flags[j] = fl;
-  a.run[j] = (fl & 1) ? 0u : 1u;
+  }
+  // the block's run starts (keys that are no duplicate of the one before):
+  // k_unique numbers the runs from these counts
+  const uint32_t starts = (uint32_t)__syncthreads_count(active && (fl & 1) == 0);
+  if (threadIdx.x == 0) a.run[blockIdx.x] = starts;
 }
 
 // After the inclusive scan of run starts: run r spans sorted positions with
 // run[j] == r+1.  The first position carries the LCP to the previous distinct
 // key, the last position the surviving (last-written) input index.
 __global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint64_t *sk, const uint32_t *sv,
-                                                const uint32_t *run_excl) {
+                                                const uint32_t *starts, int pre) {
+  __shared__ uint32_t sh[4];
   uint32_t j = grid_tid();
+  const bool first = j < a.n && (a.flags[j] & 1) == 0;
+  // the run starts before this one (block offset from the per-block counts
+  // of k_adjacent, then a block-local scan): a run start's count is its run
+  // index; later positions of the run count that start already
+  uint32_t btot;
+  const uint32_t excl = block_offset(starts, pre != 0, sh) + block_exclusive_scan_256<uint32_t>(first, sh, &btot);
   if (j >= a.n) return;
-  bool first = (a.flags[j] & 1) == 0;
   bool last = (j + 1 == a.n) || ((a.flags[j + 1] & 1) == 0);
-  // run_excl is the exclusive scan of run-start flags: a run start's value is
-  // its run index; later positions of the run count that start already.
-  uint32_t r = first ? run_excl[j] : run_excl[j] - 1u;
+  uint32_t r = first ? excl : excl - 1u;
   if (first) {
     a.u_lcp[r] = a.lcp[j];
     a.u_first[r] = (a.flags[j] >> 1) & 1;
@@ -1035,26 +1107,42 @@ __global__ __launch_bounds__(256) void k_mt_level(NuArgs a, int k) {
   a.st[off + mt_level_len(a.st_stride, k - 1) + b] = (uint8_t)mn;
 }
 
+// The tree's levels are located from its level-0 length (stride) alone,
+// level k's byte offset by a short loop and its valid entries as
+// ceil(nsp / 16^k): tables indexed by a level known only at run time would
+// live in scratch memory.
 struct ST {
   const uint8_t *t;
-  uint32_t off[kMtMaxLevels];  // byte offset of level k
-  uint32_t cnt[kMtMaxLevels];  // valid entries of level k
+  uint32_t stride;  // bytes of level 0 (mt_level_len geometry)
   int levels;
   uint32_t nsp;
-  __device__ __forceinline__ uint8_t at(int k, uint32_t q) const { return t[off[k] + q]; }
+  __device__ __forceinline__ uint32_t off(int k) const {  // byte offset of level k
+    uint32_t o = 0, len = stride;
+    for (int q = 0; q < k; q++) {
+      o += len;
+      len = ((len / 16 + 15) / 16) * 16;
+    }
+    return o;
+  }
+  __device__ __forceinline__ uint32_t cnt(int k) const {  // valid entries of level k
+    return (uint32_t)(((uint64_t)nsp + (1ull << (4 * k)) - 1) >> (4 * k));
+  }
+  __device__ __forceinline__ uint8_t at(int k, uint32_t q) const { return t[off(k) + q]; }
   // 16-bit mask of the valid entries of the block of level k at base
   // (a multiple of 16) whose value is below lim (lim <= 256)
   __device__ __forceinline__ uint32_t below(int k, uint32_t base, uint32_t lim) const {
-    const uint4 v = *(const uint4 *)(t + off[k] + base);
+    const uint4 v = *(const uint4 *)(t + off(k) + base);
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
     uint32_t m = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 16; i++) m |= (uint32_t)(((w[i >> 2] >> (8 * (i & 3))) & 255u) < lim) << i;
-    const uint32_t valid = cnt[k] - base;  // > 0 for every block a query reads
+    const uint32_t valid = cnt(k) - base;  // > 0 for every block a query reads
     return valid >= 16 ? m : m & ((1u << valid) - 1);
   }
   // rightmost (leftmost) entry below lim under level-k entry p, at level 0
+  // (the level loops stay rolled: the code is shared by many call sites)
   __device__ __forceinline__ uint32_t down_last(int k, uint32_t p, uint32_t lim) const {
+#pragma unroll 1
     for (; k > 0; k--) {
       const uint32_t base = 16 * p;
       p = base + 31 - __clz(below(k - 1, base, lim));
@@ -1062,6 +1150,7 @@ struct ST {
     return p;
   }
   __device__ __forceinline__ uint32_t down_first(int k, uint32_t p, uint32_t lim) const {
+#pragma unroll 1
     for (; k > 0; k--) {
       const uint32_t base = 16 * p;
       p = base + __ffs(below(k - 1, base, lim)) - 1;
@@ -1069,8 +1158,9 @@ struct ST {
     return p;
   }
   // largest q <= j0 with SP[q] < lim, or -1 (j0 may be -1)
-  __device__ int last_below(int64_t j0, uint32_t lim) const {
+  __device__ __forceinline__ int last_below(int64_t j0, uint32_t lim) const {
     int64_t idx = j0;
+#pragma unroll 1
     for (int k = 0; k < levels && idx >= 0; k++) {
       const uint32_t base = (uint32_t)idx & ~15u;
       const uint32_t m = below(k, base, lim) & (0xffffu >> (15 - ((uint32_t)idx - base)));
@@ -1080,9 +1170,10 @@ struct ST {
     return -1;
   }
   // smallest q >= j0 with SP[q] < lim, or nsp
-  __device__ uint32_t first_below(uint32_t j0, uint32_t lim) const {
+  __device__ __forceinline__ uint32_t first_below(uint32_t j0, uint32_t lim) const {
     uint32_t idx = j0;
-    for (int k = 0; k < levels && idx < cnt[k]; k++) {
+#pragma unroll 1
+    for (int k = 0; k < levels && idx < cnt(k); k++) {
       const uint32_t base = idx & ~15u;
       const uint32_t m = below(k, base, lim) & (0xffffu << (idx - base)) & 0xffffu;
       if (m) return down_first(k, base + __ffs(m) - 1, lim);
@@ -1091,11 +1182,11 @@ struct ST {
     return nsp;
   }
   // largest q < j with SP[q] < L, or -1
-  __device__ int prev_less(uint32_t j, uint32_t L) const { return last_below((int64_t)j - 1, L); }
+  __device__ __forceinline__ int prev_less(uint32_t j, uint32_t L) const { return last_below((int64_t)j - 1, L); }
   // smallest q > j with SP[q] < L, or nsp
-  __device__ uint32_t next_less(uint32_t j, uint32_t L) const { return first_below(j + 1, L); }
+  __device__ __forceinline__ uint32_t next_less(uint32_t j, uint32_t L) const { return first_below(j + 1, L); }
   // smallest q >= j with SP[q] <= L, or nsp
-  __device__ uint32_t next_leq(uint32_t j, uint32_t L) const { return first_below(j, L + 1); }
+  __device__ __forceinline__ uint32_t next_leq(uint32_t j, uint32_t L) const { return first_below(j, L + 1); }
 };
 
 __device__ __forceinline__ ST st_of(const NuArgs &a) {
@@ -1104,43 +1195,119 @@ __device__ __forceinline__ ST st_of(const NuArgs &a) {
   uint32_t ne = a.scal[SC_NE];
   s.nsp = ne > 0 ? ne - 1 : 0;
   s.levels = min(a.st_levels, kMtMaxLevels);
-  uint32_t off = 0, cnt = s.nsp;
+  s.stride = (uint32_t)a.st_stride;
+  return s;
+}
+
+// A workgroup's window of the split points in LDS: level 0 entries
+// [wb, wb + kSpWin) and their level-1 .. level-3 minima, queried through an
+// ST over window-relative positions (its nsp = the window's valid length).
+// With 4,096 entries each side of the tile, only a split point whose
+// interval reaches further (an LCP of 2 or less for uniform keys) falls back
+// to the global tree.
+constexpr uint32_t kSpTile = 256;                      // split points per workgroup
+constexpr uint32_t kSpHalo = 4096;                     // window entries on each side of the tile
+constexpr uint32_t kSpWin = kSpHalo + kSpTile + kSpHalo;
+constexpr uint32_t kSpL1 = kSpWin / 16;                         // level-1 entries (a multiple of 16)
+constexpr uint32_t kSpL2 = (kSpL1 + 15) / 16;                   // level-2 entries
+constexpr uint32_t kSpL2Pad = (kSpL2 + 15) / 16 * 16;
+constexpr uint32_t kSpL3 = (kSpL2 + 15) / 16;                   // level-3 entries (<= 16)
+constexpr uint32_t kSpWinBytes = kSpWin + kSpL1 + kSpL2Pad + 16;
+static_assert(kSpWin % 256 == 0 && kSpL3 <= 16, "window geometry");
+static_assert(kSpL1 == ((kSpWin / 16 + 15) / 16) * 16 && kSpL2Pad == ((kSpL1 / 16 + 15) / 16) * 16 &&
+                  ((kSpL2Pad / 16 + 15) / 16) * 16 == 16,
+              "the window's levels follow the tree's level lengths");
+
+// Loads the window of the tile starting at t0 (a multiple of kSpTile) into
+// w (kSpWinBytes, 16-byte aligned) and returns its ST.  Every thread of the
+// block calls it (it synchronises the block).
+__device__ __forceinline__ ST st_window(const ST &g, uint32_t t0, uint8_t *w, uint32_t &wb) {
+  wb = t0 >= kSpHalo ? t0 - kSpHalo : 0u;
+  const uint32_t n0 = g.nsp > wb ? min(kSpWin, g.nsp - wb) : 0u;
+  uint8_t *w1 = w + kSpWin, *w2 = w1 + kSpL1, *w3 = w2 + kSpL2Pad;
+  for (uint32_t c = threadIdx.x; c < kSpL1; c += blockDim.x) {
+    // level 0, 16 bytes per chunk (the tree's level 0 is padded to a
+    // multiple of 256 bytes, so whole 16-byte reads stay inside it), and the
+    // chunk's level-1 minimum over the valid entries
+    uint4 v = make_uint4(0, 0, 0, 0);
+    uint32_t mn = 255;
+    if (16 * c < n0) {
+      v = *(const uint4 *)(g.t + wb + 16 * c);
+      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int k = 0; k < kMtMaxLevels; k++) {
-    s.off[k] = off;
-    s.cnt[k] = cnt;
-    off += (uint32_t)mt_level_len(a.st_stride, k);
-    cnt = (cnt + 15) / 16;
+      for (uint32_t i = 0; i < 16; i++)
+        if (16 * c + i < n0) mn = min(mn, (wd[i >> 2] >> (8 * (i & 3))) & 255u);
+    }
+    ((uint4 *)w)[c] = v;
+    w1[c] = (uint8_t)mn;
   }
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < kSpL2Pad; c += blockDim.x) {
+    uint32_t mn = 255;
+    if (c < kSpL2)
+      for (uint32_t i = 0; i < 16; i++) mn = min(mn, (uint32_t)w1[16 * c + i]);
+    w2[c] = (uint8_t)mn;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    uint32_t mn = 255;
+    if (threadIdx.x < kSpL3)
+      for (uint32_t i = 0; i < 16; i++) mn = min(mn, (uint32_t)w2[16 * threadIdx.x + i]);
+    w3[threadIdx.x] = (uint8_t)mn;
+  }
+  __syncthreads();
+  ST s;  // the window's levels follow the tree's geometry with a level 0 of kSpWin bytes
+  s.t = w;
+  s.levels = 4;
+  s.nsp = n0;
+  s.stride = kSpWin;
   return s;
 }
 
 // Per split point: find its branch (the representative is the leftmost split
 // point with the same LCP inside the maximal interval whose LCPs are >= it).
-// bid[j] = 1 for representatives; an exclusive scan turns it into the
-// branch's rank.  Entries up to the launch bound are written (0 past nsp).
+// bcnt[block] = the block's representatives, from which k_init_branches
+// ranks them.  A representative also records its interval's bounding split
+// points (plohi).  The queries run on the workgroup's window of split points
+// in LDS.
 __global__ __launch_bounds__(256) void k_split_reps(NuArgs a, uint32_t ne_max) {
   // the sort needs a fallback pass, or an error is pending: the build waits
   // for the host (which may run it before reading either)
   if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
-  uint32_t j = grid_tid();
-  if (j >= ne_max) return;
+  static_assert(kSpTile == 256, "one split point per thread");
+  __shared__ __align__(16) uint8_t win[kSpWinBytes];
+  const uint32_t j = blockIdx.x * kSpTile + threadIdx.x;
   // the build's dirty-list base: the list does not change until the build's
   // last kernel advances it
   if (j == 0) a.scal[SC_DBASE] = a.P.cnt->n_dirty;
-  ST s = st_of(a);
+  const ST g = st_of(a);
+  uint32_t wb;
+  const ST w = st_window(g, blockIdx.x * kSpTile, win, wb);
   bool rep = false;
-  if (j < s.nsp) {
-    uint32_t L = s.at(0, j);
+  if (j < g.nsp) {
+    const uint32_t jw = j - wb;  // inside the window: j < wb + nsp of the window
+    const uint32_t L = w.at(0, jw);
     uint32_t r = ~0u;
     if (L != 0) {
-      int lo = s.prev_less(j, L);
-      r = s.next_leq((uint32_t)(lo + 1), L);
+      // nearest smaller to the left: in the window, else (window not at the
+      // start) the global tree
+      int lo = w.prev_less(jw, L);
+      lo = lo >= 0 ? (int)wb + lo : wb == 0 ? -1 : g.prev_less(j, L);
+      // the first entry <= L after it: at or before j, so in the window
+      // unless lo is left of it
+      r = (int64_t)lo + 1 >= (int64_t)wb ? wb + w.next_leq((uint32_t)(lo + 1) - wb, L)
+                                         : g.next_leq((uint32_t)(lo + 1), L);
       rep = r == j;
+      if (rep) {
+        uint32_t hi = w.next_less(jw, L);
+        hi = hi < w.nsp ? wb + hi : wb + w.nsp >= g.nsp ? g.nsp : g.next_less(j, L);
+        a.plohi[j] = (uint64_t)(uint32_t)lo | (uint64_t)hi << 32;
+      }
     }
     a.repof[j] = r;
   }
-  a.bid[j] = rep ? 1u : 0u;
+  const uint32_t reps = (uint32_t)__syncthreads_count(rep);
+  if (threadIdx.x == 0) a.bcnt[blockIdx.x] = reps;
 }
 
 // Node slots of this build, all fixed by position (no allocator atomics):
@@ -1168,17 +1335,26 @@ __device__ __forceinline__ void put_dirty(const NuArgs &a, uint32_t slot, ref_t 
     raise_error(a.P.cnt, ERR_POOL, 4);
 }
 
-__global__ __launch_bounds__(256) void k_init_branches(NuArgs a) {
+// Ranks the representatives (the branches, in split-point order: the block's
+// offset from k_split_reps' counts, then a block-local scan), records each
+// rank in bid and the total in SC_NREP_TOT, and initialises the branches.
+__global__ __launch_bounds__(256) void k_init_branches(NuArgs a, int pre) {
   // the sort needs a fallback pass, or an error is pending: the build waits
   // for the host (which may run it before reading either)
   if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
+  __shared__ uint32_t sh[4];
   uint32_t j = grid_tid();
   ST s = st_of(a);
-  if (j >= s.nsp) return;
-  uint32_t L = s.at(0, j);
-  if (L == 0 || a.repof[j] != j) return;
+  const uint32_t L = j < s.nsp ? s.at(0, j) : 0u;
+  const bool rep = L != 0 && a.repof[j] == j;
+  uint32_t btot;
+  const uint32_t base = block_offset(a.bcnt, pre != 0, sh);
+  const uint32_t rank = base + block_exclusive_scan_256<uint32_t>(rep, sh, &btot);
+  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) a.scal[SC_NREP_TOT] = base + btot;
+  if (!rep) return;
+  a.bid[j] = rank;
   Slots q = slots_of(a);
-  uint32_t id = a.base_branch + a.bid[j];
+  uint32_t id = a.base_branch + rank;
   if (id >= a.P.cap_branch) {
     raise_error(a.P.cnt, ERR_POOL, 1);
     return;
@@ -1189,8 +1365,8 @@ __global__ __launch_bounds__(256) void k_init_branches(NuArgs a) {
   for (int k = 0; k < 16; k++) w[k] = 0;
   B.depth = (uint16_t)(L - 1);
   B.dirty = 1;
-  a.bdesc[a.bid[j]] = L - 1;  // the links below count its children
-  put_dirty(a, q.dbase + q.ne + a.bid[j], make_ref(T_BRANCH, id));
+  a.bdesc[rank] = L - 1;  // the links below count its children
+  put_dirty(a, q.dbase + q.ne + rank, make_ref(T_BRANCH, id));
 }
 
 // One more child linked into the build's branch of rank `rank` (the record
@@ -1275,18 +1451,16 @@ __device__ __forceinline__ void install(const NuArgs &a, const GroupInfo &gi, re
 }
 
 // Branches: link each to its parent (through an extension when depths skip).
-__global__ __launch_bounds__(256) void k_link_branches(NuArgs a) {
-  // the sort needs a fallback pass, or an error is pending: the build waits
-  // for the host (which may run it before reading either)
-  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
-  uint32_t j = grid_tid();
+__device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j) {
   ST s = st_of(a);
   if (j >= s.nsp) return;
   uint32_t L = s.at(0, j);
   if (L == 0 || a.repof[j] != j) return;
   Slots q = slots_of(a);
-  int lo = s.prev_less(j, L);
-  uint32_t hi = s.next_less(j, L);
+  // the interval's bounding split points, found by k_split_reps
+  const uint64_t lh = a.plohi[j];
+  const int lo = (int)(uint32_t)lh;
+  const uint32_t hi = (uint32_t)(lh >> 32);
   uint32_t La = lo >= 0 ? s.at(0, (uint32_t)lo) : 0u;
   uint32_t Lb = hi < s.nsp ? s.at(0, hi) : 0u;
   uint32_t e0 = (uint32_t)(lo + 1);
@@ -1341,11 +1515,7 @@ __device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32
   if (size) *size = V.size;
 }
 
-__global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
-  // the sort needs a fallback pass, or an error is pending: the build waits
-  // for the host (which may run it before reading either)
-  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
-  uint32_t e = grid_tid();
+__device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e) {
   ST s = st_of(a);
   Slots q = slots_of(a);
   if (e == 0) {
@@ -1415,6 +1585,17 @@ __global__ __launch_bounds__(256) void k_link_elems(NuArgs a) {
   }
 }
 
+// The links of the build's branches and elements: thread j links the branch
+// whose representative is split point j, if any, and element j.
+__global__ __launch_bounds__(256) void k_link(NuArgs a) {
+  // the sort needs a fallback pass, or an error is pending: the build waits
+  // for the host (which may run it before reading either)
+  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
+  const uint32_t j = grid_tid();
+  link_branch(a, j);
+  link_elem(a, j);
+}
+
 // Existing keys: swap in a new value node (runs before the links, so a leaf
 // that is also split carries the new value into its remainder).
 // The key's value node is rewritten in place (no node is allocated per
@@ -1457,9 +1638,6 @@ hipError_t nu_prepare_keys(const NuArgs &a, void *temp, hipStream_t s) {
   GMPT_LAUNCH_CHECK();
   k_keys_reduce<<<1, kB, 0, s>>>(a, a.ecount, a.grp, nblk(a.n));
   GMPT_LAUNCH_CHECK();
-  hipError_t e =
-      scan_exclusive_u64(a.kpoff, a.kpoff, a.n, (uint64_t *)(a.scal + SC_KTOT), temp, s, a.scal + SC_UNIFORM);
-  if (e != hipSuccess) return e;
   k_keys_pack<<<nblk(a.n), kB, 0, s>>>(a);
   return hipGetLastError();
 }
@@ -1630,10 +1808,15 @@ hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, 
 }
 
 hipError_t nu_unique(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, void *temp, hipStream_t s) {
-  // run[] holds run-start flags; exclusive scan gives each start its run id
-  hipError_t e = scan_exclusive_u32(a.run, a.run, a.n, nullptr, temp, s);
-  if (e != hipSuccess) return e;
-  k_unique<<<nblk(a.n), kB, 0, s>>>(a, sk, sv, a.run);
+  // run[] holds the run starts per block of k_adjacent; with many blocks
+  // they are scanned here, else each block of k_unique adds up those before it
+  const uint32_t nb = nblk(a.n);
+  const bool pre = nb > kSelfSumBlocks;
+  if (pre) {
+    hipError_t e = scan_exclusive_u32(a.run, a.run, nb, nullptr, temp, s);
+    if (e != hipSuccess) return e;
+  }
+  k_unique<<<nb, kB, 0, s>>>(a, sk, sv, a.run, pre ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1723,13 +1906,16 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
   }
   k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a, ne_max);
   GMPT_LAUNCH_CHECK();
-  hipError_t e = scan_exclusive_u32(a.bid, a.bid, ne_max, a.scal + SC_NREP_TOT, temp, s);
-  if (e != hipSuccess) return e;
-  k_init_branches<<<nblk(ne_max), kB, 0, s>>>(a);
+  // the per-block representative counts: scanned here when there are many
+  // blocks, else each block of k_init_branches adds up those before it
+  const bool pre = nblk(ne_max) > kSelfSumBlocks;
+  if (pre) {
+    hipError_t e = scan_exclusive_u32(a.bcnt, a.bcnt, nblk(ne_max), nullptr, temp, s);
+    if (e != hipSuccess) return e;
+  }
+  k_init_branches<<<nblk(ne_max), kB, 0, s>>>(a, pre ? 1 : 0);
   GMPT_LAUNCH_CHECK();
-  k_link_branches<<<nblk(ne_max), kB, 0, s>>>(a);
-  GMPT_LAUNCH_CHECK();
-  k_link_elems<<<nblk(ne_max), kB, 0, s>>>(a);
+  k_link<<<nblk(ne_max), kB, 0, s>>>(a);
   return hipGetLastError();
 }
 
