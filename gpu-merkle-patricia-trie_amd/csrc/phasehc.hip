@@ -834,27 +834,57 @@ __device__ __forceinline__ void keccak_lane(const uint64_t *m, uint32_t nb, uint
   }
 }
 
-// Inner levels above the spread threshold whose nodes are mostly 1-block
-// branches and extensions, encoded and hashed in one launch with the
-// messages in LDS: one wave per block, one node per lane for the
-// permutation, but the encoding's loads spread over the whole wave.
-//  1. Each lane reads its branch's 16 child references and writes the bytes
-//     that depend on the branch alone into its LDS row: list header, 0x80
-//     for each empty slot, 0x80 for slot 16, Keccak padding.  An extension
-//     (a short node over a branch) is written whole by its lane: [compact
-//     key, 0xa0 + the child's digest].
-//  2. The wave's present children (a prefix sum of the lanes' child counts
-//     numbers them) are dealt out to the 64 lanes: each loads its children's
-//     digests -- every load of the wave in flight at once -- and writes
-//     0xa0 + digest at the child's offset in its parent's row.
+// ORs the item 0xa0 + digest (33 bytes; digest words d0, d1 as stored) into
+// a zeroed LDS row at byte offset off: ten dword ORs at aligned addresses,
+// so items of neighbouring children that share a dword need no ordering.
+__device__ __forceinline__ void or_child_item(uint32_t *row32, uint32_t off, const uint4 &d0, const uint4 &d1) {
+  // the item's dwords (byte 0 = 0xa0, bytes 1 .. 32 = the digest)
+  uint32_t it[10];
+  it[0] = 0xa0u | (d0.x << 8);
+  it[1] = __builtin_amdgcn_alignbyte(d0.y, d0.x, 3);
+  it[2] = __builtin_amdgcn_alignbyte(d0.z, d0.y, 3);
+  it[3] = __builtin_amdgcn_alignbyte(d0.w, d0.z, 3);
+  it[4] = __builtin_amdgcn_alignbyte(d1.x, d0.w, 3);
+  it[5] = __builtin_amdgcn_alignbyte(d1.y, d1.x, 3);
+  it[6] = __builtin_amdgcn_alignbyte(d1.z, d1.y, 3);
+  it[7] = __builtin_amdgcn_alignbyte(d1.w, d1.z, 3);
+  it[8] = d1.w >> 24;
+  it[9] = 0;
+  const uint32_t r = off & 3u, w0 = off >> 2;
+  // shifted by r bytes: dword m takes item bytes [4m - r, 4m - r + 4)
+#pragma unroll
+  for (int m = 0; m < 10; m++) {
+    const uint32_t prev = m ? it[m - 1] : 0u;
+    const uint32_t v = r ? __builtin_amdgcn_alignbyte(it[m], prev, 4 - r) : it[m];
+    if (v) atomicOr(row32 + w0 + m, v);
+  }
+}
+
+// Inner levels above the spread threshold whose nodes are branches of at
+// most NB Keccak blocks and extensions (all but a sixteenth of them),
+// encoded and hashed in one launch with the messages in LDS: one wave per
+// block, one node per lane for the permutation, but the encoding's memory
+// work spread over the whole wave:
+//  1. Each lane reads its branch's 16 child references (one 64-byte load),
+//     keeps them in LDS and writes the bytes that depend on the branch alone
+//     into its LDS row: list header, 0x80 for each empty slot, 0x80 for slot
+//     16, Keccak padding.  An extension (a short node over a branch) is
+//     written whole by its lane: [compact key, 0xa0 + the child's digest].
+//  2. The wave's children are dealt out to the lanes: for rows of 1 or 2
+//     blocks (a few children per node) the present children, numbered by a
+//     prefix sum of the lanes' child counts, round robin; for rows of 4
+//     blocks all 64 x 16 (node, slot) pairs, 16 per lane, four at a time so
+//     their loads overlap.  Each present child's digest is loaded and ORed,
+//     as 0xa0 + digest, into its parent's row at the child's offset (the
+//     rank of its slot among the node's present children).
 //  3. Each lane absorbs its row and permutes.
 // A node whose message would be taller than the rows (NB blocks), a branch
 // with a value in slot 16 or an embedded child, and any other short node
 // take the generic encoder and their scratch slot.  Blocks run in reverse
 // order: the level's nodes are sorted by block count with the extensions
-// last, so the few multi-block nodes start first and finish under the bulk.
-// Entries [skip_lo, skip_hi) are hashed elsewhere (by kernels on a second
-// stream) and left alone here.
+// last, so the larger nodes and extensions start first and finish under the
+// bulk.  Entries [skip_lo, skip_hi) are hashed elsewhere (by kernels on a
+// second stream) and left alone here.
 template <int NB>
 __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__restrict__ list,
                                                       const uint64_t *__restrict__ slot, uint32_t n,
@@ -863,9 +893,11 @@ __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__re
   if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   constexpr uint32_t RS = 17 * NB + 1;  // row stride in words (odd: lane-strided reads spread over the banks)
   __shared__ uint64_t rows[64 * RS];
+  constexpr bool kDense = NB >= 4;      // many children per node: (node, slot) pairs dealt out directly
+  __shared__ ref_t cref[kDense ? 64 * 16 : 1];  // the lanes' branches' child references (dense dealing)
   __shared__ ref_t nref[64];            // the lanes' nodes
-  __shared__ uint32_t nbase[64];        // first child number of each node
-  __shared__ uint32_t ninfo[64];        // child mask | header length << 16
+  __shared__ uint32_t nbase[64];        // first child number of each node (compact dealing)
+  __shared__ uint32_t ninfo[64];        // child mask (0 unless the row path encodes the node) | header length << 16
   __shared__ uint32_t bad[64];          // 1: the node takes the generic encoder
   const uint32_t lane = threadIdx.x;
   const uint32_t i = (gridDim.x - 1 - blockIdx.x) * 64 + lane;
@@ -881,6 +913,7 @@ __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__re
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint4 v = cw[q];
+      if (kDense) ((uint4 *)(cref + 16 * lane))[q] = v;
       mask |= (v.x != REF_NULL ? 1u : 0u) << (4 * q);
       mask |= (v.y != REF_NULL ? 2u : 0u) << (4 * q);
       mask |= (v.z != REF_NULL ? 4u : 0u) << (4 * q);
@@ -895,17 +928,17 @@ __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__re
   } else if (active && ref_type(r) == T_SHORT) {
     ext = is_direct_ext(P.shrt[ref_idx(r)]);
   }
-  const uint32_t k = fast ? (uint32_t)__popc(mask) : 0u;
-  const uint32_t incl = wave_inclusive_scan(k);
-  const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+  const uint32_t kc = fast ? (uint32_t)__popc(mask) : 0u;
+  const uint32_t incl = wave_inclusive_scan(kc);
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);  // the wave's row-path children
   nref[lane] = r;
-  nbase[lane] = incl - k;
-  ninfo[lane] = mask | (hn << 16);
+  nbase[lane] = incl - kc;
+  ninfo[lane] = (fast ? mask : 0u) | (hn << 16);
   bad[lane] = fast ? 0u : 1u;
   uint64_t *row = rows + lane * RS;
   if (fast)
     for (uint32_t w = 0; w < 17 * nb; w++) row[w] = 0;
-  __syncthreads();  // rows zeroed; nodes, bases, masks visible to the wave
+  __syncthreads();  // rows zeroed; child references, masks visible to the wave
   if (fast) {
     uint8_t *rb = (uint8_t *)row;
     for (uint32_t q = 0; q < hn; q++) rb[q] = (uint8_t)(hdr >> (8 * q));
@@ -945,36 +978,72 @@ __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__re
       fast = true;
     }
   }
-  // the wave's children, numbered node by node, dealt out round robin
-  for (uint32_t t = lane; t < total; t += 64) {
-    uint32_t lo = 0, hi = 63;  // the node: the last with nbase <= t (it has children)
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) / 2;
-      if (nbase[mid] <= t)
-        lo = mid;
-      else
-        hi = mid - 1;
+  if (!kDense) {
+    // the wave's present children, numbered node by node, dealt out round
+    // robin (a few per lane)
+    for (uint32_t t = lane; t < total; t += 64) {
+      uint32_t lo = 0, hi = 63;  // the node: the last with nbase <= t (it has children)
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (nbase[mid] <= t)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const uint32_t j = lo, info = ninfo[j];
+      const uint32_t o = t - nbase[j];  // the child's rank among the node's present children
+      uint32_t mm = info & 0xffffu;     // its slot: the o-th set bit of the mask
+      for (uint32_t q = 0; q < o; q++) mm &= mm - 1;
+      const uint32_t sl = __ffs(mm) - 1;
+      const ref_t c = P.branch[ref_idx(nref[j])].child[sl];
+      const bool sh = ref_type(c) == T_SHORT;
+      const uint8_t *h = sh ? P.shrt[ref_idx(c)].hash : P.branch[ref_idx(c)].hash;
+      const uint4 d0 = ((const uint4 *)h)[0], d1 = ((const uint4 *)h)[1];
+      const uint32_t hs = sh ? P.shrt[ref_idx(c)].hash_size : P.branch[ref_idx(c)].hash_size;
+      if (hs != 32) {
+        bad[j] = 1;  // an embedded child: the node's own lane encodes it generically
+        continue;
+      }
+      or_child_item((uint32_t *)(rows + j * RS), (info >> 16) + 33 * o + (sl - o), d0, d1);
     }
-    const uint32_t j = lo, info = ninfo[j];
-    const uint32_t o = t - nbase[j];  // the child's rank among the node's present children
-    uint32_t s = 0;                   // its slot: the o-th set bit of the mask
-    uint32_t mm = info & 0xffffu;
-    for (uint32_t q = 0; q < o; q++) mm &= mm - 1;
-    s = __ffs(mm) - 1;
-    const ref_t c = P.branch[ref_idx(nref[j])].child[s];
-    const bool sh = ref_type(c) == T_SHORT;
-    const uint8_t *h = sh ? P.shrt[ref_idx(c)].hash : P.branch[ref_idx(c)].hash;
-    const uint4 d0 = ((const uint4 *)h)[0], d1 = ((const uint4 *)h)[1];
-    const uint32_t hs = sh ? P.shrt[ref_idx(c)].hash_size : P.branch[ref_idx(c)].hash_size;
-    if (hs != 32) {
-      bad[j] = 1;  // an embedded child: the node's own lane encodes it generically
-      continue;
-    }
-    uint8_t *rb = (uint8_t *)(rows + j * RS) + (info >> 16) + 33 * o + (s - o);
-    rb[0] = 0xa0;
-    const uint32_t dw[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+  }
+  // (dense) the wave's (node, slot) pairs: pair p = node p / 16, slot p % 16;
+  // lane l takes pairs l + 64 q, four at a time so their digest loads overlap
+#pragma unroll 1
+  for (uint32_t q0 = 0; kDense && q0 < 16; q0 += 4) {
+    uint4 d0[4], d1[4];
+    uint32_t hs[4], off[4], jj[4];
+    bool has[4];
 #pragma unroll
-    for (int q = 0; q < 32; q++) rb[1 + q] = (uint8_t)(dw[q >> 2] >> (8 * (q & 3)));
+    for (int u = 0; u < 4; u++) {
+      const uint32_t p = lane + 64 * (q0 + u);
+      const uint32_t j = p >> 4, s = p & 15u;
+      const uint32_t info = ninfo[j];
+      const uint32_t m = info & 0xffffu;
+      const uint32_t o = __popc(m & ((1u << s) - 1u));  // the child's rank among the node's present children
+      has[u] = (m >> s) & 1u;
+      jj[u] = j;
+      off[u] = (info >> 16) + 33 * o + (s - o);
+      hs[u] = 0;
+      d0[u] = d1[u] = make_uint4(0, 0, 0, 0);
+      if (has[u]) {
+        const ref_t c = cref[16 * j + s];
+        const bool sh = ref_type(c) == T_SHORT;
+        const uint8_t *h = sh ? P.shrt[ref_idx(c)].hash : P.branch[ref_idx(c)].hash;
+        d0[u] = ((const uint4 *)h)[0];
+        d1[u] = ((const uint4 *)h)[1];
+        hs[u] = sh ? P.shrt[ref_idx(c)].hash_size : P.branch[ref_idx(c)].hash_size;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (!has[u]) continue;
+      if (hs[u] != 32) {
+        bad[jj[u]] = 1;  // an embedded child: the node's own lane encodes it generically
+        continue;
+      }
+      or_child_item((uint32_t *)(rows + jj[u] * RS), off[u], d0[u], d1[u]);
+    }
   }
   __syncthreads();  // every row complete
   if (!active) return;
@@ -1563,18 +1632,26 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     if (perm_end) return hipEventRecord(perm_end, s);
     return hipSuccess;
   }
-  // a level of 1- and 2-block branches and extensions (all but a sixteenth
-  // of its nodes): one launch with the messages in LDS, the rest generic
-  // inside it; rows of one block when no node needs two
-  if (!leaves && cls && 16ull * ((uint64_t)cls[0] + cls[1] + cls[kExtClass]) >= 15ull * n) {
-    // the few nodes of 3 or more blocks (classes 2 .. 6, between the 2-block
-    // class and the extensions) would take the generic encoder and hold the
-    // launch open long after the bulk: with a second stream they are encoded
-    // by 16-lane groups and permuted there, concurrently with the bulk
+  // a level of branches of at most 4 blocks and extensions (all but a
+  // sixteenth of its nodes): one launch with the messages in LDS, the rest
+  // generic inside it; rows as tall as the level's tallest class needs (1, 2
+  // or 4 blocks: taller rows leave fewer waves per CU)
+  const uint32_t c01 = cls ? cls[0] + cls[1] : 0u, c23 = cls ? cls[2] + cls[3] : 0u;
+  const uint32_t cext = cls ? cls[kExtClass] : 0u;
+  const int rows_nb = !cls || leaves ? 0
+                      : 16ull * ((uint64_t)c01 + cext) >= 15ull * n       ? (cls[1] ? 2 : 1)
+                      : 16ull * ((uint64_t)c01 + c23 + cext) >= 15ull * n ? 4
+                                                                            : 0;
+  if (rows_nb) {
+    // the few nodes taller than the rows (classes rows_nb .. 6, between the
+    // row classes and the extensions) would take the generic encoder and hold
+    // the launch open long after the bulk: with a second stream they are
+    // encoded by 16-lane groups and permuted there, concurrently with the bulk
+    const uint32_t cov = std::min(n, rows_nb == 4 ? c01 + c23 : rows_nb == 2 ? c01 : cls[0]);
     uint32_t lo = 0, hi = 0;
-    const uint32_t multi = n - std::min(n, cls[0] + cls[1] + cls[kExtClass]);
+    const uint32_t multi = n - std::min(n, cov + cext);
     if (side && multi) {
-      lo = std::min(n, cls[0] + cls[1]);
+      lo = cov;
       hi = lo + multi;
       hipError_t e;
       if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
@@ -1586,7 +1663,9 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
       GMPT_LAUNCH_CHECK();
       if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
     }
-    if (cls[1])
+    if (rows_nb == 4)
+      k_hc_inner_wave<4><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
+    else if (rows_nb == 2)
       k_hc_inner_wave<2><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
     else
       k_hc_inner_wave<1><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
