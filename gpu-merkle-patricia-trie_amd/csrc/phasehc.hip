@@ -1360,11 +1360,13 @@ struct SpreadLane {
 __device__ __forceinline__ uint32_t bperm(uint32_t addr, uint32_t v) {
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)v);
 }
-// xor over the lane's 8-lane half-row (all eight lanes get it)
+// xor over the lane's 8-lane half-row (all eight lanes get it); every
+// source lane lies in the lane's own row, so each step is one xor with a
+// DPP operand (the update form lets the compiler fold the move into it)
 __device__ __forceinline__ uint32_t xor8(uint32_t v) {
-  v ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1, 0, 3, 2]
-  v ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2, 3, 0, 1]
-  v ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);   // quad_perm [1, 0, 3, 2]
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);   // quad_perm [2, 3, 0, 1]
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true);  // row_half_mirror
   return v;
 }
 

@@ -937,8 +937,14 @@ __global__ __launch_bounds__(256) void k_mt_top(NuArgs a, int k0) {
     const size_t len = mt_level_len(a.st_stride, k - 1);
     const uint32_t nout = (cnt + 15) / 16;
     for (uint32_t b = threadIdx.x; b < nout; b += blockDim.x) {
+      // the entry's 16 children in one load (levels are 16-byte multiples
+      // at 16-byte aligned offsets); children past cnt do not count
+      const uint4 v = *(const uint4 *)(a.st + off + 16 * (size_t)b);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
       uint32_t mn = 255;
-      for (uint32_t j = 16 * b; j < 16 * b + 16 && j < cnt; j++) mn = min(mn, (uint32_t)a.st[off + j]);
+#pragma unroll
+      for (uint32_t i = 0; i < 16; i++)
+        if (16 * b + i < cnt) mn = min(mn, (w[i >> 2] >> (8 * (i & 3))) & 255u);
       a.st[off + len + b] = (uint8_t)mn;
     }
     __syncthreads();  // the level is complete for the next one
