@@ -1331,24 +1331,29 @@ __global__ __launch_bounds__(256) void k_hc_ext_direct(Pools P, const ref_t *__r
 // three-step DPP xor butterfly (quad_perm, quad_perm, row_half_mirror) with
 // no LDS traffic; the parities of columns x - 1 and x + 1, and pi with chi's
 // two row neighbours, are gathered with ds_bpermute: 10 permutes per round.
+// A padding lane takes its column parities from lane 40 (a lane of the
+// all-padding group 5, whose parity is zero), does not rotate, and gathers
+// pi and chi's three sources from itself: it stays zero with no masking.
 struct SpreadLane {
-  uint32_t valid, iota, rho, cm, cp, s0, s1, s2;  // cm .. s2: source byte addresses for ds_bpermute
+  uint32_t iota, flip, t, cm, cp, s0, s1, s2;  // cm .. s2: source byte addresses for ds_bpermute
   // pi moves A[x', y'] to B[y', 2x' + 3y']: B[x, y] comes from A[3 (y - 3x) mod 5, x]
   static __device__ __forceinline__ uint32_t pi_src(uint32_t x, uint32_t y) {
     return 8 * ((3 * (y + 5 - (3 * x) % 5)) % 5) + x;
   }
   __device__ __forceinline__ explicit SpreadLane(uint32_t lane) {
-    const uint32_t x0 = lane >> 3, y0 = lane & 7;
-    const bool ok = x0 < 5 && y0 < 5;
-    const uint32_t x = ok ? x0 : 0u, y = ok ? y0 : 0u;
-    valid = ok ? ~0u : 0u;
+    const uint32_t x = lane >> 3, y = lane & 7;
+    const bool ok = x < 5 && y < 5;
     iota = lane == 0 ? ~0u : 0u;
-    rho = (uint32_t)kKeccak.rho[x + 5 * y];
-    cm = 4 * (8 * ((x + 4) % 5) + y);
-    cp = 4 * (8 * ((x + 1) % 5) + y);
-    s0 = 4 * pi_src(x, y);
-    s1 = 4 * pi_src((x + 1) % 5, y);
-    s2 = 4 * pi_src((x + 2) % 5, y);
+    // rho as two funnel shifts by t = (32 - r) mod 32 of the halves, swapped
+    // first when flip: rotations by 0 and 32 need no other case
+    const uint32_t rho = ok ? (uint32_t)kKeccak.rho[x + 5 * y] : 0u, rr = rho & 31;
+    flip = (rho >= 32) != (rr == 0) ? ~0u : 0u;
+    t = (32 - rr) & 31;
+    cm = ok ? 4 * (8 * ((x + 4) % 5) + y) : 4 * 40;
+    cp = ok ? 4 * (8 * ((x + 1) % 5) + y) : 4 * 40;
+    s0 = ok ? 4 * pi_src(x, y) : 4 * lane;
+    s1 = ok ? 4 * pi_src((x + 1) % 5, y) : 4 * lane;
+    s2 = ok ? 4 * pi_src((x + 2) % 5, y) : 4 * lane;
   }
   // message word this lane absorbs (x + 5y < 17), or -1
   __device__ __forceinline__ static int word_of(uint32_t lane) {
@@ -1380,16 +1385,15 @@ __device__ __forceinline__ void keccak_spread(uint32_t &l, uint32_t &h, const Sp
     l = xor3(l, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
     h = xor3(h, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
     // rho: rotate this lane's word by its own offset
-    const uint32_t rr = c.rho & 31;
-    const uint32_t a = c.rho >= 32 ? h : l, b = c.rho >= 32 ? l : h;
-    const uint32_t nl = rr ? __builtin_amdgcn_alignbit(a, b, 32 - rr) : a;
-    const uint32_t nh = rr ? __builtin_amdgcn_alignbit(b, a, 32 - rr) : b;
+    const uint32_t a = c.flip ? h : l, b = c.flip ? l : h;
+    const uint32_t nl = __builtin_amdgcn_alignbit(a, b, c.t);
+    const uint32_t nh = __builtin_amdgcn_alignbit(b, a, c.t);
     // pi + chi: B[x, y] ^ (~B[x + 1, y] & B[x + 2, y])
     l = chi3(bperm(c.s0, nl), bperm(c.s1, nl), bperm(c.s2, nl));
     h = chi3(bperm(c.s0, nh), bperm(c.s1, nh), bperm(c.s2, nh));
-    // iota (unrolled: the constants are immediates); padding lanes back to zero
-    l = (l ^ ((uint32_t)kKeccak.rc[ir] & c.iota)) & c.valid;
-    h = (h ^ ((uint32_t)(kKeccak.rc[ir] >> 32) & c.iota)) & c.valid;
+    // iota (unrolled: the constants are immediates)
+    l ^= (uint32_t)kKeccak.rc[ir] & c.iota;
+    h ^= (uint32_t)(kKeccak.rc[ir] >> 32) & c.iota;
   }
 }
 
