@@ -99,6 +99,53 @@ __global__ __launch_bounds__(256) void k_scan_down(const T *__restrict__ in, T *
   if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = off + tot;
 }
 
+// A scan of at most kScanSmallTiles tiles in one workgroup: each tile as
+// k_scan_down scans it, offset by the tiles before (one launch in place of
+// the reduce + scan pair).
+constexpr size_t kScanSmallTiles = 4;
+template <typename T>
+__global__ __launch_bounds__(256) void k_scan_small(const T *__restrict__ in, T *__restrict__ out, size_t n,
+                                                    T *__restrict__ total, const uint32_t *skip,
+                                                    const uint32_t *range, uint32_t ntiles) {
+  if (skip && *skip) return;  // uniform over the grid
+  __shared__ T tile[kScanTile];
+  __shared__ T sh[4];
+  const ScanRange rg(range, n);
+  T off = 0;
+  for (uint32_t b = 0; b < ntiles; b++) {
+    const size_t base = (size_t)b * kScanTile;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      const size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
+      tile[i * kScanBlock + threadIdx.x] = rg.in(j) ? in[j] : T(0);
+    }
+    __syncthreads();
+    T v[kScanItems];
+    T acc = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      v[i] = tile[threadIdx.x * kScanItems + i];
+      acc += v[i];
+    }
+    T tot;
+    T pre = block_exclusive_scan_256<T>(acc, sh, &tot) + off;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      tile[threadIdx.x * kScanItems + i] = pre;
+      pre += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      const size_t j = base + (size_t)i * kScanBlock + threadIdx.x;
+      if (rg.in(j)) out[j] = tile[i * kScanBlock + threadIdx.x];
+    }
+    __syncthreads();  // the tile buffer is refilled next
+    off += tot;
+  }
+  if (total && threadIdx.x == 0) *total = off;
+}
+
 template <typename T>
 size_t scan_temp_elems(size_t n) {
   size_t e = 0;
@@ -117,8 +164,8 @@ hipError_t scan_impl(const T *in, T *out, size_t n, T *total, T *temp, hipStream
     return hipSuccess;
   }
   size_t nb = (n + kScanTile - 1) / kScanTile;
-  if (nb == 1) {
-    k_scan_down<T><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, nullptr, total, skip, range);
+  if (nb <= kScanSmallTiles) {  // one workgroup, tile after tile
+    k_scan_small<T><<<1, kScanBlock, 0, s>>>(in, out, n, total, skip, range, (uint32_t)nb);
     return hipGetLastError();
   }
   T *sums = temp;
