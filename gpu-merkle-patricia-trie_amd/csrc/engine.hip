@@ -522,10 +522,11 @@ int Forest::sort_unique(NuArgs &a, uint32_t trie_bits, void *temp, int pass, int
     if (pop >= 1 && pop <= 56) vmask = m;  // saves at least one 8-bit digit
   }
   HIP_TRY(nu_sort(a, nwords, trie_bits, temp, stream_, &sk, &sv, pass == 1, cur, vmask, &sk_mask));
-  HIP_TRY(nu_adjacent(a, sk, sv, pass == 2 ? 1 : 0, stream_, sk_mask));
   // the sorted keys are the plain windows unless the whole keys were sorted
   // or the windows were compressed to their varying bits
   a.u_win_ok = pass != 2 && sk_mask == 0;
+  if (pass > 0) HIP_TRY(nu_reset_dup(a, stream_));
+  HIP_TRY(nu_adjacent(a, sk, sv, pass == 2 ? 1 : 0, stream_, sk_mask));
   HIP_TRY(nu_unique(a, sk, sv, temp, stream_));
   return 0;
 }

@@ -94,11 +94,11 @@ int Forest::capture_rebuild(const BatchDesc &b) {
   uint64_t sk_mask = 0;
   int cur = 0;
   if ((e = nu_prepare_keys(a, temp, stream_)) != hipSuccess ||
-      (e = nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv, false, &cur, 0, &sk_mask)) != hipSuccess ||
-      (e = nu_adjacent(a, sk, sv, 0, stream_, sk_mask)) != hipSuccess)
+      (e = nu_sort(a, 0, trie_bits, temp, stream_, &sk, &sv, false, &cur, 0, &sk_mask)) != hipSuccess)
     return fail(-1);
   a.u_win_ok = sk_mask == 0;
-  if ((e = nu_unique(a, sk, sv, temp, stream_)) != hipSuccess ||
+  if ((e = nu_adjacent(a, sk, sv, 0, stream_, sk_mask)) != hipSuccess ||
+      (e = nu_unique(a, sk, sv, temp, stream_)) != hipSuccess ||
       (e = nu_descend_and_group(a, temp, stream_, true)) != hipSuccess ||
       (e = nu_emit(a, temp, stream_, true)) != hipSuccess || (e = nu_build(a, temp, stream_, true)) != hipSuccess)
     return fail(-1);

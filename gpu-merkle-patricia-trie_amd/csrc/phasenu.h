@@ -124,7 +124,8 @@ enum {
   SC_NREP = 9, SC_DBASE = 10, SC_NREP_TOT = 11,
   SC_WOR = 12 /* u64: OR of the windows */, SC_WAND = 14 /* u64: AND of the windows */,
   SC_UNIFORM = 16 /* 1: every key has key 0's length (packed by the validation pass) */,
-  SC_NFRESH = 17 /* LockNU: keys the insert creates */, SC_COUNT = 20
+  SC_NFRESH = 17 /* LockNU: keys the insert creates */,
+  SC_DUP = 18 /* nonzero: the sorted batch holds equal keys (set by k_adjacent) */, SC_COUNT = 20
 };
 
 // Zeroes the call's device scalars (SC_W starts at all ones for its minimum).
@@ -148,6 +149,9 @@ hipError_t nu_window_mask(const NuArgs &a, const uint64_t *sk, hipStream_t s);
 hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s,
                        uint64_t sk_mask = 0);
 hipError_t nu_unique(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, void *temp, hipStream_t s);
+// Before a repeated sort pass (an eager fallback): clears SC_DUP, which the
+// pass's k_adjacent sets again.
+hipError_t nu_reset_dup(const NuArgs &a, hipStream_t s);
 // Descends every unique key and groups them by stop point; sets SC_MERGE when
 // a split group is interrupted by keys below the split node (nu_regroup).
 // empty: every trie of the forest is empty (the host knows it), so the keys

@@ -571,6 +571,17 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
   }
   a.lcp[j] = l;
   a.flags[j] = fl;
+  // the unique-order entries for a batch with no equal keys, where key j is
+  // unique key j (k_unique rewrites them, compacted, when SC_DUP is set)
+  a.u_lcp[j] = l;
+  a.u_first[j] = (fl >> 1) & 1;
+  a.u_trie[j] = tb;
+  a.u_idx[j] = ib;
+  if (a.skref) {
+    a.u_kref[j] = rb;
+    if (a.u_win_ok) a.u_win[j] = sk[j];
+  }
+  if (fl & 1) atomicOr(&a.scal[SC_DUP], 1u);
   }
   // the block's run starts (keys that are no duplicate of the one before):
   // k_unique numbers the runs from these counts
@@ -583,6 +594,11 @@ __global__ __launch_bounds__(256) void k_adjacent(NuArgs a, const uint64_t *sk, 
 // key, the last position the surviving (last-written) input index.
 __global__ __launch_bounds__(256) void k_unique(NuArgs a, const uint64_t *sk, const uint32_t *sv,
                                                 const uint32_t *starts, int pre) {
+  // no equal keys: k_adjacent wrote the unique-order entries in place
+  if (a.scal[SC_DUP] == 0) {  // uniform over the grid
+    if (grid_tid() == 0) a.scal[SC_M_UNIQ] = a.n;
+    return;
+  }
   __shared__ uint32_t sh[4];
   uint32_t j = grid_tid();
   const bool first = j < a.n && (a.flags[j] & 1) == 0;
@@ -1810,6 +1826,15 @@ hipError_t nu_sort(NuArgs &a, int full, uint32_t trie_bits, void *temp, hipStrea
 hipError_t nu_adjacent(const NuArgs &a, const uint64_t *sk, const uint32_t *sv, int full, hipStream_t s,
                        uint64_t sk_mask) {
   k_adjacent<<<nblk(a.n), kB, 0, s>>>(a, sk, sv, full, sk_mask);
+  return hipGetLastError();
+}
+
+namespace {
+__global__ void k_reset_dup(uint32_t *scal) { scal[SC_DUP] = 0; }
+}  // namespace
+
+hipError_t nu_reset_dup(const NuArgs &a, hipStream_t s) {
+  k_reset_dup<<<1, 1, 0, s>>>(a.scal);
   return hipGetLastError();
 }
 

@@ -869,3 +869,16 @@ def test_index_pairs_sharing_one_key(gmpt, workloads, oracle, mode, where):
         db.free()
     assert t.get_root_hash() == oracle.root_of(b)
     assert t.get_values(b.subset([0])) == [b.value(n - 1)]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_of_more_than_4096_blocks(gmpt, workloads, oracle, mode):
+    """1,100,000 keys in one batch: more than 4,096 blocks of 256, so the
+    run numbering and the branch ranks take their block counts through a
+    device scan instead of each block adding up the counts before it, and
+    the split-point queries leave their LDS window for the widest
+    intervals; the root equals the oracle's."""
+    b = _rand_batch(workloads, 1101, 1_100_000, 1, 16)
+    t = _build(gmpt, b, mode)
+    assert t.get_root_hash() == oracle.root_of(b)
+    t.close()
