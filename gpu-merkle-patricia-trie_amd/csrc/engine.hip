@@ -584,6 +584,7 @@ int Forest::puts_2phase(const BatchDesc &b, void **targets, int *n_targets) {
     }
     for (int run = 0;; run++) {
       if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return r;
+      a.e_ident = empty && !has_tid ? 1u : 0u;
       HIP_TRY(nu_prepare_keys(a, temp, stream_));
       int cur = 0;
       for (int pass = 0; pass >= 0;) {

@@ -89,6 +89,7 @@ int Forest::capture_rebuild(const BatchDesc &b) {
   NuArgs a;
   void *temp = nullptr;
   if ((r = carve_nu(a, d, vbase, has_tid, &temp))) return fail(r);
+  a.e_ident = has_tid ? 0u : 1u;  // the pass builds into emptied tries
   uint64_t *sk = nullptr;
   uint32_t *sv = nullptr;
   uint64_t sk_mask = 0;

@@ -86,6 +86,8 @@ struct NuArgs {
   // E elements (M <= 2m)
   uint32_t *e_src;       // bit31: virtual (short node idx) else unique index
   uint32_t *e_grp;
+  uint32_t e_ident;      // 1: a build into one empty trie, where element e is unique key e of group 0
+                         // (e_src and e_grp are then not written)
   uint8_t *st;           // minimum tree over the split points (mt_* below); level 0 = split-point lcp+1
   size_t st_stride;      // bytes of level 0 (a multiple of 256)
   int st_levels;

@@ -892,25 +892,17 @@ __device__ __forceinline__ void mt_block_levels(const NuArgs &a, uint32_t sp) {
     a.st[off1 + mt_level_len(a.st_stride, 1) + blockIdx.x] = (uint8_t)min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
 }
 
-// Element i of a build into empty tries: its stop, group and element
-// entries; returns split point i (between elements i and i + 1: 0 where the
-// next element starts another trie), or 255 for the last element.
+// Element i of a build into empty tries: its group and element entries;
+// returns split point i (between elements i and i + 1: 0 where the next
+// element starts another trie), or 255 for the last element.  (The stops,
+// group numbers and merge targets the descent writes for other builds are
+// read by no later kernel of this one; a single trie's element entries are
+// the identity, e_ident.)
 __device__ __forceinline__ uint32_t empty_element(const NuArgs &a, uint32_t i, uint32_t m, int multi) {
   const uint32_t t = a.u_trie[i];
   if (t >= a.ntries) raise_error(a.P.cnt, ERR_BAD_KEY, a.u_idx[i]);
   const bool start = i == 0 || (multi && a.u_trie[i - 1] != t);
   const uint32_t g = multi ? a.grp[i] + (start ? 1u : 0u) : 1u;  // 1-based
-  a.grp[i] = g;
-  Stop st;
-  st.stop = REF_NULL;
-  st.attach = REF_NULL;
-  st.base = 0;
-  st.div = 0;
-  st.kind = STOP_ROOT_EMPTY;
-  st.slot = 0;
-  st.less = 0;
-  st.pad = 0;
-  a.stop[i] = st;
   if (start) {
     GroupInfo gi;
     gi.attach = REF_NULL;
@@ -920,10 +912,11 @@ __device__ __forceinline__ uint32_t empty_element(const NuArgs &a, uint32_t i, u
     gi.slot = 0;
     gi.has_split = 0;
     a.ginfo[g - 1] = gi;
-    a.merge[g - 1] = g - 1;
   }
-  a.e_src[i] = i;
-  a.e_grp[i] = g - 1;
+  if (!a.e_ident) {
+    a.e_src[i] = i;
+    a.e_grp[i] = g - 1;
+  }
   if (i + 1 == m) {
     a.scal[SC_NGROUP] = g;
     a.scal[SC_NE] = m;
@@ -1415,7 +1408,7 @@ struct ElemKey {
 
 __device__ __forceinline__ ElemKey elem_key(const NuArgs &a, uint32_t e) {
   ElemKey k;
-  uint32_t src = a.e_src[e];
+  uint32_t src = a.e_ident ? e : a.e_src[e];
   if (src & 0x80000000u) {
     const Short &S = a.P.shrt[src & 0x7fffffffu];
     k.off = S.key_off;
@@ -1491,7 +1484,7 @@ __device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j) {
   bool is_root = La == 0 && Lb == 0;
   GroupInfo gi;
   if (is_root) {
-    gi = a.ginfo[a.e_grp[e0]];
+    gi = a.ginfo[a.e_ident ? 0u : a.e_grp[e0]];
     lo_depth = gi.base;
   } else {
     uint32_t p = La >= Lb ? (uint32_t)lo : hi;
@@ -1556,7 +1549,7 @@ __device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e) {
   bool is_root = La == 0 && Lb == 0;
   GroupInfo gi;
   if (is_root) {
-    gi = a.ginfo[a.e_grp[e]];
+    gi = a.ginfo[a.e_ident ? 0u : a.e_grp[e]];
     start = gi.base;
   } else {
     uint32_t p = La >= Lb ? e - 1 : e;
