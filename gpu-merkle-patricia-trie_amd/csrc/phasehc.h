@@ -12,6 +12,8 @@ constexpr uint32_t kHcBins = kHcLevels * kHcClasses;
 constexpr uint32_t kHcSpreadMaxDefault = 1024;
 // Extensions per inner level that get their own (leaf word builder) kernel.
 constexpr uint32_t kExtDirectMin = 16384;
+// Spread-layout levels of at least this many nodes put two states in a wave.
+constexpr uint32_t kSpread2Min = 2048;
 
 // Classification results, read back by the host in one copy.
 struct HcMeta {

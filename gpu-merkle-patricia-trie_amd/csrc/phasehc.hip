@@ -1462,6 +1462,143 @@ __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restri
   if (lane == 0) node_done(P, r, 32);
 }
 
+// Two states per wave, one per 32-lane half (mid-size levels: one
+// ds_bpermute serves both states, halving the permutes per node).  In a
+// half, A[x, y] of columns 0 .. 3 sits in lane 8x + y and column 4's word y
+// in lane {5, 6, 7, 13, 14}[y] (the first row's spare lanes); the other
+// seven lanes are padding, which may hold anything: they are masked out of
+// the column parities and are no lane's source.  Columns 0 .. 3 take their
+// parity from an xor butterfly over the masked 8-lane group, column 4 from
+// one over the first 16-lane row; lane 5 of a half holds C[4], lane 8x
+// C[x], for the neighbour gathers.
+struct SpreadLane2 {
+  uint32_t m03, m4, cc4, iota, flip, t, cm, cp, s0, s1, s2;  // cm .. s2: ds_bpermute byte addresses
+  static __device__ __forceinline__ uint32_t lane_of(uint32_t x, uint32_t y) {
+    return x < 4 ? 8 * x + y : (y < 3 ? 5 + y : 10 + y);  // column 4: lanes 5, 6, 7, 13, 14
+  }
+  // (x, y) of half-lane l, or x = 5 for padding
+  static __device__ __forceinline__ void role(uint32_t l, uint32_t &x, uint32_t &y) {
+    const uint32_t g = l >> 3, r = l & 7;
+    if (g < 4 && r < 5) {
+      x = g;
+      y = r;
+    } else if (l == 5 || l == 6 || l == 7 || l == 13 || l == 14) {
+      x = 4;
+      y = l < 8 ? l - 5 : l - 10;
+    } else {
+      x = 5;
+      y = 0;
+    }
+  }
+  __device__ __forceinline__ explicit SpreadLane2(uint32_t lane) {
+    const uint32_t h = lane >> 5, l = lane & 31, base = 32 * h;
+    uint32_t x, y;
+    role(l, x, y);
+    const bool ok = x < 5;
+    m03 = x < 4 ? ~0u : 0u;
+    m4 = x == 4 ? ~0u : 0u;
+    cc4 = l == 5 ? ~0u : 0u;
+    iota = l == 0 ? ~0u : 0u;
+    const uint32_t rho = ok ? (uint32_t)kKeccak.rho[x + 5 * y] : 0u, rr = rho & 31;
+    flip = (rho >= 32) != (rr == 0) ? ~0u : 0u;
+    t = (32 - rr) & 31;
+    if (ok) {
+      const uint32_t xm = (x + 4) % 5, xp = (x + 1) % 5;
+      cm = 4 * (base + (xm == 4 ? 5u : 8 * xm));
+      cp = 4 * (base + (xp == 4 ? 5u : 8 * xp));
+      // pi: B[x, y] comes from A[3 (y - 3x) mod 5, x]
+      s0 = 4 * (base + lane_of((3 * (y + 5 - (3 * x) % 5)) % 5, x));
+      s1 = 4 * (base + lane_of((3 * (y + 5 - (3 * ((x + 1) % 5)) % 5)) % 5, (x + 1) % 5));
+      s2 = 4 * (base + lane_of((3 * (y + 5 - (3 * ((x + 2) % 5)) % 5)) % 5, (x + 2) % 5));
+    } else {
+      cm = cp = s0 = s1 = s2 = 4 * lane;
+    }
+  }
+  // message word this lane absorbs (x + 5y < 17), or -1
+  __device__ __forceinline__ static int word_of(uint32_t lane) {
+    uint32_t x, y;
+    role(lane & 31, x, y);
+    return x < 5 && x + 5 * y < 17 ? (int)(x + 5 * y) : -1;
+  }
+};
+
+// xor over the lane's 16-lane row
+__device__ __forceinline__ uint32_t xor16(uint32_t v) {
+  v = xor8(v);
+  return v ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, true);  // row_mirror
+}
+
+__device__ __forceinline__ void keccak_spread2(uint32_t &l, uint32_t &h, const SpreadLane2 &c) {
+#pragma unroll
+  for (int ir = 0; ir < 24; ir++) {
+    // theta's column parities: C[x] in every lane of group x, C[4] in the
+    // first row; the holders' word for the gathers below
+    const uint32_t c8l = xor8(l & c.m03), c8h = xor8(h & c.m03);
+    const uint32_t c16l = xor16(l & c.m4), c16h = xor16(h & c.m4);
+    const uint32_t ccl = (c16l & c.cc4) | (c8l & ~c.cc4), cch = (c16h & c.cc4) | (c8h & ~c.cc4);
+    const uint32_t cml = bperm(c.cm, ccl), cmh = bperm(c.cm, cch);
+    const uint32_t cpl = bperm(c.cp, ccl), cph = bperm(c.cp, cch);
+    l = xor3(l, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
+    h = xor3(h, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
+    // rho
+    const uint32_t a = c.flip ? h : l, b = c.flip ? l : h;
+    const uint32_t nl = __builtin_amdgcn_alignbit(a, b, c.t);
+    const uint32_t nh = __builtin_amdgcn_alignbit(b, a, c.t);
+    // pi + chi
+    l = chi3(bperm(c.s0, nl), bperm(c.s1, nl), bperm(c.s2, nl));
+    h = chi3(bperm(c.s0, nh), bperm(c.s1, nh), bperm(c.s2, nh));
+    // iota
+    l ^= (uint32_t)kKeccak.rc[ir] & c.iota;
+    h ^= (uint32_t)(kKeccak.rc[ir] >> 32) & c.iota;
+  }
+}
+
+// Small inner levels of many nodes: two nodes per wave, one per half.
+// Lanes 0 .. 15 of a half encode its node (encode_inner_node), then both
+// halves run the sponge together, each over its own message (the wave runs
+// as many blocks as its taller node; a half keeps its digest after its own
+// last block).
+__global__ __launch_bounds__(256) void k_hc_small2(Pools P, const ref_t *__restrict__ list,
+                                                   const uint64_t *__restrict__ slot, uint32_t n,
+                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
+  __shared__ uint64_t lds[256 / 32][kInnerWords];
+  const uint32_t lane = __lane_id(), half = lane >> 5, hl = lane & 31;
+  const uint32_t g = 2 * (threadIdx.x / 64) + half;  // the node's index in the block
+  const uint32_t i = blockIdx.x * (256 / 32) + g;
+  const bool active = i < n;
+  const ref_t r = active ? list[i] : REF_NULL;
+  uint64_t *out = active ? scratch + slot[i] / 8 : scratch;
+  bool in_lds = false;
+  uint32_t enc = encode_inner_node(P, r, active, hl < kInnerGroup, hl & (kInnerGroup - 1), lds[g], out, &in_lds);
+  enc = (uint32_t)__shfl((int)enc, 32 * half, 64);
+  in_lds = __shfl((int)in_lds, 32 * half, 64) != 0;
+  if (active && hl == 0) enc_len[i] = enc;
+  __syncthreads();  // the nodes' message words are visible to all their lanes
+  const uint32_t nb = active && enc >= 32 ? keccak_blocks(enc) : 0u;  // uniform over the half
+  const uint32_t nbmax = max((uint32_t)__shfl((int)nb, 0, 64), (uint32_t)__shfl((int)nb, 32, 64));
+  const SpreadLane2 c(lane);
+  const int w = SpreadLane2::word_of(lane);
+  const uint64_t *m = in_lds ? lds[g] : out;
+  uint32_t l = 0, h = 0, dl = 0, dh = 0;
+  for (uint32_t b = 0; b < nbmax; b++) {
+    if (w >= 0 && b < nb) {
+      const uint64_t v = m[17 * b + (uint32_t)w];
+      l ^= (uint32_t)v;
+      h ^= (uint32_t)(v >> 32);
+    }
+    keccak_spread2(l, h, c);
+    if (b + 1 == nb) {
+      dl = l;
+      dh = h;
+    }
+  }
+  if (nb == 0) return;
+  // digest word q is A[q, 0]: half-lane 8q
+  if ((hl & 7) == 0 && hl < 32) ((uint64_t *)node_hash(P, r))[hl >> 3] = ((uint64_t)dh << 32) | dl;
+  if (hl == 0) node_done(P, r, 32);
+}
+
 __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries, const unsigned long long *parts) {
   uint32_t t = grid_tid();
   if (t == 0) P.cnt->n_dirty = 0;  // every listed node has been hashed
@@ -1633,7 +1770,10 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
       hipError_t e = hipEventRecord(perm_begin, s);
       if (e != hipSuccess) return e;
     }
-    k_hc_small<<<(n + 3) / 4, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+    if (n >= kSpread2Min)  // enough nodes that the permutes' throughput, not one node's latency, sets the time
+      k_hc_small2<<<(n + 7) / 8, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
+    else
+      k_hc_small<<<(n + 3) / 4, kB, 0, s>>>(P, list, slot, n, scratch, enc_len);
     GMPT_LAUNCH_CHECK();
     if (perm_end) return hipEventRecord(perm_end, s);
     return hipSuccess;
