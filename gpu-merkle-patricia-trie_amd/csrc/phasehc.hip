@@ -834,12 +834,15 @@ __device__ __forceinline__ void keccak_lane(const uint64_t *m, uint32_t nb, uint
   }
 }
 
-// ORs the item 0xa0 + digest (33 bytes; digest words d0, d1 as stored) into
-// a zeroed LDS row at byte offset off: ten dword ORs at aligned addresses,
-// so items of neighbouring children that share a dword need no ordering.
+// Writes the item 0xa0 + digest (33 bytes; digest words d0, d1 as stored)
+// into a zeroed LDS row at byte offset off.  With r = off mod 4 the item
+// covers bytes [r, r + 33) of dwords off / 4 + 0 .. 8: dwords 1 .. 7 lie
+// inside it and are stored plainly; dwords 0 and 8 may hold a neighbour's
+// bytes (the previous or next item, an empty slot's 0x80, the list header)
+// and are ORed in, so neighbouring children need no ordering.
 __device__ __forceinline__ void or_child_item(uint32_t *row32, uint32_t off, const uint4 &d0, const uint4 &d1) {
   // the item's dwords (byte 0 = 0xa0, bytes 1 .. 32 = the digest)
-  uint32_t it[10];
+  uint32_t it[9];
   it[0] = 0xa0u | (d0.x << 8);
   it[1] = __builtin_amdgcn_alignbyte(d0.y, d0.x, 3);
   it[2] = __builtin_amdgcn_alignbyte(d0.z, d0.y, 3);
@@ -849,14 +852,17 @@ __device__ __forceinline__ void or_child_item(uint32_t *row32, uint32_t off, con
   it[6] = __builtin_amdgcn_alignbyte(d1.z, d1.y, 3);
   it[7] = __builtin_amdgcn_alignbyte(d1.w, d1.z, 3);
   it[8] = d1.w >> 24;
-  it[9] = 0;
   const uint32_t r = off & 3u, w0 = off >> 2;
+  uint32_t *p = row32 + w0;
   // shifted by r bytes: dword m takes item bytes [4m - r, 4m - r + 4)
 #pragma unroll
-  for (int m = 0; m < 10; m++) {
+  for (int m = 0; m < 9; m++) {
     const uint32_t prev = m ? it[m - 1] : 0u;
     const uint32_t v = r ? __builtin_amdgcn_alignbyte(it[m], prev, 4 - r) : it[m];
-    if (v) atomicOr(row32 + w0 + m, v);
+    if (m == 0 || m == 8)
+      atomicOr(p + m, v);
+    else
+      p[m] = v;
   }
 }
 
