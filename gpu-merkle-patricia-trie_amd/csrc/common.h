@@ -15,6 +15,7 @@
 // one such key (the reference points ShortNode::key into the batch's hex keys).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace gmpt {
@@ -33,16 +34,19 @@ constexpr int kMaxKeyNibbles = 254;  // hex key length limit (incl. terminator)
 // 128 bytes: one L2 line.
 struct Branch {
   uint8_t hash[32];      // Keccak-256 digest, or the raw encoding when it is < 32 B
-  ref_t child[17];       // slots 0..15 by nibble, 16 for a key ending here
-  uint16_t depth;        // nibble position this branch dispatches on
   uint8_t hash_size;     // 32, or the raw encoding length (< 32)
   uint8_t pad0;
+  uint16_t depth;        // nibble position this branch dispatches on
   uint32_t dirty;        // 1 while the node waits for re-hashing
-  uint32_t pad1;
   uint64_t lock;         // OLC version word: bit1 locked, bit0 obsolete, +4 per write
+  ref_t child[17];       // slots 0..15 by nibble, 16 for a key ending here
+  uint32_t pad1;
   uint8_t pad2[8];
 };
 static_assert(sizeof(Branch) == 128, "Branch layout");
+// A parent reading a child's digest and size touches the child's first 64
+// bytes only; the 16 nibble slots load as four 16-byte words.
+static_assert(offsetof(Branch, hash_size) < 64 && offsetof(Branch, child) % 16 == 0, "Branch layout");
 
 // 64 bytes.
 struct Short {

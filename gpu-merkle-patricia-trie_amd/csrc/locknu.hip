@@ -147,18 +147,21 @@ __device__ __forceinline__ void init_short(Short &S, uint64_t key_off, uint32_t 
 // A fresh branch at depth d with two children, written as its sixteen words
 // (layout of Branch).
 __device__ __forceinline__ void init_branch(Branch &B, uint32_t d, uint32_t s0, ref_t r0, uint32_t s1, ref_t r1) {
+  static_assert(offsetof(Branch, depth) == 34 && offsetof(Branch, lock) == 40 && offsetof(Branch, child) == 48,
+                "init_branch writes Branch by words");
   uint64_t *w = (uint64_t *)&B;
 #pragma unroll
   for (int q = 0; q < 4; q++) st(w + q, 0ull);        // hash
+  st(w + 4, (uint64_t)d << 16 | 1ull << 32);          // hash_size 0, depth, dirty 1
+  st(w + 5, 0ull);                                    // lock
 #pragma unroll
-  for (uint32_t q = 0; q < 9; q++) {                  // child[2q], child[2q + 1] (child[16], depth)
+  for (uint32_t q = 0; q < 8; q++) {                  // child[2q], child[2q + 1]
     const uint32_t lo = 2 * q, hi = 2 * q + 1;
     uint64_t x = (uint64_t)(lo == s0 ? r0 : lo == s1 ? r1 : 0u);
-    x |= q < 8 ? (uint64_t)(hi == s0 ? r0 : hi == s1 ? r1 : 0u) << 32 : (uint64_t)d << 32;
-    st(w + 4 + q, x);
+    x |= (uint64_t)(hi == s0 ? r0 : hi == s1 ? r1 : 0u) << 32;
+    st(w + 6 + q, x);
   }
-  st(w + 13, 1ull);                                   // dirty 1
-  st(w + 14, 0ull);                                   // lock
+  st(w + 14, (uint64_t)(16 == s0 ? r0 : 16 == s1 ? r1 : 0u));  // child[16] (a key ending here), pad
   st(w + 15, 0ull);
 }
 
