@@ -96,6 +96,8 @@ struct Counters {
   unsigned long long leaf_perms; // PhaseHC: permutations of the leaf level
   unsigned long long leaf_bytes; // PhaseHC: encoding bytes of the leaf level
   uint32_t scal[20];             // the current insert's device scalars (PhaseNU / LockNU)
+  uint32_t root0_size;           // trie 0's root as the last hash pass left it (k_hc_roots), so that
+  uint8_t root0[32];             // the pass's closing counter read also brings the root to the host
 };
 
 enum : uint32_t {

@@ -187,6 +187,7 @@ class Forest {
   HostStager *stager_ = nullptr; // pinned chunked staging of large host batches (made on first use)
   DevBuf hc_sorted_, hc_meta_, hc_cursor_, hc_slot_, hc_enc_, hc_scratch_, hc_lmeta_, hc_lmeta_tmp_;
   void *h_meta_ = nullptr;       // pinned HcMeta mirror
+  bool root0_ok_ = false;        // cnt_.root0 is trie 0's current root (set by hash, cleared by every other call)
   uint32_t hc_spread_max_ = 0;   // levels up to this size use the spread permutation
   bool hc_fuse_leaves_ = true;   // level-0 leaves: one encode+hash kernel (k_hc_leaf_direct)
   bool hc_fuse_inner_ = true;    // large inner levels: encode + hash with the messages in LDS (k_hc_inner_wave)

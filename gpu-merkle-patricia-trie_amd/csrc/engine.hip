@@ -139,6 +139,7 @@ Forest::~Forest() {
 }
 
 int Forest::set_device() {
+  root0_ok_ = false;  // any call but a root read may change the tries
   HIP_TRY(hipSetDevice(device_));
   return 0;
 }
@@ -726,6 +727,7 @@ int Forest::hash(void *targets, int n_targets) {
       stats_.leaf_perm_ms = ms;
     }
   }
+  root0_ok_ = cnt_.root0_size <= 32;
   return 0;
 }
 
@@ -767,6 +769,11 @@ int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves
 }
 
 int Forest::root_hash(uint32_t trie, uint8_t *out, int *size) {
+  if (trie == 0 && ntries_ > 0 && root0_ok_) {  // no call since the last hash pass, whose counter read brought it
+    *size = (int)cnt_.root0_size;
+    if (out) memcpy(out, cnt_.root0, cnt_.root0_size);
+    return 0;
+  }
   int r;
   if ((r = set_device())) return r;
   if (trie >= ntries_) {

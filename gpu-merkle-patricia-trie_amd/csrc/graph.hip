@@ -66,6 +66,7 @@ int Forest::capture_rebuild(const BatchDesc &b) {
   GRAPH_TRY(hipStreamSynchronize(stream_));
   // the host side of clear(): the graph's first kernel clears the device side
   memset(&cnt_, 0, sizeof(cnt_));
+  root0_ok_ = false;
   key_used_ = 0;
   vchunk_used_ = 0;
   rec_ = LeafRecords();
@@ -193,6 +194,7 @@ bool Forest::graph_ok() {
   const Counters c = *h_cnt_;
   if (c.err != ERR_NONE || c.graph_bad != 0 || c.flag_fallback != 0) return false;
   cnt_ = c;
+  root0_ok_ = c.root0_size <= 32;  // the replay's counter read follows its root pass
   key_used_ = plan_.key_used;
   all_empty_ = false;
   rec_ = LeafRecords();

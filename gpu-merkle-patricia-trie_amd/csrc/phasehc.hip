@@ -132,9 +132,10 @@ __device__ __forceinline__ void leaf_meta(const Pools &P, ref_t r, uint64_t *m) 
 // scratch + permutation kernels.
 constexpr uint32_t kLdsLeafBins = kHcClasses;  // the bins of level 0
 
-__global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n) {
+__global__ __launch_bounds__(256) void k_hc_zero(uint32_t *p, uint32_t n, uint32_t *p2 = nullptr, uint32_t n2 = 0) {
   uint32_t i = grid_tid();
   if (i < n) p[i] = 0;
+  if (i < n2) p2[i] = 0;
 }
 
 // Counting sort of the dirty list by bin.  The first pass computes each
@@ -1638,8 +1639,13 @@ __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries, cons
     hs = node_hash_size(P, r);
     h = node_hash(P, r);
   }
-  for (int q = 0; q < 32; q++) T.root_hash[q] = q < (int)hs ? h[q] : 0;
+  for (int q = 0; q < 32; q++) {
+    const uint8_t b = q < (int)hs ? h[q] : 0;
+    T.root_hash[q] = b;
+    if (t == 0) P.cnt->root0[q] = b;
+  }
   T.root_hash_size = hs;
+  if (t == 0) P.cnt->root0_size = hs;
 }
 
 // ---------------------------------------------- standalone Keccak (test hook) --
@@ -1747,10 +1753,10 @@ size_t hc_temp_bytes(uint32_t n) { return scan_temp_bytes(n > kHcBins ? n : kHcB
 hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *meta, uint32_t *cursor, ref_t *out,
                        uint64_t *slot, uint32_t *packed, void *temp, hipStream_t s, uint64_t *lmeta_tmp,
                        uint64_t *lmeta, LeafRecords rec) {
-  k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins);
-  GMPT_LAUNCH_CHECK();
-  // the pass's work counters (permutations, bytes, leaf permutations, leaf bytes)
-  k_hc_zero<<<1, kB, 0, s>>>((uint32_t *)&P.cnt->n_perms, 8);
+  // the bins, and the pass's work counters (permutations, bytes, leaf
+  // permutations, leaf bytes)
+  static_assert(kHcBins >= 8, "one launch clears both");
+  k_hc_zero<<<nblk(kHcBins), kB, 0, s>>>(meta->hist, kHcBins, (uint32_t *)&P.cnt->n_perms, 8);
   GMPT_LAUNCH_CHECK();
   if (n == 0) {
     k_hc_zero<<<1, kB, 0, s>>>(&meta->slot_lo, 4);  // slot_lo, nvalid, scratch_bytes

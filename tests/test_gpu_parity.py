@@ -872,6 +872,34 @@ def test_index_pairs_sharing_one_key(gmpt, workloads, oracle, mode, where):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_root_reads_between_other_calls(gmpt, workloads, oracle, mode):
+    """A hash pass brings trie 0's root back with its closing counter read;
+    every other call drops that copy, so a root read always shows the tries'
+    current header: equal to the one-copy read of all roots, to the oracle's
+    after each hash, unchanged by an insert that is not hashed yet, empty
+    after a clear."""
+    base = _rand_batch(workloads, 71, 3000, 1, 90)
+    more = _rand_batch(workloads, 72, 1000, 1, 90)
+    t = gmpt.MPT(device=0)
+    _put(t, base, mode)
+    r1 = t.get_root_hash()
+    assert r1 == oracle.root_of(base)
+    assert t.get_root_hash() == r1 and t.get_root_hashes(0, 1) == [r1]
+    fn = t.puts_2phase_with_valuehp if mode == "2phase" else t.puts_latching_with_valuehp_v2
+    targets, n = fn(more)
+    assert t.get_root_hash() == r1 == t.get_root_hashes(0, 1)[0]  # the header changes with the hash only
+    t.hash_onepass_v2(targets, n)
+    r2 = t.get_root_hash()
+    assert r2 == oracle.root_of(workloads.concat_batches([base, more])) and r2 != r1
+    assert t.get_root_hashes(0, 1) == [r2]
+    t.clear()
+    assert t.get_root_hash() == b"" == t.get_root_hashes(0, 1)[0]
+    _put(t, more, mode)
+    assert t.get_root_hash() == oracle.root_of(more)
+    t.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_batch_of_more_than_4096_blocks(gmpt, workloads, oracle, mode):
     """1,100,000 keys in one batch: more than 4,096 blocks of 256, so the
     run numbering and the branch ranks take their block counts through a
