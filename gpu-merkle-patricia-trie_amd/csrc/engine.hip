@@ -44,6 +44,7 @@ size_t nu_ws_bytes(size_t n, bool tries) {
   b += n * 8 * 5;      // sorted krefs; unique-order krefs and windows (two copies each)
   b += n * (4 * 7 + 1 + sizeof(Stop));
   b += ne * (4 + 4 + 4 + 4 + 8) + mt_bytes(align_up(ne, 256)) + 4 * (ne / 256 + 2);
+  b += 2 * align_up(ne, 256) * (4 + 4 + 1) + 4 * (ne / 256 + 2);  // links between blocks of the build
   b += std::max(radix_temp_bytes(n), scan_temp_bytes(ne));
   b += 64 * 256;  // alignment slack for ~40 carves
   return b;
@@ -483,6 +484,10 @@ int Forest::carve_nu(NuArgs &a, const BatchDesc &d, uint64_t vbase, bool has_tid
   a.bid = c.take<uint32_t>(ne);
   a.plohi = c.take<uint64_t>(ne);
   a.bcnt = c.take<uint32_t>(ne / 256 + 2);
+  a.xp = c.take<uint32_t>(2 * align_up(ne, 256));
+  a.xm = c.take<uint32_t>(2 * align_up(ne, 256));
+  a.xs = c.take<uint8_t>(2 * align_up(ne, 256));
+  a.xcnt = c.take<uint32_t>(ne / 256 + 2);
   *temp = c.take<uint8_t>(std::max(radix_temp_bytes(n), scan_temp_bytes(ne)));
   if (c.off > ws_.cap) {
     set_error("workspace carve overflow");

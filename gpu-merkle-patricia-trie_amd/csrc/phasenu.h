@@ -96,6 +96,9 @@ struct NuArgs {
   uint32_t *bcnt;        // per block of 256 split points: representatives (k_split_reps)
   uint64_t *plohi;       // per representative split point: its interval's bounding split points
                          // (nearest to the left and right with a smaller value) lo | hi << 32, -1 / nsp past the ends
+  uint32_t *xp, *xm;     // links whose parent another block initialises: parent representative, child
+  uint8_t *xs;           // and slot (512 entries per block of 256 split points)
+  uint32_t *xcnt;        // per block: its cross links
   // LockNU
   uint64_t seq_base;     // batch order stamp of key 0
   uint32_t base_branch, base_short, base_value;  // pre-assigned node slots

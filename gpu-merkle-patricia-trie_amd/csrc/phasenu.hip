@@ -1281,7 +1281,7 @@ __device__ __forceinline__ ST st_window(const ST &g, uint32_t t0, uint8_t *w, ui
 
 // Per split point: find its branch (the representative is the leftmost split
 // point with the same LCP inside the maximal interval whose LCPs are >= it).
-// bcnt[block] = the block's representatives, from which k_init_branches
+// bcnt[block] = the block's representatives, from which k_build_links
 // ranks them.  A representative also records its interval's bounding split
 // points (plohi).  The queries run on the workgroup's window of split points
 // in LDS.
@@ -1336,52 +1336,11 @@ __global__ __launch_bounds__(256) void k_split_reps(NuArgs a, uint32_t ne_max) {
 struct Slots {
   uint32_t ne, nrep, dbase;
 };
-__device__ __forceinline__ Slots slots_of(const NuArgs &a) {
-  Slots q;
-  q.ne = a.scal[SC_NE];
-  q.nrep = a.scal[SC_NREP_TOT];  // the branch count, from the representatives' scan
-  q.dbase = a.scal[SC_DBASE];
-  return q;
-}
 __device__ __forceinline__ void put_dirty(const NuArgs &a, uint32_t slot, ref_t r) {
   if (slot < a.P.cap_dirty)
     a.P.dirty[slot] = r;
   else
     raise_error(a.P.cnt, ERR_POOL, 4);
-}
-
-// Ranks the representatives (the branches, in split-point order: the block's
-// offset from k_split_reps' counts, then a block-local scan), records each
-// rank in bid and the total in SC_NREP_TOT, and initialises the branches.
-__global__ __launch_bounds__(256) void k_init_branches(NuArgs a, int pre) {
-  // the sort needs a fallback pass, or an error is pending: the build waits
-  // for the host (which may run it before reading either)
-  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
-  __shared__ uint32_t sh[4];
-  uint32_t j = grid_tid();
-  ST s = st_of(a);
-  const uint32_t L = j < s.nsp ? s.at(0, j) : 0u;
-  const bool rep = L != 0 && a.repof[j] == j;
-  uint32_t btot;
-  const uint32_t base = block_offset(a.bcnt, pre != 0, sh);
-  const uint32_t rank = base + block_exclusive_scan_256<uint32_t>(rep, sh, &btot);
-  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) a.scal[SC_NREP_TOT] = base + btot;
-  if (!rep) return;
-  a.bid[j] = rank;
-  Slots q = slots_of(a);
-  uint32_t id = a.base_branch + rank;
-  if (id >= a.P.cap_branch) {
-    raise_error(a.P.cnt, ERR_POOL, 1);
-    return;
-  }
-  Branch &B = a.P.branch[id];
-  uint64_t *w = (uint64_t *)&B;
-#pragma unroll
-  for (int k = 0; k < 16; k++) w[k] = 0;
-  B.depth = (uint16_t)(L - 1);
-  B.dirty = 1;
-  a.bdesc[rank] = L - 1;  // the links below count its children
-  put_dirty(a, q.dbase + q.ne + rank, make_ref(T_BRANCH, id));
 }
 
 // One more child linked into the build's branch of rank `rank` (the record
@@ -1466,12 +1425,13 @@ __device__ __forceinline__ void install(const NuArgs &a, const GroupInfo &gi, re
 }
 
 // Branches: link each to its parent (through an extension when depths skip).
-__device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j) {
+template <class Sink>
+__device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j, uint32_t rank, const Slots &q,
+                                            const Sink &sink) {
   ST s = st_of(a);
   if (j >= s.nsp) return;
   uint32_t L = s.at(0, j);
   if (L == 0 || a.repof[j] != j) return;
-  Slots q = slots_of(a);
   // the interval's bounding split points, found by k_split_reps
   const uint64_t lh = a.plohi[j];
   const int lo = (int)(uint32_t)lh;
@@ -1480,7 +1440,7 @@ __device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j) {
   uint32_t Lb = hi < s.nsp ? s.at(0, hi) : 0u;
   uint32_t e0 = (uint32_t)(lo + 1);
   ElemKey ek = elem_key(a, e0);
-  uint32_t depth = L - 1, lo_depth, parent_bid = 0, slot = 0;
+  uint32_t depth = L - 1, lo_depth, parent = 0, slot = 0;
   bool is_root = La == 0 && Lb == 0;
   GroupInfo gi;
   if (is_root) {
@@ -1489,11 +1449,10 @@ __device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j) {
   } else {
     uint32_t p = La >= Lb ? (uint32_t)lo : hi;
     uint32_t Lp = max(La, Lb) - 1;
-    parent_bid = a.base_branch + a.bid[a.repof[p]];
+    parent = a.repof[p];
     slot = elem_nib(a.P, ek, Lp, a.scal[SC_W]);
     lo_depth = Lp + 1;
   }
-  uint32_t rank = a.bid[j];
   ref_t me = make_ref(T_BRANCH, a.base_branch + rank);
   ref_t ext = REF_NULL;
   if (depth > lo_depth) {
@@ -1507,12 +1466,10 @@ __device__ __forceinline__ void link_branch(const NuArgs &a, uint32_t j) {
     }
   }
   put_dirty(a, q.dbase + q.ne + q.nrep + rank, ext);
-  if (is_root) {
+  if (is_root)
     install(a, gi, me);
-  } else {
-    a.P.branch[parent_bid].child[slot] = me;
-    count_child(a, parent_bid - a.base_branch, slot);
-  }
+  else
+    sink(a, parent, slot, me);
 }
 
 // Elements: new keys become a leaf short node + value node; a split short
@@ -1530,9 +1487,9 @@ __device__ __forceinline__ void init_value(const NuArgs &a, uint32_t vid, uint32
   if (size) *size = V.size;
 }
 
-__device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e) {
+template <class Sink>
+__device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e, const Slots &q, const Sink &sink) {
   ST s = st_of(a);
-  Slots q = slots_of(a);
   if (e == 0) {
     // the build's node and dirty-list slots are all assigned by position:
     // advance the pool counters past them (no kernel of the build reads them)
@@ -1545,7 +1502,7 @@ __device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e) {
   ElemKey ek = elem_key(a, e);
   uint32_t La = e > 0 ? s.at(0, e - 1) : 0u;
   uint32_t Lb = e < s.nsp ? s.at(0, e) : 0u;
-  uint32_t start, slot = 0, parent_bid = 0;
+  uint32_t start, slot = 0, parent = 0;
   bool is_root = La == 0 && Lb == 0;
   GroupInfo gi;
   if (is_root) {
@@ -1554,7 +1511,7 @@ __device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e) {
   } else {
     uint32_t p = La >= Lb ? e - 1 : e;
     uint32_t Lp = max(La, Lb) - 1;
-    parent_bid = a.base_branch + a.bid[a.repof[p]];
+    parent = a.repof[p];
     slot = elem_nib(a.P, ek, Lp, a.scal[SC_W]);
     start = Lp + 1;
   }
@@ -1592,23 +1549,140 @@ __device__ __forceinline__ void link_elem(const NuArgs &a, uint32_t e) {
   }
   a.leaf_pk[e] = pk;
   put_dirty(a, q.dbase + e, sh);
-  if (is_root) {
+  if (is_root)
     install(a, gi, me);
-  } else {
-    a.P.branch[parent_bid].child[slot] = me;
-    count_child(a, parent_bid - a.base_branch, slot);
-  }
+  else
+    sink(a, parent, slot, me);
 }
 
-// The links of the build's branches and elements: thread j links the branch
-// whose representative is split point j, if any, and element j.
-__global__ __launch_bounds__(256) void k_link(NuArgs a) {
+// The build's branches and links in one launch.  Block b ranks the
+// representatives among its split points [256b, 256b + 256) (the block's
+// offset from k_split_reps' counts, then a block scan: rank r is branch
+// base_branch + r, bid records it, SC_NREP_TOT the total) and owns their
+// branches: it builds their child arrays in LDS from every link whose parent
+// it owns -- most links, since a branch's children sit beside its
+// representative in batch order -- and then writes each owned branch whole,
+// once, with coalesced stores.  A link whose parent another block owns goes
+// to the cross list (xp, xs, xm), which k_link_cross applies afterwards; no
+// other branch line is written piecewise.  Thread j links the branch whose
+// representative is split point j, if any, and element j.
+constexpr uint32_t kCrossPerBlock = 2 * kSpTile;  // a thread makes at most two links
+struct OwnedLinks {
+  ref_t (*kid)[17];
+  uint32_t *krec;
+  const uint16_t *own;
+  uint32_t *xn;  // the block's cross links so far (LDS)
+  uint32_t lo;   // the block's first split point
+  __device__ __forceinline__ void operator()(const NuArgs &a, uint32_t parent, uint32_t slot, ref_t me) const {
+    if (parent - lo < kSpTile) {  // a representative of this block
+      const uint32_t l = own[parent - lo];
+      kid[l][slot] = me;
+      if (slot == kNibbleT)
+        atomicOr(&krec[l], kBranchRecChase);
+      else
+        atomicAdd(&krec[l], kBranchRecChild);
+    } else {  // into the block's own segment of the cross list
+      const uint32_t x = blockIdx.x * kCrossPerBlock + atomicAdd(xn, 1u);
+      a.xp[x] = parent;
+      a.xs[x] = (uint8_t)slot;
+      a.xm[x] = me;
+    }
+  }
+};
+
+__global__ __launch_bounds__(256) void k_build_links(NuArgs a, int pre) {
   // the sort needs a fallback pass, or an error is pending: the build waits
   // for the host (which may run it before reading either)
   if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
+  static_assert(kSpTile == 256, "one split point per thread");
+  static_assert(offsetof(Branch, hash_size) == 32 && offsetof(Branch, depth) == 34 && offsetof(Branch, dirty) == 36 &&
+                    offsetof(Branch, lock) == 40 && offsetof(Branch, child) == 48 && sizeof(Branch) == 128,
+                "owned branches are written as eight 16-byte words");
+  __shared__ uint32_t sh[4];
+  __shared__ uint64_t sh2[4];
+  __shared__ ref_t kid[kSpTile][17];
+  __shared__ uint32_t krec[kSpTile];   // the owned branches' records (bdesc): depth, children, slot-16 flag
+  __shared__ uint16_t own[kSpTile];    // per split point of the block: its branch's local index
+  __shared__ uint32_t xn;
   const uint32_t j = grid_tid();
-  link_branch(a, j);
-  link_elem(a, j);
+  const ST s = st_of(a);
+  const uint32_t L = j < s.nsp ? s.at(0, j) : 0u;
+  const bool rep = L != 0 && a.repof[j] == j;
+  // the block's offset among the representatives and their total
+  uint32_t base, nrep;
+  if (pre) {
+    base = a.bcnt[blockIdx.x];
+    nrep = a.scal[SC_NREP_TOT];  // the count scan's total
+  } else {
+    uint64_t part = 0;  // representatives before the block | all of them << 32
+#pragma unroll
+    for (uint32_t q = 0; q < kSelfSumBlocks / 256; q++) {
+      const uint32_t t = threadIdx.x + 256 * q;
+      const uint64_t c = t < gridDim.x ? a.bcnt[t] : 0u;
+      part += (t < blockIdx.x ? c : 0ull) | c << 32;
+    }
+    uint64_t tot;
+    block_exclusive_scan_256<uint64_t>(part, sh2, &tot);
+    base = (uint32_t)tot;
+    nrep = (uint32_t)(tot >> 32);
+  }
+  uint32_t btot;
+  const uint32_t lrank = block_exclusive_scan_256<uint32_t>(rep, sh, &btot);
+  const uint32_t rank = base + lrank;
+  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) a.scal[SC_NREP_TOT] = nrep;
+  Slots q;
+  q.ne = a.scal[SC_NE];
+  q.nrep = nrep;
+  q.dbase = a.scal[SC_DBASE];
+  own[threadIdx.x] = rep ? (uint16_t)lrank : (uint16_t)0xffffu;
+  if (threadIdx.x == 0) xn = 0;
+  for (uint32_t k = threadIdx.x; k < btot * 17; k += kSpTile) (&kid[0][0])[k] = REF_NULL;
+  if (rep) {
+    krec[lrank] = L - 1;
+    a.bid[j] = rank;
+    put_dirty(a, q.dbase + q.ne + rank, make_ref(T_BRANCH, a.base_branch + rank));
+  }
+  __syncthreads();  // child arrays cleared, local indices set
+  const OwnedLinks sink{kid, krec, own, &xn, blockIdx.x * kSpTile};
+  link_branch(a, j, rank, q, sink);
+  link_elem(a, j, q, sink);
+  __syncthreads();  // every owned link in LDS
+  // the owned branches, 16 bytes per thread: hash (zero), hash_size 0 |
+  // depth | dirty 1 | lock 0, child[0 .. 15], child[16] and padding
+  const uint32_t b0 = a.base_branch + base;
+  for (uint32_t k = threadIdx.x; k < btot * 8; k += kSpTile) {
+    const uint32_t l = k / 8, w = k % 8;
+    if (b0 + l >= a.P.cap_branch) {
+      raise_error(a.P.cnt, ERR_POOL, 1);
+      continue;
+    }
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (w == 2) {
+      v.x = (krec[l] & 0xffffu) << 16;  // the representative's L - 1: the depth
+      v.y = 1;
+    } else if (w >= 3 && w <= 6) {
+      v = make_uint4(kid[l][4 * (w - 3)], kid[l][4 * (w - 3) + 1], kid[l][4 * (w - 3) + 2], kid[l][4 * (w - 3) + 3]);
+    } else if (w == 7) {
+      v.x = kid[l][16];
+    }
+    ((uint4 *)&a.P.branch[b0 + l])[w] = v;
+  }
+  for (uint32_t k = threadIdx.x; k < btot; k += kSpTile) a.bdesc[base + k] = krec[k];
+  if (threadIdx.x == 0) a.xcnt[blockIdx.x] = xn;
+}
+
+// The cross links of k_build_links (block b's in its segment of the list):
+// each into its parent's child slot, and counted in the parent's record.
+__global__ __launch_bounds__(256) void k_link_cross(NuArgs a) {
+  if (a.P.cnt->flag_fallback || a.P.cnt->err) return;
+  const uint32_t n = a.xcnt[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    const uint32_t x = blockIdx.x * kCrossPerBlock + k;
+    const uint32_t rank = a.bid[a.xp[x]], slot = a.xs[x];
+    if (a.base_branch + rank >= a.P.cap_branch) continue;  // raised by the branch's own block
+    a.P.branch[a.base_branch + rank].child[slot] = a.xm[x];
+    count_child(a, rank, slot);
+  }
 }
 
 // Existing keys: swap in a new value node (runs before the links, so a leaf
@@ -1931,15 +2005,15 @@ hipError_t nu_build(const NuArgs &a, void *temp, hipStream_t s, bool empty) {
   k_split_reps<<<nblk(ne_max), kB, 0, s>>>(a, ne_max);
   GMPT_LAUNCH_CHECK();
   // the per-block representative counts: scanned here when there are many
-  // blocks, else each block of k_init_branches adds up those before it
+  // blocks, else each block of k_build_links adds up those before it
   const bool pre = nblk(ne_max) > kSelfSumBlocks;
   if (pre) {
-    hipError_t e = scan_exclusive_u32(a.bcnt, a.bcnt, nblk(ne_max), nullptr, temp, s);
+    hipError_t e = scan_exclusive_u32(a.bcnt, a.bcnt, nblk(ne_max), a.scal + SC_NREP_TOT, temp, s);
     if (e != hipSuccess) return e;
   }
-  k_init_branches<<<nblk(ne_max), kB, 0, s>>>(a, pre ? 1 : 0);
+  k_build_links<<<nblk(ne_max), kB, 0, s>>>(a, pre ? 1 : 0);
   GMPT_LAUNCH_CHECK();
-  k_link<<<nblk(ne_max), kB, 0, s>>>(a);
+  k_link_cross<<<nblk(ne_max), kB, 0, s>>>(a);
   return hipGetLastError();
 }
 
