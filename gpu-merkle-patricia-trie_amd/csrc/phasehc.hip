@@ -1804,7 +1804,7 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     // the few nodes taller than the rows (classes rows_nb .. 6, between the
     // row classes and the extensions) would take the generic encoder and hold
     // the launch open long after the bulk: with a second stream they are
-    // encoded by 16-lane groups and permuted there, concurrently with the bulk
+    // encoded and hashed there, concurrently with the bulk
     const uint32_t cov = std::min(n, rows_nb == 4 ? c01 + c23 : rows_nb == 2 ? c01 : cls[0]);
     uint32_t lo = 0, hi = 0;
     const uint32_t multi = n - std::min(n, cov + cext);
@@ -1814,10 +1814,9 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
       hipError_t e;
       if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
-      k_hc_encode_inner<<<(multi + kInnerNodes - 1) / kInnerNodes, kB, 0, side->stream>>>(P, list + lo, slot + lo, multi,
-                                                                                         scratch, enc_len + lo);
-      GMPT_LAUNCH_CHECK();
-      k_hc_perm_lane<<<nblk(multi), kB, 0, side->stream>>>(P, list + lo, slot + lo, enc_len + lo, multi, scratch);
+      // two nodes per wave in the spread layout: latency-bound waves that
+      // share the SIMDs with the bulk's permutations
+      k_hc_small2<<<(multi + 7) / 8, kB, 0, side->stream>>>(P, list + lo, slot + lo, multi, scratch, enc_len + lo);
       GMPT_LAUNCH_CHECK();
       if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
     }
