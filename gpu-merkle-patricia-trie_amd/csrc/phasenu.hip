@@ -120,7 +120,25 @@ __device__ __forceinline__ KeyStager stage_keys(const NuArgs &a, uint8_t *lds) {
   uintptr_t e = ((uintptr_t)(a.kh + hi) + 16) & ~(uintptr_t)15;
   if (e - b > kKeyStage) return st;
   uint32_t nw = (uint32_t)((e - b) / 16);
-  for (uint32_t k = threadIdx.x; k < nw; k += blockDim.x) ((uint4 *)lds)[k] = ((const uint4 *)b)[k];
+  // every load of the thread issued before its LDS stores (a store waits for
+  // its load, so a rolled loop would make one memory round trip per word)
+  constexpr uint32_t kStagePer = (kKeyStage / 16 + 255) / 256;
+  uint32_t v[4 * kStagePer];
+#pragma unroll
+  for (uint32_t q = 0; q < kStagePer; q++) {
+    const uint32_t k = threadIdx.x + 256 * q;
+    uint4 x = make_uint4(0, 0, 0, 0);
+    if (k < nw) x = ((const uint4 *)b)[k];
+    v[4 * q] = x.x;
+    v[4 * q + 1] = x.y;
+    v[4 * q + 2] = x.z;
+    v[4 * q + 3] = x.w;
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < kStagePer; q++) {
+    const uint32_t k = threadIdx.x + 256 * q;
+    if (k < nw) ((uint4 *)lds)[k] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  }
   __syncthreads();
   st.base = b;
   return st;
@@ -1240,20 +1258,28 @@ __device__ __forceinline__ ST st_window(const ST &g, uint32_t t0, uint8_t *w, ui
   wb = t0 >= kSpHalo ? t0 - kSpHalo : 0u;
   const uint32_t n0 = g.nsp > wb ? min(kSpWin, g.nsp - wb) : 0u;
   uint8_t *w1 = w + kSpWin, *w2 = w1 + kSpL1, *w3 = w2 + kSpL2Pad;
-  for (uint32_t c = threadIdx.x; c < kSpL1; c += blockDim.x) {
-    // level 0, 16 bytes per chunk (the tree's level 0 is padded to a
-    // multiple of 256 bytes, so whole 16-byte reads stay inside it), and the
-    // chunk's level-1 minimum over the valid entries
-    uint4 v = make_uint4(0, 0, 0, 0);
-    uint32_t mn = 255;
-    if (16 * c < n0) {
-      v = *(const uint4 *)(g.t + wb + 16 * c);
-      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+  // level 0, 16 bytes per chunk (the tree's level 0 is padded to a multiple
+  // of 256 bytes, so whole 16-byte reads stay inside it), and each chunk's
+  // level-1 minimum over the valid entries; the thread's chunk loads are all
+  // issued before any is used
+  constexpr uint32_t kPer = (kSpL1 + kSpTile - 1) / kSpTile;
+  uint4 v[kPer];
 #pragma unroll
-      for (uint32_t i = 0; i < 16; i++)
-        if (16 * c + i < n0) mn = min(mn, (wd[i >> 2] >> (8 * (i & 3))) & 255u);
-    }
-    ((uint4 *)w)[c] = v;
+  for (uint32_t q = 0; q < kPer; q++) {
+    const uint32_t c = threadIdx.x + kSpTile * q;
+    v[q] = make_uint4(0, 0, 0, 0);
+    if (c < kSpL1 && 16 * c < n0) v[q] = *(const uint4 *)(g.t + wb + 16 * c);
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < kPer; q++) {
+    const uint32_t c = threadIdx.x + kSpTile * q;
+    if (c >= kSpL1) break;
+    uint32_t mn = 255;
+    const uint32_t wd[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++)
+      if (16 * c + i < n0) mn = min(mn, (wd[i >> 2] >> (8 * (i & 3))) & 255u);
+    ((uint4 *)w)[c] = v[q];
     w1[c] = (uint8_t)mn;
   }
   __syncthreads();
