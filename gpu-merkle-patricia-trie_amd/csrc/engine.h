@@ -199,6 +199,7 @@ class Forest {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   hipEvent_t ev_meta_ = nullptr;               // the classification bins have reached the host
+  hipEvent_t tail_join_ = nullptr;             // the second stream has hashed the pass's deepest levels
   SideStream side_{};                          // second stream for work beside a hash level's main kernel
   // the captured rebuild pass and the batch it was captured for
   struct GraphPlan {

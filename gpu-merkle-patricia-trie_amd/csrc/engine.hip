@@ -77,6 +77,7 @@ Forest::Forest(int device, uint32_t ntries) : device_(device), ntries_(ntries) {
   if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return;
   if (hipEventCreate(&ev2_) != hipSuccess || hipEventCreate(&ev3_) != hipSuccess) return;
   if (hipEventCreateWithFlags(&ev_meta_, hipEventDisableTiming) != hipSuccess) return;
+  if (hipEventCreateWithFlags(&tail_join_, hipEventDisableTiming) != hipSuccess) return;
   // default priority, like the main stream
   if (hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking) != hipSuccess) return;
   if (hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming) != hipSuccess) return;
@@ -130,6 +131,7 @@ Forest::~Forest() {
   if (ev2_) (void)hipEventDestroy(ev2_);
   if (ev3_) (void)hipEventDestroy(ev3_);
   if (ev_meta_) (void)hipEventDestroy(ev_meta_);
+  if (tail_join_) (void)hipEventDestroy(tail_join_);
   if (side_.fork) (void)hipEventDestroy(side_.fork);
   if (side_.join) (void)hipEventDestroy(side_.join);
   if (side_.stream) {
@@ -748,10 +750,42 @@ int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves
   uint32_t n_direct = 0;  // the leaves of level 0, all hashed by the direct kernel
   if (direct_leaves)
     for (uint32_t c = 0; c < kHcClasses; c++) n_direct += hist[c];
-  uint32_t start = 0, n_leaf = 0;
-  for (uint32_t lv = 0; lv < kHcLevels; lv++) {
+  const auto level_count = [&](uint32_t lv) {
     uint32_t cnt = 0;
     for (uint32_t c = 0; c < kHcClasses; c++) cnt += hist[lv * kHcClasses + c];
+    return cnt;
+  };
+  // The tail: the deepest inner levels (levels 2 .. tail_end - 1), when they
+  // are small.  They depend on nothing but their own leaf children and each
+  // other, so the second stream hashes those leaves and then the tail levels
+  // while the leaf-level launch runs; the forest's stream waits for them
+  // before the first level above the tail.  Leaves the word builder cannot
+  // take (level 1) are hashed after the leaf launch, so a pass with any
+  // hashes no tail.
+  uint32_t tail_end = 2, tail_n = 0;
+  if (direct_leaves && n_direct && !captured && side_.stream && ev_meta_ && tail_join_ && level_count(1) == 0) {
+    for (uint32_t lv = 2; lv < kHcLevels; lv++) {
+      const uint32_t cnt = level_count(lv);
+      if (cnt > kTailLevelMax || tail_n + cnt > kTailMax) break;
+      tail_n += cnt;
+      tail_end = lv + 1;
+    }
+  }
+  const bool tail = tail_n > 0;
+  bool joined = !tail;
+  uint32_t start = 0, n_leaf = 0;
+  for (uint32_t lv = 0; lv < kHcLevels; lv++) {
+    const uint32_t cnt = level_count(lv);
+    const bool in_tail = tail && lv >= 2 && lv < tail_end;
+    if (tail && lv == 2) {
+      // after the classification (its bins' copy), beside the leaf launch
+      HIP_TRY(hipStreamWaitEvent(side_.stream, ev_meta_, 0));
+      HIP_TRY(hc_tail_leaves(P, sorted + start, tail_n, side_.stream));
+    }
+    if (cnt && !in_tail && !joined && lv >= 2) {
+      HIP_TRY(hipStreamWaitEvent(stream_, tail_join_, 0));
+      joined = true;
+    }
     if (cnt) {
       const bool leaves = lv <= 1;  // level 1: leaves the direct kernel does not take
       const uint32_t skip = lv == 0 ? n_direct : 0u;  // already queued
@@ -759,16 +793,19 @@ int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves
       const uint32_t n_ext = leaves ? 0u : hist[lv * kHcClasses + kHcClasses - 1];
       if (cnt > skip)
         HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
-                              encl + start + skip, leaves, hc_spread_max_, stream_, timed ? ev2_ : nullptr,
-                              timed ? ev3_ : nullptr, n_ext, hc_fuse_inner_ ? &hist[lv * kHcClasses] : nullptr,
-                              side_.stream && !captured ? &side_ : nullptr));
+                              encl + start + skip, leaves, hc_spread_max_, in_tail ? side_.stream : stream_,
+                              timed ? ev2_ : nullptr, timed ? ev3_ : nullptr, n_ext,
+                              hc_fuse_inner_ ? &hist[lv * kHcClasses] : nullptr,
+                              side_.stream && !captured && !in_tail ? &side_ : nullptr));
       stats_.hash_launches++;
       stats_.hashed_nodes += cnt;
       if (!leaves && n_ext >= kExtDirectMin && cnt > hc_spread_max_) stats_.ext_direct_nodes += n_ext;
       if (leaves) n_leaf += cnt;
     }
+    if (tail && lv + 1 == tail_end) HIP_TRY(hipEventRecord(tail_join_, side_.stream));
     start += cnt;
   }
+  if (!joined) HIP_TRY(hipStreamWaitEvent(stream_, tail_join_, 0));  // every inner level was in the tail
   *n_leaf_out = n_leaf;
   return 0;
 }

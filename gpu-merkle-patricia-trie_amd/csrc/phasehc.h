@@ -78,6 +78,14 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
 // launch, so the launch need not wait for the bins to reach the host.
 hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64_t *lmeta, const HcMeta *meta,
                                  uint32_t n_upper, uint32_t *enc_len, hipStream_t s);
+// Hashes the leaf children of the inner nodes list[0 .. n) (the deepest
+// levels, taken off the critical path onto a second stream), every one a
+// leaf the word builder takes.
+hipError_t hc_tail_leaves(const Pools &P, const ref_t *list, uint32_t n, hipStream_t s);
+// The deepest inner levels a pass may hash beside its leaf-level launch: at
+// most this many nodes in all, none of them in a level above kTailLevelMax.
+constexpr uint32_t kTailMax = 8192;
+constexpr uint32_t kTailLevelMax = 4096;
 // The pass's permutations and encoding bytes (from the enc_len array of the
 // meta->nvalid hashed nodes), and those of the first n_leaf entries (the
 // leaf level), as kHcCountParts per-block partial sums (4 words each) in

@@ -1299,6 +1299,44 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
   enc_len[i] = hash_leaf_shaped(P, list[i], m0, lmeta[2 * (size_t)i + 1], head[threadIdx.x], wlo, whi);
 }
 
+// The leaf children of list[0 .. n) (inner nodes of the deepest levels),
+// one thread per (node, slot): hashed here, on a second stream, so that
+// those levels can be hashed beside the leaf-level launch, which hashes the
+// same leaves again and stores the same bytes (a reader sees equal old and
+// new values).  Only leaves the word builder takes reach here (the host
+// checks that no leaf needs the generic encoder).
+__global__ __launch_bounds__(256) void k_hc_tail_leaves(Pools P, const ref_t *__restrict__ list, uint32_t n) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
+  __builtin_amdgcn_s_setprio(2);  // ahead of the leaf-level launch's waves on a shared SIMD
+  __shared__ uint64_t head[256][kLeafDirectHead];
+  __shared__ ref_t job[256];
+  __shared__ uint32_t sh[4];
+  const uint32_t t = grid_tid(), i = t / 16, s = t % 16;
+  ref_t leaf = REF_NULL;
+  if (i < n) {
+    const ref_t r = list[i];
+    if (ref_type(r) == T_BRANCH) {  // an extension's child is a branch
+      const ref_t c = P.branch[ref_idx(r)].child[s];
+      if (ref_type(c) == T_SHORT && ref_type(P.shrt[ref_idx(c)].child) == T_VALUE) leaf = c;
+    }
+  }
+  // the block's leaves packed into its first lanes (a few per node): the
+  // waves left without one exit
+  uint32_t tot;
+  const uint32_t pos = block_exclusive_scan_256<uint32_t>(leaf != REF_NULL ? 1u : 0u, sh, &tot);
+  if (leaf != REF_NULL) job[pos] = leaf;
+  __syncthreads();
+  if (threadIdx.x >= tot) return;  // no block barrier below
+  const ref_t c = job[threadIdx.x];
+  uint64_t m[2];
+  leaf_meta(P, c, m);
+  if (leaf_too_long(m[0])) {
+    raise_error(P.cnt, ERR_INTERNAL, 7);
+    return;
+  }
+  (void)hash_leaf_shaped(P, c, m[0], m[1], head[threadIdx.x]);
+}
+
 // The extension class of an inner level (kExtClass): each extension encodes
 // like a leaf (ext_direct_meta), so the leaf word builder hashes it with no
 // scratch message.  A node that is not such an extension (none is binned
@@ -1447,6 +1485,9 @@ __global__ __launch_bounds__(256) void k_hc_small(Pools P, const ref_t *__restri
                                                   const uint64_t *__restrict__ slot, uint32_t n,
                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
   if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
+  // latency-bound waves: ahead of a concurrent launch's permutations (the
+  // leaf level's, or a level's bulk) on a shared SIMD
+  __builtin_amdgcn_s_setprio(2);
   __shared__ uint64_t lds[256 / 64][kInnerWords];
   const uint32_t lane = __lane_id(), g = threadIdx.x / 64;
   const uint32_t i = blockIdx.x * (256 / 64) + g;
@@ -1569,6 +1610,7 @@ __global__ __launch_bounds__(256) void k_hc_small2(Pools P, const ref_t *__restr
                                                    const uint64_t *__restrict__ slot, uint32_t n,
                                                    uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
   if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
+  __builtin_amdgcn_s_setprio(2);  // as in k_hc_small
   __shared__ uint64_t lds[256 / 32][kInnerWords];
   const uint32_t lane = __lane_id(), half = lane >> 5, hl = lane & 31;
   const uint32_t g = 2 * (threadIdx.x / 64) + half;  // the node's index in the block
@@ -1862,6 +1904,12 @@ hipError_t hc_hash_leaves_direct(const Pools &P, const ref_t *list, const uint64
                                  uint32_t n_upper, uint32_t *enc_len, hipStream_t s) {
   if (n_upper == 0) return hipSuccess;
   k_hc_leaf_direct<<<nblk(n_upper), kB, 0, s>>>(P, list, lmeta, meta->hist, enc_len);
+  return hipGetLastError();
+}
+
+hipError_t hc_tail_leaves(const Pools &P, const ref_t *list, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_hc_tail_leaves<<<(unsigned)((16ull * n + 255) / 256), kB, 0, s>>>(P, list, n);
   return hipGetLastError();
 }
 
