@@ -80,7 +80,7 @@ __device__ __forceinline__ uint32_t pack8(uint64_t h) {
 // Hex-key staging: the block's keys are read from LDS when their bytes span
 // at most kKeyStage bytes (keys stored in batch order), so the block's HBM
 // reads are coalesced 16-byte loads instead of one strided stream per lane.
-constexpr uint32_t kKeyStage = 24576;
+constexpr uint32_t kKeyStage = 17408;
 
 struct KeyStager {
   uintptr_t base = 0;  // device address of lds[0] (16-byte aligned), 0 when not staged
