@@ -700,7 +700,10 @@ int Forest::hash(void *targets, int n_targets) {
       // the leaves of level 0 (its bins): launched before the bins reach the
       // host, so the GPU hashes them while the host plans the other levels
       if (ev2_) HIP_TRY(hipEventRecord(ev2_, stream_));
-      HIP_TRY(hc_hash_leaves_direct(P, sorted, lmeta, (const HcMeta *)hc_meta_.p, n, encl, stream_));
+      // the launch's bound on the leaves: the last build's branch and
+      // extension entries, when they end the list, hold none
+      const uint32_t n_upper = rec_.bdesc && rec_.blo + 2ull * rec_.bn == n ? n - 2 * rec_.bn : n;
+      HIP_TRY(hc_hash_leaves_direct(P, sorted, lmeta, (const HcMeta *)hc_meta_.p, n_upper, encl, stream_));
       if (ev3_) HIP_TRY(hipEventRecord(ev3_, stream_));
     }
     if (ev_meta_)
