@@ -1303,8 +1303,9 @@ __global__ __launch_bounds__(256) void k_hc_leaf_direct(Pools P, const ref_t *__
 // one thread per (node, slot): hashed here, on a second stream, so that
 // those levels can be hashed beside the leaf-level launch, which hashes the
 // same leaves again and stores the same bytes (a reader sees equal old and
-// new values).  Only leaves the word builder takes reach here (the host
-// checks that no leaf needs the generic encoder).
+// new values).  A child the word builder cannot take is a clean leaf, hashed
+// by an earlier pass (the host runs no tail while such a leaf is dirty), and
+// is left as it is.
 __global__ __launch_bounds__(256) void k_hc_tail_leaves(Pools P, const ref_t *__restrict__ list, uint32_t n) {
   if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   __builtin_amdgcn_s_setprio(2);  // ahead of the leaf-level launch's waves on a shared SIMD
@@ -1330,10 +1331,7 @@ __global__ __launch_bounds__(256) void k_hc_tail_leaves(Pools P, const ref_t *__
   const ref_t c = job[threadIdx.x];
   uint64_t m[2];
   leaf_meta(P, c, m);
-  if (leaf_too_long(m[0])) {
-    raise_error(P.cnt, ERR_INTERNAL, 7);
-    return;
-  }
+  if (leaf_too_long(m[0])) return;  // clean: its hash stands
   (void)hash_leaf_shaped(P, c, m[0], m[1], head[threadIdx.x]);
 }
 

@@ -872,6 +872,28 @@ def test_index_pairs_sharing_one_key(gmpt, workloads, oracle, mode, where):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_tail_beside_a_clean_long_leaf(gmpt, workloads, oracle, mode):
+    """The deepest small levels are hashed on a second stream after their
+    leaf children; a clean leaf child too long for the leaf word builder
+    (a 20,000-byte value from an earlier batch, beside a key inserted now)
+    keeps the hash its own pass gave it, and the root equals the oracle's."""
+    rng = np.random.default_rng(81)
+    def key(prefix):
+        return bytes(prefix) + bytes(rng.integers(0, 256, 32 - len(prefix), dtype=np.uint8))
+    k1, k2, k3 = key([0x00, 0x00]), key([0x00, 0x01]), key([0x00, 0x02])
+    others = [key([int(b)]) for b in rng.integers(0x10, 0x100, 40)]
+    big = bytes(rng.integers(0, 256, 20_000, dtype=np.uint8))
+    first = workloads.batch_from_bytes([k1, k2] + others, [big, b"v2"] + [b"w%d" % i for i in range(len(others))])
+    second = workloads.batch_from_bytes([k3], [b"v3"])
+    t = gmpt.MPT(device=0)
+    _put(t, first, mode)
+    assert t.get_root_hash() == oracle.root_of(first)
+    _put(t, second, mode)
+    assert t.get_root_hash() == oracle.root_of(workloads.concat_batches([first, second]))
+    t.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_root_reads_between_other_calls(gmpt, workloads, oracle, mode):
     """A hash pass brings trie 0's root back with its closing counter read;
     every other call drops that copy, so a root read always shows the tries'
