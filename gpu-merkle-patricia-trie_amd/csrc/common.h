@@ -80,6 +80,8 @@ struct TrieHdr {
   uint64_t lock;         // OLC version word guarding `root`
   uint8_t root_hash[32];
 };
+// k_hc_roots writes root_hash as 8-byte words and Counters::root0 as 4-byte words
+static_assert(offsetof(TrieHdr, root_hash) == 16 && sizeof(TrieHdr) % 8 == 0, "TrieHdr layout");
 
 // Device-side counters, read back by the host after each operation.
 struct Counters {
@@ -99,6 +101,7 @@ struct Counters {
   uint32_t root0_size;           // trie 0's root as the last hash pass left it (k_hc_roots), so that
   uint8_t root0[32];             // the pass's closing counter read also brings the root to the host
 };
+static_assert(offsetof(Counters, root0) % 4 == 0, "Counters layout");
 
 enum : uint32_t {
   ERR_NONE = 0,

@@ -156,7 +156,9 @@ class Forest {
   void drop_graph();
   bool is_pool(const DevBuf &b) const;
   void release(DevBuf &b);
-  int sync_counters();
+  // Reads the counters back (raising a kernel's error); with mark, the host
+  // waits for that event, recorded after the read, instead of the stream.
+  int sync_counters(hipEvent_t mark = nullptr);
   bool grow_key_arena_after_error();
   // With defer, a host batch large enough for the pinned stager leaves its
   // values out and describes them in *defer (bytes 0 when nothing is left):
@@ -196,9 +198,10 @@ class Forest {
   // the host link, so a forest can hold more than the GPU's memory
   bool pool_host_ = false;
   LeafRecords rec_;              // bins the last build recorded for its new leaves (in ws_)
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around all hash kernels
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // around a hash pass (after the bins' copy; after the counters' read)
   hipEvent_t ev2_ = nullptr, ev3_ = nullptr;  // around the leaf level's permutation kernel
   hipEvent_t ev_meta_ = nullptr;               // the classification bins have reached the host
+  hipEvent_t meta_mark_ = nullptr;             // the event hash() recorded after the bins' copy (ev0_ or ev_meta_)
   hipEvent_t tail_join_ = nullptr;             // the second stream has hashed the pass's deepest levels
   SideStream side_{};                          // second stream for work beside a hash level's main kernel
   // the captured rebuild pass and the batch it was captured for

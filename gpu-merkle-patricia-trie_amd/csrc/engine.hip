@@ -270,9 +270,16 @@ int Forest::ensure_pools(uint64_t nb, uint64_t ns, uint64_t nv, uint64_t key_byt
   return 0;
 }
 
-int Forest::sync_counters() {
+int Forest::sync_counters(hipEvent_t mark) {
   HIP_TRY(hipMemcpyAsync(h_cnt_, d_cnt_, sizeof(Counters), hipMemcpyDeviceToHost, stream_));
-  HIP_TRY(hipStreamSynchronize(stream_));
+  if (mark) {
+    // the host waits for the caller's event, recorded after the copy: one
+    // packet instead of a synchronisation marker beside the event's own
+    HIP_TRY(hipEventRecord(mark, stream_));
+    HIP_TRY(hipEventSynchronize(mark));
+  } else {
+    HIP_TRY(hipStreamSynchronize(stream_));
+  }
   cnt_ = *h_cnt_;
   if (cnt_.err != ERR_NONE) {
     uint32_t code = cnt_.err, arg = cnt_.err_arg;
@@ -673,6 +680,7 @@ int Forest::hash(void *targets, int n_targets) {
   stats_.ext_direct_nodes = 0;
   stats_.last_hash_ms = 0;
   unsigned long long *parts = nullptr;  // the pass's work counts (hc_count), folded by hc_roots
+  bool leaf_from_ev0 = false;           // the leaf launch's start event is ev0_
   if (n > 0) {
     Pools P = pools();
     if ((r = ensure_ws(hc_sorted_, sizeof(ref_t) * (size_t)n))) return r;
@@ -694,34 +702,37 @@ int Forest::hash(void *targets, int n_targets) {
     HIP_TRY(hc_classify(P, (const ref_t *)dirty_.p, n, (HcMeta *)hc_meta_.p, (uint32_t *)hc_cursor_.p, sorted, slot,
                         encl, ws2_.p, stream_, lmeta_tmp, lmeta, rec_));
     HIP_TRY(hipMemcpyAsync(h_meta_, hc_meta_.p, sizeof(HcMeta), hipMemcpyDeviceToHost, stream_));
-    if (ev_meta_) HIP_TRY(hipEventRecord(ev_meta_, stream_));
-    if (ev0_) HIP_TRY(hipEventRecord(ev0_, stream_));
+    // One event marks the bins' copy (the host waits for it), the start of the
+    // hash timing and, with the direct leaf kernel, the start of that launch:
+    // each record is a packet the GPU processes between two kernels.
+    meta_mark_ = ev0_ ? ev0_ : ev_meta_;
+    if (meta_mark_) HIP_TRY(hipEventRecord(meta_mark_, stream_));
     if (lmeta) {
       // the leaves of level 0 (its bins): launched before the bins reach the
       // host, so the GPU hashes them while the host plans the other levels
-      if (ev2_) HIP_TRY(hipEventRecord(ev2_, stream_));
+      leaf_from_ev0 = true;
       // the launch's bound on the leaves: the last build's branch and
       // extension entries, when they end the list, hold none
       const uint32_t n_upper = rec_.bdesc && rec_.blo + 2ull * rec_.bn == n ? n - 2 * rec_.bn : n;
       HIP_TRY(hc_hash_leaves_direct(P, sorted, lmeta, (const HcMeta *)hc_meta_.p, n_upper, encl, stream_));
       if (ev3_) HIP_TRY(hipEventRecord(ev3_, stream_));
     }
-    if (ev_meta_)
-      HIP_TRY(hipEventSynchronize(ev_meta_));
+    if (meta_mark_)
+      HIP_TRY(hipEventSynchronize(meta_mark_));
     else
       HIP_TRY(hipStreamSynchronize(stream_));
     const HcMeta &M = *(const HcMeta *)h_meta_;
     if ((r = ensure_ws(hc_scratch_, M.scratch_bytes + 64))) return r;
     uint32_t n_leaf = 0;
     if ((r = hash_levels(P, M.hist, lmeta != nullptr, false, &n_leaf))) return r;
-    if (ev1_) HIP_TRY(hipEventRecord(ev1_, stream_));
     parts = (unsigned long long *)((uint8_t *)hc_meta_.p + sizeof(HcMeta));
     HIP_TRY(hc_count(encl, (const HcMeta *)hc_meta_.p, n, n_leaf, parts, stream_));
     stats_.leaf_nodes = n_leaf;
   }
   HIP_TRY(hc_roots(pools(), ntries_, parts, stream_));  // also empties the dirty list
   rec_ = LeafRecords();
-  if ((r = sync_counters())) return r;
+  // ev1_ closes the hash timing after the roots and the counters' read
+  if ((r = sync_counters(n > 0 && ev0_ ? ev1_ : nullptr))) return r;
   // the counters are zeroed by the classification, so only a pass that ran sets them
   stats_.perms = n ? cnt_.n_perms : 0;
   stats_.enc_bytes = n ? cnt_.enc_bytes : 0;
@@ -733,7 +744,7 @@ int Forest::hash(void *targets, int n_targets) {
     HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.last_hash_ms = ms;
     if (stats_.leaf_nodes) {
-      HIP_TRY(hipEventElapsedTime(&ms, ev2_, ev3_));
+      HIP_TRY(hipEventElapsedTime(&ms, leaf_from_ev0 ? ev0_ : ev2_, ev3_));
       stats_.leaf_perm_ms = ms;
     }
   }
@@ -766,7 +777,7 @@ int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves
   // take (level 1) are hashed after the leaf launch, so a pass with any
   // hashes no tail.
   uint32_t tail_end = 2, tail_n = 0;
-  if (direct_leaves && n_direct && !captured && side_.stream && ev_meta_ && tail_join_ && level_count(1) == 0) {
+  if (direct_leaves && n_direct && !captured && side_.stream && meta_mark_ && tail_join_ && level_count(1) == 0) {
     for (uint32_t lv = 2; lv < kHcLevels; lv++) {
       const uint32_t cnt = level_count(lv);
       if (cnt > kTailLevelMax || tail_n + cnt > kTailMax) break;
@@ -782,7 +793,7 @@ int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves
     const bool in_tail = tail && lv >= 2 && lv < tail_end;
     if (tail && lv == 2) {
       // after the classification (its bins' copy), beside the leaf launch
-      HIP_TRY(hipStreamWaitEvent(side_.stream, ev_meta_, 0));
+      HIP_TRY(hipStreamWaitEvent(side_.stream, meta_mark_, 0));
       HIP_TRY(hc_tail_leaves(P, sorted + start, tail_n, side_.stream));
     }
     if (cnt && !in_tail && !joined && lv >= 2) {
@@ -798,8 +809,7 @@ int Forest::hash_levels(const Pools &P, const uint32_t *hist, bool direct_leaves
         HIP_TRY(hc_hash_range(P, sorted + start + skip, slot + start + skip, cnt - skip, scratch,
                               encl + start + skip, leaves, hc_spread_max_, in_tail ? side_.stream : stream_,
                               timed ? ev2_ : nullptr, timed ? ev3_ : nullptr, n_ext,
-                              hc_fuse_inner_ ? &hist[lv * kHcClasses] : nullptr,
-                              side_.stream && !captured && !in_tail ? &side_ : nullptr));
+                              hc_fuse_inner_ ? &hist[lv * kHcClasses] : nullptr));
       stats_.hash_launches++;
       stats_.hashed_nodes += cnt;
       if (!leaves && n_ext >= kExtDirectMin && cnt > hc_spread_max_) stats_.ext_direct_nodes += n_ext;

@@ -65,13 +65,13 @@ struct SideStream {
 // spread_max of 1- and 2-block branches and extensions (all but a sixteenth
 // of its nodes) is encoded and hashed with its messages in LDS in one launch
 // (k_hc_inner_wave); other levels, or all without cls, by a 16-lane encoding
-// kernel, an HBM scratch message and a permutation kernel.  With side, such a
-// level's few nodes of 3 or more blocks are hashed on the side stream beside
-// the LDS kernel (the main stream waits for them before this returns).
+// kernel, an HBM scratch message and a permutation kernel.  The LDS kernel's
+// first blocks hash the level's few nodes taller than its rows, two per
+// block, beside the bulk.
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s,
                          hipEvent_t perm_begin = nullptr, hipEvent_t perm_end = nullptr, uint32_t n_ext = 0,
-                         const uint32_t *cls = nullptr, const SideStream *side = nullptr);
+                         const uint32_t *cls = nullptr);
 // Encodes and hashes the classification's level-0 entries (leaves with
 // metadata, see hc_classify; their count is the sum of meta->hist[0 .. 8))
 // in one kernel; the count is read on the device, n_upper (>= it) sizes the

@@ -890,16 +890,30 @@ __device__ __forceinline__ void or_child_item(uint32_t *row32, uint32_t off, con
 // take the generic encoder and their scratch slot.  Blocks run in reverse
 // order: the level's nodes are sorted by block count with the extensions
 // last, so the larger nodes and extensions start first and finish under the
-// bulk.  Entries [skip_lo, skip_hi) are hashed elsewhere (by kernels on a
-// second stream) and left alone here.
+// bulk.  Entries [skip_lo, skip_hi), the level's nodes taller than the rows
+// (classes above NB blocks, before the extensions), are hashed two per block
+// by the launch's first nx blocks in the spread layout (hash_pair_spread2):
+// dispatched first, they run beside the bulk instead of holding the launch
+// open after it.
+__device__ __forceinline__ void hash_pair_spread2(const Pools &P, const ref_t *__restrict__ list,
+                                                  const uint64_t *__restrict__ slot, uint32_t n,
+                                                  uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len,
+                                                  uint32_t pair, uint64_t (*lds)[kInnerWords]);
 template <int NB>
 __global__ __launch_bounds__(64) void k_hc_inner_wave(Pools P, const ref_t *__restrict__ list,
                                                       const uint64_t *__restrict__ slot, uint32_t n,
                                                       uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len,
-                                                      uint32_t skip_lo, uint32_t skip_hi) {
+                                                      uint32_t skip_lo, uint32_t skip_hi, uint32_t nx) {
   if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
   constexpr uint32_t RS = 17 * NB + 1;  // row stride in words (odd: lane-strided reads spread over the banks)
   __shared__ uint64_t rows[64 * RS];
+  static_assert(64 * RS >= 2 * kInnerWords, "a pair's messages fit the rows");
+  if (blockIdx.x < nx) {  // uniform over the block
+    __builtin_amdgcn_s_setprio(2);  // latency-bound waves beside the bulk's permutations
+    hash_pair_spread2(P, list + skip_lo, slot + skip_lo, skip_hi - skip_lo, scratch, enc_len + skip_lo, blockIdx.x,
+                      (uint64_t(*)[kInnerWords])rows);
+    return;
+  }
   constexpr bool kDense = NB >= 4;      // many children per node: (node, slot) pairs dealt out directly
   __shared__ ref_t cref[kDense ? 64 * 16 : 1];  // the lanes' branches' child references (dense dealing)
   __shared__ ref_t nref[64];            // the lanes' nodes
@@ -1599,25 +1613,23 @@ __device__ __forceinline__ void keccak_spread2(uint32_t &l, uint32_t &h, const S
   }
 }
 
-// Small inner levels of many nodes: two nodes per wave, one per half.
-// Lanes 0 .. 15 of a half encode its node (encode_inner_node), then both
-// halves run the sponge together, each over its own message (the wave runs
-// as many blocks as its taller node; a half keeps its digest after its own
-// last block).
-__global__ __launch_bounds__(256) void k_hc_small2(Pools P, const ref_t *__restrict__ list,
-                                                   const uint64_t *__restrict__ slot, uint32_t n,
-                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
-  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
-  __builtin_amdgcn_s_setprio(2);  // as in k_hc_small
-  __shared__ uint64_t lds[256 / 32][kInnerWords];
+// Two nodes in one wave, one per half: nodes 2 * pair and 2 * pair + 1 of
+// list[0 .. n).  Lanes 0 .. 15 of a half encode its node (encode_inner_node,
+// the message in lds[half] when it fits), then both halves run the sponge
+// together, each over its own message (the wave runs as many blocks as its
+// taller node; a half keeps its digest after its own last block).  Every
+// thread of the block calls it (it holds a barrier).
+__device__ __forceinline__ void hash_pair_spread2(const Pools &P, const ref_t *__restrict__ list,
+                                                  const uint64_t *__restrict__ slot, uint32_t n,
+                                                  uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len,
+                                                  uint32_t pair, uint64_t (*lds)[kInnerWords]) {
   const uint32_t lane = __lane_id(), half = lane >> 5, hl = lane & 31;
-  const uint32_t g = 2 * (threadIdx.x / 64) + half;  // the node's index in the block
-  const uint32_t i = blockIdx.x * (256 / 32) + g;
+  const uint32_t i = 2 * pair + half;
   const bool active = i < n;
   const ref_t r = active ? list[i] : REF_NULL;
   uint64_t *out = active ? scratch + slot[i] / 8 : scratch;
   bool in_lds = false;
-  uint32_t enc = encode_inner_node(P, r, active, hl < kInnerGroup, hl & (kInnerGroup - 1), lds[g], out, &in_lds);
+  uint32_t enc = encode_inner_node(P, r, active, hl < kInnerGroup, hl & (kInnerGroup - 1), lds[half], out, &in_lds);
   enc = (uint32_t)__shfl((int)enc, 32 * half, 64);
   in_lds = __shfl((int)in_lds, 32 * half, 64) != 0;
   if (active && hl == 0) enc_len[i] = enc;
@@ -1626,7 +1638,7 @@ __global__ __launch_bounds__(256) void k_hc_small2(Pools P, const ref_t *__restr
   const uint32_t nbmax = max((uint32_t)__shfl((int)nb, 0, 64), (uint32_t)__shfl((int)nb, 32, 64));
   const SpreadLane2 c(lane);
   const int w = SpreadLane2::word_of(lane);
-  const uint64_t *m = in_lds ? lds[g] : out;
+  const uint64_t *m = in_lds ? lds[half] : out;
   uint32_t l = 0, h = 0, dl = 0, dh = 0;
   for (uint32_t b = 0; b < nbmax; b++) {
     if (w >= 0 && b < nb) {
@@ -1646,18 +1658,41 @@ __global__ __launch_bounds__(256) void k_hc_small2(Pools P, const ref_t *__restr
   if (hl == 0) node_done(P, r, 32);
 }
 
+// Small inner levels of many nodes: two nodes per wave (hash_pair_spread2).
+__global__ __launch_bounds__(256) void k_hc_small2(Pools P, const ref_t *__restrict__ list,
+                                                   const uint64_t *__restrict__ slot, uint32_t n,
+                                                   uint64_t *__restrict__ scratch, uint32_t *__restrict__ enc_len) {
+  if (P.cnt->graph_bad) return;  // a replayed pass whose batch is unlike the captured one
+  __builtin_amdgcn_s_setprio(2);  // as in k_hc_small
+  __shared__ uint64_t lds[256 / 32][kInnerWords];
+  const uint32_t wave = threadIdx.x / 64;
+  hash_pair_spread2(P, list, slot, n, scratch, enc_len, 4 * blockIdx.x + wave, lds + 2 * wave);
+}
+
 __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries, const unsigned long long *parts) {
   uint32_t t = grid_tid();
+  // the root's hash first: its two dependent loads overlap the counts' reduction
+  uint32_t hs = 0;
+  uint4 h[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+  if (t < ntries) {
+    const ref_t r = P.tries[t].root;
+    if (ref_type(r) == T_SHORT || ref_type(r) == T_BRANCH) {
+      hs = node_hash_size(P, r);
+      const uint4 *src = (const uint4 *)node_hash(P, r);  // offset 0 of a 64- or 128-byte node
+      h[0] = src[0];
+      h[1] = src[1];
+    }
+  }
   if (t == 0) P.cnt->n_dirty = 0;  // every listed node has been hashed
   if (parts && blockIdx.x == 0) {  // the pass's work counts (uniform branch)
     static_assert(kHcCountParts == 256, "one partial per thread");
     __shared__ unsigned long long sh[4][4];
     unsigned long long v[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      v[q] = parts[4 * threadIdx.x + q];
+    for (int q = 0; q < 4; q++) v[q] = parts[4 * threadIdx.x + q];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
       for (int off = 32; off > 0; off >>= 1) v[q] += (unsigned long long)__shfl_xor((long long)v[q], off);
-    }
     if (__lane_id() == 0)
 #pragma unroll
       for (int q = 0; q < 4; q++) sh[threadIdx.x >> 6][q] = v[q];
@@ -1671,21 +1706,24 @@ __global__ __launch_bounds__(256) void k_hc_roots(Pools P, uint32_t ntries, cons
     }
   }
   if (t >= ntries) return;
+  // bytes past the hash size (a root encoding under 32 bytes) read as zero
+  uint32_t w[8] = {h[0].x, h[0].y, h[0].z, h[0].w, h[1].x, h[1].y, h[1].z, h[1].w};
+#pragma unroll
+  for (uint32_t q = 0; q < 8; q++) {
+    const uint32_t keep = hs >= 4 * q + 4 ? 4u : hs > 4 * q ? hs - 4 * q : 0u;
+    w[q] = keep == 4 ? w[q] : keep ? w[q] & ((1u << (8 * keep)) - 1) : 0u;
+  }
   TrieHdr &T = P.tries[t];
-  ref_t r = T.root;
-  uint32_t hs = 0;
-  const uint8_t *h = nullptr;
-  if (ref_type(r) == T_SHORT || ref_type(r) == T_BRANCH) {
-    hs = node_hash_size(P, r);
-    h = node_hash(P, r);
-  }
-  for (int q = 0; q < 32; q++) {
-    const uint8_t b = q < (int)hs ? h[q] : 0;
-    T.root_hash[q] = b;
-    if (t == 0) P.cnt->root0[q] = b;
-  }
+  uint64_t *dst = (uint64_t *)T.root_hash;  // offset 16 of the header
+#pragma unroll
+  for (uint32_t q = 0; q < 4; q++) dst[q] = ((uint64_t)w[2 * q + 1] << 32) | w[2 * q];
   T.root_hash_size = hs;
-  if (t == 0) P.cnt->root0_size = hs;
+  if (t == 0) {
+    uint32_t *d0 = (uint32_t *)P.cnt->root0;
+#pragma unroll
+    for (uint32_t q = 0; q < 8; q++) d0[q] = w[q];
+    P.cnt->root0_size = hs;
+  }
 }
 
 // ---------------------------------------------- standalone Keccak (test hook) --
@@ -1815,7 +1853,7 @@ hipError_t hc_classify(const Pools &P, const ref_t *list, uint32_t n, HcMeta *me
 
 hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot, uint32_t n, uint64_t *scratch,
                          uint32_t *enc_len, bool leaves, uint32_t spread_max, hipStream_t s, hipEvent_t perm_begin,
-                         hipEvent_t perm_end, uint32_t n_ext, const uint32_t *cls, const SideStream *side) {
+                         hipEvent_t perm_end, uint32_t n_ext, const uint32_t *cls) {
   if (n == 0) return hipSuccess;
   if (!leaves && n <= spread_max) {  // small inner level: encode and hash in one launch
     if (perm_begin) {
@@ -1846,28 +1884,17 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
     // the launch open long after the bulk: with a second stream they are
     // encoded and hashed there, concurrently with the bulk
     const uint32_t cov = std::min(n, rows_nb == 4 ? c01 + c23 : rows_nb == 2 ? c01 : cls[0]);
-    uint32_t lo = 0, hi = 0;
     const uint32_t multi = n - std::min(n, cov + cext);
-    if (side && multi) {
-      lo = cov;
-      hi = lo + multi;
-      hipError_t e;
-      if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
-      // two nodes per wave in the spread layout: latency-bound waves that
-      // share the SIMDs with the bulk's permutations
-      k_hc_small2<<<(multi + 7) / 8, kB, 0, side->stream>>>(P, list + lo, slot + lo, multi, scratch, enc_len + lo);
-      GMPT_LAUNCH_CHECK();
-      if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
-    }
+    const uint32_t lo = multi ? cov : 0u, hi = lo + multi;
+    const uint32_t nx = (multi + 1) / 2;  // the launch's first blocks: two of them per block
+    const unsigned grid = (unsigned)((n + 63) / 64 + nx);
     if (rows_nb == 4)
-      k_hc_inner_wave<4><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
+      k_hc_inner_wave<4><<<grid, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi, nx);
     else if (rows_nb == 2)
-      k_hc_inner_wave<2><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
+      k_hc_inner_wave<2><<<grid, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi, nx);
     else
-      k_hc_inner_wave<1><<<(n + 63) / 64, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi);
+      k_hc_inner_wave<1><<<grid, 64, 0, s>>>(P, list, slot, n, scratch, enc_len, lo, hi, nx);
     GMPT_LAUNCH_CHECK();
-    if (side && multi) return hipStreamWaitEvent(s, side->join, 0);
     return hipSuccess;
   }
   // the level's extension class, sorted last: its own launch when it is large

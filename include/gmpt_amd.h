@@ -49,7 +49,7 @@ typedef struct gmpt_batch {
 } gmpt_batch;
 
 typedef struct gmpt_stats {
-  double last_hash_ms;            /* device time of the last hash's Keccak kernels */
+  double last_hash_ms;            /* device time of the last hash pass, from its bins to its counters read */
   uint32_t hash_launches;         /* hash kernel launches (levels) of the last hash */
   uint32_t hashed_nodes;          /* nodes re-hashed by the last hash */
   uint32_t restarts;              /* LockNU optimistic restarts in the last insert */
