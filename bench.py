@@ -44,7 +44,7 @@ PERM_PEAK_G = VALU_PEAK_TOPS * 1e12 / INSTR_PER_PERM / 1e9
 # workload: rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per dispatch,
 # the factors being what the same counters read for the 1 GiB k_copy16
 # calibration copy (--config hbm-copy).  Keyed by "leaves fused".
-LEAF_TRAFFIC_BYTES = {False: None, True: 586_512_920}
+LEAF_TRAFFIC_BYTES = {False: None, True: 586_695_130}
 
 
 def _import_package():
