@@ -1881,8 +1881,8 @@ hipError_t hc_hash_range(const Pools &P, const ref_t *list, const uint64_t *slot
   if (rows_nb) {
     // the few nodes taller than the rows (classes rows_nb .. 6, between the
     // row classes and the extensions) would take the generic encoder and hold
-    // the launch open long after the bulk: with a second stream they are
-    // encoded and hashed there, concurrently with the bulk
+    // the launch open long after the bulk: the launch's first blocks, which
+    // are dispatched first, encode and hash them concurrently with the bulk
     const uint32_t cov = std::min(n, rows_nb == 4 ? c01 + c23 : rows_nb == 2 ? c01 : cls[0]);
     const uint32_t multi = n - std::min(n, cov + cext);
     const uint32_t lo = multi ? cov : 0u, hi = lo + multi;
